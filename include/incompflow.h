@@ -1,0 +1,196 @@
+/*
+ * incompflow.h - C-ABI of the MI355X steady SIMPLE solver (collocated structured grids).
+ *
+ * This is the drop-in boundary for the reference's `Solver` class and its compile-time
+ * `config.h` contract.  Every entry point takes plain pointers and sizes, returns an int
+ * status (IFL_OK == 0) and leaves a thread-local message for ifl_last_error().  No C++
+ * or torch type crosses it.  A handle owns all of its device memory and one HIP stream;
+ * buffers passed in are copied, never retained.  Handles are independent: one handle per
+ * host thread, no device-wide synchronisation.
+ *
+ * Reference surface replaced (IncompFlow-CUDA):
+ *   include/config.h:6-81     compile-time case     -> ifl_config (runtime, same fields)
+ *   include/constants.h:7-36  face/slot/BC codes    -> IFL_EAST.., IFL_SLOT_*, IFL_WALL..
+ *   include/Solver.h:46       Solver(u,v,w,p,temp)  -> ifl_create (ifl_config.init[])
+ *   include/Solver.h:48       ~Solver()             -> ifl_destroy
+ *   include/Solver.h:50       solve()               -> ifl_solve
+ *   include/Solver.h:52       writeVTK(path)        -> ifl_write_vtk
+ *   include/Solver.h:54-71    private stage methods -> ifl_run_stage (tests only)
+ */
+#ifndef INCOMPFLOW_H
+#define INCOMPFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFL_ABI_VERSION 1
+
+/* ---- status codes ---- */
+enum {
+    IFL_OK = 0,
+    IFL_ERR_ARG = 1,        /* invalid argument / config */
+    IFL_ERR_DEVICE = 2,     /* HIP runtime error */
+    IFL_ERR_DIVERGED = 3,   /* non-finite residual detected on the device */
+    IFL_ERR_NOT_CONVERGED = 4, /* stop rule not met within num_outer_iter (only when requested) */
+    IFL_ERR_IO = 5,
+    IFL_ERR_NOMEM = 6
+};
+
+/* ---- boundary faces, include/constants.h:7-12 ---- */
+enum { IFL_EAST = 0, IFL_WEST = 1, IFL_NORTH = 2, IFL_SOUTH = 3, IFL_TOP = 4, IFL_BOTTOM = 5 };
+
+/* ---- coefficient slots, include/constants.h:14-20 (plane a of a coefficient array) ---- */
+enum { IFL_SLOT_C = 0, IFL_SLOT_E = 1, IFL_SLOT_W = 2, IFL_SLOT_N = 3, IFL_SLOT_S = 4,
+       IFL_SLOT_T = 5, IFL_SLOT_B = 6 };
+
+/* ---- velocity BC codes, include/constants.h:34-36 ---- */
+enum { IFL_WALL = 0, IFL_INLET = 1, IFL_OUTLET = 2 };
+
+/* ---- smoother (linear solver) ---- */
+enum {
+    IFL_SMOOTHER_RBGS = 0,   /* red-black Gauss-Seidel (deterministic form of GaussSeidelIterate) */
+    IFL_SMOOTHER_JACOBI = 1, /* point Jacobi, pointJacobiIterate with the result copied back */
+    IFL_SMOOTHER_LEX = 2     /* lexicographic Gauss-Seidel: CPU oracle only */
+};
+
+/* ---- outer stop rule ---- */
+enum {
+    IFL_STOP_NONE = 0,      /* run exactly num_outer_iter outer iterations */
+    IFL_STOP_FAITHFUL = 1,  /* Solver.cu:237-268 as written (all maxima tracked from pNorm) */
+    IFL_STOP_FIXED = 2,     /* same test, each variable normalised by its own running max */
+    IFL_STOP_SCALED = 3     /* scaled FP64 residual < tol_residual (see ifl_iter_record) */
+};
+
+/* ---- field identifiers for ifl_get_field / ifl_set_field ---- */
+enum {
+    IFL_FIELD_U = 0, IFL_FIELD_V = 1, IFL_FIELD_W = 2, IFL_FIELD_P = 3, IFL_FIELD_TEMP = 4,
+    IFL_FIELD_PCORR = 5,
+    IFL_FIELD_UF = 6,   /* (nx+1)*ny*nz, index i + (nx+1)(j + ny k) */
+    IFL_FIELD_VF = 7,   /* nx*(ny+1)*nz, index i + nx(j + (ny+1) k) */
+    IFL_FIELD_WF = 8,   /* nx*ny*(nz+1), index i + nx(j + ny k), k in [0,nz] */
+    IFL_FIELD_MCOEF = 9,  /* momentum matrix, (1+2dim) planes of N (one set for u,v,w) */
+    IFL_FIELD_PCOEF = 10, /* pressure-correction matrix, (1+2dim) planes of N */
+    IFL_FIELD_BU = 11, IFL_FIELD_BV = 12, IFL_FIELD_BW = 13,
+    IFL_FIELD_BP = 14,    /* mass imbalance b' */
+    IFL_FIELD_COUNT = 15
+};
+
+/* ---- stages for ifl_run_stage: one SIMPLE outer iteration is 1..9 in order ---- */
+enum {
+    IFL_STAGE_INIT_FACES = 0,   /* Solver.cu:369-408  initFaceVel (first iteration only) */
+    IFL_STAGE_FACE_BC = 1,      /* Solver.cu:410-465  applyBCsToFaceVel */
+    IFL_STAGE_MOM_ASSEMBLE = 2, /* Solver.cu:167-175  coefficients + sources + BC rows */
+    IFL_STAGE_MOM_RESIDUAL = 3, /* FP64 ||b - A phi|| per momentum component (no reference) */
+    IFL_STAGE_SMOOTH_U = 4,     /* Solver.cu:178 */
+    IFL_STAGE_SMOOTH_V = 5,     /* Solver.cu:180 */
+    IFL_STAGE_SMOOTH_W = 6,     /* Solver.cu:183 */
+    IFL_STAGE_RHIE_CHOW = 7,    /* Solver.cu:186-187  RhieChowInterpolate + applyBCsToFaceVel */
+    IFL_STAGE_PCORR_ASSEMBLE = 8, /* Solver.cu:189-193 coefficients, b', massNorm, p'=0 */
+    IFL_STAGE_SMOOTH_P = 9,     /* Solver.cu:196 (+ FP64-defect refinement cycles) */
+    IFL_STAGE_UPDATE = 10,      /* Solver.cu:198      updateField */
+    IFL_STAGE_OUTER = 11,       /* one whole outer iteration incl. the stop test */
+    IFL_STAGE_COUNT = 12
+};
+
+/* Runtime case description; field-for-field the reference's config.h (line cited). */
+typedef struct ifl_config {
+    int32_t dim;                 /* config.h:6   2 or 3 */
+    int32_t nx, ny, nz;          /* config.h:8-10 (nz = 1 in 2D) */
+    double xmin, xmax;           /* config.h:12-13 */
+    double ymin, ymax;           /* config.h:14-15 */
+    double zmin, zmax;           /* config.h:16-17 */
+    double density;              /* config.h:21 */
+    double viscosity;            /* config.h:22 dynamicViscosity */
+    int32_t bc_type[6];          /* config.h:26-33 velBCs::type[face] */
+    double bc_val[6][3];         /* config.h:35-42 velBCs::val[face][u,v,w] */
+    int32_t num_outer_iter;      /* config.h:65 */
+    int32_t n_iter[4];           /* config.h:67-70 nIter_u, nIter_v, nIter_w, nIter_p */
+    double relax[4];             /* config.h:72-75 relax_u, relax_v, relax_w, relax_p */
+    double tol[4];               /* config.h:77-80 tol_u, tol_v, tol_w, tol_p */
+    double tol_mass;             /* config.h:81 */
+    double init[5];              /* Solver.h:46 uniform initial u, v, w, p, temp */
+    /* additions with no reference field */
+    int32_t smoother;            /* IFL_SMOOTHER_* (reference: Gauss-Seidel, Solver.cu:178) */
+    int32_t stop_rule;           /* IFL_STOP_* (reference: IFL_STOP_FAITHFUL) */
+    double tol_residual;         /* threshold for IFL_STOP_SCALED */
+    int32_t p_refine;            /* FP64-defect refinement cycles of the p' solve (0 = off) */
+    int32_t log_every;           /* print the Solver.cu:270-285 block every k iterations (0 = never) */
+    int32_t check_every;         /* outer iterations between host reads of the stop flag (0 = auto) */
+    int32_t reserved[7];
+} ifl_config;
+
+/* One record per outer iteration (device-written, FP64). */
+typedef struct ifl_iter_record {
+    double p_norm, u_norm, v_norm, w_norm;  /* RMS of applied increments, Solver.cu:226-235 */
+    double uf_norm, vf_norm, wf_norm;
+    double mass_norm;                       /* RMS of b' before correction */
+    double res_cont, res_u, res_v, res_w;   /* FP64 L2: ||b'||, ||b_u - A u|| ... before sweeps */
+    double scaled;                          /* max_x res_x / max(res_x over iterations 1..5) */
+    double rel_p, rel_u, rel_v, rel_w, rel_uf, rel_vf, rel_wf; /* stop-rule relative norms */
+    int32_t sweeps[4];                      /* smoother sweeps executed: u, v, w, p' (incl. refine) */
+    int32_t converged;                      /* stop rule met at this iteration */
+    int32_t pad;
+} ifl_iter_record;
+
+typedef struct ifl_report {
+    int32_t iterations;          /* outer iterations executed by this call */
+    int32_t converged;           /* stop rule met */
+    int32_t status;              /* IFL_OK or IFL_ERR_DIVERGED */
+    int32_t pad;
+    double elapsed_ms;           /* device time of the call (HIP events) */
+    double final_scaled;         /* last scaled residual */
+    double sweeps_total;         /* smoother cell-sweeps summed over equations and iterations */
+} ifl_report;
+
+typedef struct ifl_solver ifl_solver;
+
+/* Thread-local description of the last failure on this thread. */
+const char *ifl_last_error(void);
+int ifl_abi_version(void);
+
+/* Fill cfg with the reference's shipped defaults (include/config.h). */
+void ifl_config_default(ifl_config *cfg);
+
+/* Solver.h:46 / Solver.cu:13-104. Allocates all device state, uniform initial fields. */
+int ifl_create(const ifl_config *cfg, ifl_solver **out);
+/* Solver.h:48 / Solver.cu:106-151. */
+int ifl_destroy(ifl_solver *h);
+
+/* Per-face profile of the face-normal velocity (replaces velBCs::val[face][normal] for
+ * every boundary cell of that face; n = cells on the face: W/E ny*nz, N/S nx*nz, T/B nx*ny).
+ * vals == NULL restores the uniform value.  No reference counterpart (config.h:35-42 is uniform). */
+int ifl_set_face_profile(ifl_solver *h, int face, const double *vals, size_t n);
+
+/* Solver.h:50 / Solver.cu:153-299: run outer iterations until the stop rule holds or
+ * max_iter (<=0: cfg.num_outer_iter) iterations ran.  history (optional) receives one record
+ * per executed iteration, at most hist_cap. */
+int ifl_solve(ifl_solver *h, int max_iter, ifl_report *report,
+              ifl_iter_record *history, size_t hist_cap);
+
+/* Host copies of device state.  n must equal the field's element count. */
+int ifl_get_field(ifl_solver *h, int field, float *dst, size_t n);
+int ifl_set_field(ifl_solver *h, int field, const float *src, size_t n);
+size_t ifl_field_size(const ifl_solver *h, int field);
+
+/* Solver.h:52 / Solver.cu:301-367: legacy-VTK ASCII RECTILINEAR_GRID at path (as given;
+ * the reference prefixes "output/", which ifl::Solver in Solver.h reproduces). */
+int ifl_write_vtk(ifl_solver *h, const char *path);
+
+/* Run one stage on the handle's device state (parity tests).  out (optional, >= 8 doubles)
+ * receives stage-specific FP64 values:
+ *   SMOOTH_U/V/W/P : [0] sweeps executed, [1] sum of squared increments of the last sweep
+ *   MOM_RESIDUAL   : [0..2] ||b_u - A u||, ||b_v - A v||, ||b_w - A w|| (L2)
+ *   PCORR_ASSEMBLE : [0] sum of b'^2
+ *   UPDATE         : [0..6] sums of squared increments of p, u, v, w, uf, vf, wf
+ *   OUTER          : [0] converged flag, [1] scaled residual */
+int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INCOMPFLOW_H */

@@ -1,0 +1,552 @@
+// capi.hip - host side of the C-ABI in include/incompflow.h (C++20 + HIP runtime).
+//
+// A handle owns one device arena (all fields, coefficient planes, reduction partials, the
+// CaseState monitor and a ring of per-iteration records) and one HIP stream.  One SIMPLE outer
+// iteration (Solver.cu:160-290) is captured once into a hipGraph and replayed; the device-side
+// stop flag makes every kernel of an already finished case return at once, so the host reads the
+// monitor only every `check_every` iterations instead of after every sweep (Solver.cu:951-953)
+// and every iteration (Solver.cu:207-224).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/incompflow.h"
+#include "ifl_device.hpp"
+#include "ifl_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define IFL_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(IFL_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+constexpr int kHistCap = 4096;
+constexpr int kMaxSweeps = 100000;
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct ifl_solver {
+    ifl_config cfg{};
+    ifl::CaseParams P{};
+    hipStream_t stream = nullptr;
+    char *arena = nullptr;
+    float *temp = nullptr;
+    float *prof[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t graph = nullptr;
+    bool graphDirty = true;
+    long long iter = 0;   // host mirror of CaseState::iter
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float tolf[4];
+};
+
+namespace {
+
+int validate(const ifl_config &c) {
+    if (c.dim != 2 && c.dim != 3) return fail(IFL_ERR_ARG, "dim must be 2 or 3");
+    if (c.nx < 4 || c.ny < 4) return fail(IFL_ERR_ARG, "nx and ny must be >= 4");
+    if (c.dim == 2 && c.nz != 1) return fail(IFL_ERR_ARG, "nz must be 1 in 2D");
+    if (c.dim == 3 && c.nz < 4) return fail(IFL_ERR_ARG, "nz must be >= 4 in 3D");
+    if ((long long)c.nx * c.ny * c.nz > (1LL << 31) / 8) return fail(IFL_ERR_ARG, "grid too large");
+    if (!(c.xmax > c.xmin) || !(c.ymax > c.ymin) || !(c.zmax > c.zmin)) return fail(IFL_ERR_ARG, "empty domain");
+    for (int f = 0; f < 6; ++f)
+        if (c.bc_type[f] < 0 || c.bc_type[f] > 2) return fail(IFL_ERR_ARG, "bc_type must be 0 (wall), 1 (inlet) or 2 (outlet)");
+    for (int q = 0; q < 4; ++q)
+        if (c.n_iter[q] < 0 || c.n_iter[q] > kMaxSweeps) return fail(IFL_ERR_ARG, "n_iter out of range");
+    if (c.n_iter[3] < 1) return fail(IFL_ERR_ARG, "n_iter[p] must be >= 1");
+    if (c.smoother != IFL_SMOOTHER_RBGS && c.smoother != IFL_SMOOTHER_JACOBI)
+        return fail(IFL_ERR_ARG, "smoother must be IFL_SMOOTHER_RBGS or IFL_SMOOTHER_JACOBI on the device");
+    if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
+    if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
+    return IFL_OK;
+}
+
+void fillParams(ifl_solver *h) {
+    const ifl_config &c = h->cfg;
+    ifl::CaseParams &P = h->P;
+    P.dim = c.dim; P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
+    P.N = (long long)c.nx * c.ny * c.nz;
+    // constants.h:22-28 in float, exactly as the reference's constexprs
+    const float xmin = (float)c.xmin, xmax = (float)c.xmax, ymin = (float)c.ymin, ymax = (float)c.ymax;
+    const float zmin = (float)c.zmin, zmax = (float)c.zmax;
+    P.dx = (xmax - xmin) / (float)c.nx;
+    P.dy = (ymax - ymin) / (float)c.ny;
+    P.dz = (zmax - zmin) / (float)c.nz;
+    P.Ax = P.dy * P.dz;
+    P.Ay = P.dz * P.dx;
+    P.Az = P.dx * P.dy;
+    P.rho = (float)c.density;
+    P.mu = (float)c.viscosity;
+    for (int f = 0; f < 6; ++f) {
+        P.bct[f] = c.bc_type[f];
+        for (int q = 0; q < 3; ++q) P.bcv[f][q] = (float)c.bc_val[f][q];
+        P.prof[f] = h->prof[f];
+    }
+    for (int q = 0; q < 4; ++q) {
+        P.relax[q] = (float)c.relax[q];
+        h->tolf[q] = (float)c.tol[q];
+        P.tol[q] = (double)h->tolf[q];
+    }
+    P.tolMass = (double)(float)c.tol_mass;
+    P.tolRes = c.tol_residual;
+    P.stopRule = c.stop_rule;
+    P.histCap = kHistCap;
+}
+
+unsigned maxBlocks(const ifl::CaseParams &P) {
+    long long cells = (P.N + ifl::kBlock - 1) / ifl::kBlock;
+    long long hx = (P.nx + 1) / 2;
+    long long rb = (hx * P.ny * P.nz + ifl::kBlock - 1) / ifl::kBlock;
+    return (unsigned)std::max(cells, rb);
+}
+
+// Solver.cu:160-290 stage order, all on the handle's stream.
+void enqueueSmoother(ifl_solver *h, float *fld, const float *coef, const float *src, int q, int slot) {
+    const ifl::CaseParams &P = h->P;
+    for (int s = 0; s < h->cfg.n_iter[q]; ++s)
+        ifl::launch_sweep(P, h->cfg.smoother, fld, coef, src, P.relax[q], P.tol[q], slot, q, s, h->stream);
+}
+
+void enqueuePressureCorrection(ifl_solver *h) {
+    const ifl::CaseParams &P = h->P;
+    enqueueSmoother(h, P.pc, P.pcoef, P.bp, 3, 3);                     // Solver.cu:196
+    for (int r = 0; r < h->cfg.p_refine; ++r) {                       // FP64-defect refinement
+        ifl::launch_pcorr_defect(P, h->stream);
+        enqueueSmoother(h, P.tmp2, P.pcoef, P.tmp, 3, 4);
+        ifl::launch_pcorr_accumulate(P, h->stream);
+    }
+}
+
+void enqueueOuter(ifl_solver *h) {
+    const ifl::CaseParams &P = h->P;
+    ifl::launch_mom_assemble(P, h->stream);                           // :165-175
+    enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0);                     // :178
+    enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1);                     // :180
+    if (P.dim == 3) enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);     // :183
+    ifl::launch_rhie_chow(P, h->stream);                              // :186-187
+    ifl::launch_pcorr_assemble(P, h->stream);                         // :189-193
+    enqueuePressureCorrection(h);                                     // :196
+    ifl::launch_update(P, 1, h->stream);                              // :198-268
+}
+
+int buildGraph(ifl_solver *h) {
+    if (!h->graphDirty && h->graph) return IFL_OK;
+    if (h->graph) {
+        IFL_HIP(hipGraphExecDestroy(h->graph));
+        h->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    IFL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueueOuter(h);
+    IFL_HIP(hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(IFL_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    h->graphDirty = false;
+    return IFL_OK;
+}
+
+int readState(ifl_solver *h, ifl::CaseState &s) {
+    IFL_HIP(hipMemcpyAsync(&s, h->P.st, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    return IFL_OK;
+}
+
+int setDone(ifl_solver *h, int v) {
+    IFL_HIP(hipMemcpyAsync(&h->P.st->done, &v, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    return IFL_OK;
+}
+
+float *fieldPtr(ifl_solver *h, int id) {
+    ifl::CaseParams &P = h->P;
+    switch (id) {
+        case IFL_FIELD_U: return P.u;
+        case IFL_FIELD_V: return P.v;
+        case IFL_FIELD_W: return P.w;
+        case IFL_FIELD_P: return P.p;
+        case IFL_FIELD_TEMP: return h->temp;
+        case IFL_FIELD_PCORR: return P.pc;
+        case IFL_FIELD_UF: return P.uf;
+        case IFL_FIELD_VF: return P.vf;
+        case IFL_FIELD_WF: return P.wf;
+        case IFL_FIELD_MCOEF: return P.mcoef;
+        case IFL_FIELD_PCOEF: return P.pcoef;
+        case IFL_FIELD_BU: return P.bu;
+        case IFL_FIELD_BV: return P.bv;
+        case IFL_FIELD_BW: return P.bw;
+        case IFL_FIELD_BP: return P.bp;
+        default: return nullptr;
+    }
+}
+
+size_t fieldSize(const ifl_config &c, int id) {
+    size_t nx = c.nx, ny = c.ny, nz = c.nz, n = nx * ny * nz;
+    switch (id) {
+        case IFL_FIELD_UF: return (nx + 1) * ny * nz;
+        case IFL_FIELD_VF: return nx * (ny + 1) * nz;
+        case IFL_FIELD_WF: return nx * ny * (nz + 1);
+        case IFL_FIELD_MCOEF:
+        case IFL_FIELD_PCOEF: return n * (1 + 2 * c.dim);
+        default: return (id >= 0 && id < IFL_FIELD_COUNT) ? n : 0;
+    }
+}
+
+void printLog(const ifl_iter_record &r, long long it, int dim) {
+    // Solver.cu:270-285
+    std::printf("iter: %lld\n", it + 1);
+    std::printf("pRelNorm : %.3E\n", r.rel_p);
+    std::printf("uRelNorm : %.3E\n", r.rel_u);
+    std::printf("vRelNorm : %.3E\n", r.rel_v);
+    if (dim == 3) std::printf("wRelNorm : %.3E\n", r.rel_w);
+    std::printf("ufRelNorm: %.3E\n", r.rel_uf);
+    std::printf("vfRelNorm: %.3E\n", r.rel_vf);
+    if (dim == 3) std::printf("wfRelNorm: %.3E\n", r.rel_wf);
+    std::printf("massNorm : %.3E\n", r.mass_norm);
+    std::printf("------------------------------------------------------------\n");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ifl_last_error(void) { return g_err.c_str(); }
+int ifl_abi_version(void) { return IFL_ABI_VERSION; }
+
+void ifl_config_default(ifl_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    // include/config.h:6-81 as shipped
+    c->dim = 2;
+    c->nx = 1024; c->ny = 256; c->nz = 1;
+    c->xmin = -2; c->xmax = 2; c->ymin = -0.5; c->ymax = 0.5; c->zmin = 0; c->zmax = 1;
+    c->density = 50; c->viscosity = 1;
+    const int types[6] = {IFL_OUTLET, IFL_INLET, IFL_WALL, IFL_WALL, IFL_WALL, IFL_WALL};
+    for (int f = 0; f < 6; ++f) c->bc_type[f] = types[f];
+    c->bc_val[IFL_WEST][0] = 1;
+    c->num_outer_iter = 20000;
+    c->n_iter[0] = c->n_iter[1] = c->n_iter[2] = 10;
+    c->n_iter[3] = 20;
+    c->relax[0] = c->relax[1] = c->relax[2] = 0.75;
+    c->relax[3] = 0.25;
+    c->tol[0] = c->tol[1] = c->tol[2] = 1e-5;
+    c->tol[3] = 1e-3;
+    c->tol_mass = 1e-9;
+    c->init[4] = 273;   // Solver.h:46 default temp
+    c->smoother = IFL_SMOOTHER_RBGS;
+    c->stop_rule = IFL_STOP_FAITHFUL;
+    c->tol_residual = 1e-5;
+    c->log_every = 1000;
+}
+
+int ifl_create(const ifl_config *cfg, ifl_solver **out) {
+    if (!cfg || !out) return fail(IFL_ERR_ARG, "null argument");
+    *out = nullptr;
+    int st = validate(*cfg);
+    if (st != IFL_OK) return st;
+    ifl_solver *h = new (std::nothrow) ifl_solver();
+    if (!h) return fail(IFL_ERR_NOMEM, "out of host memory");
+    h->cfg = *cfg;
+    fillParams(h);
+    ifl::CaseParams &P = h->P;
+    const size_t N = (size_t)P.N, nx = P.nx, ny = P.ny, nz = P.nz;
+    const size_t ncoef = N * (1 + 2 * P.dim);
+    const unsigned nblk = maxBlocks(P);
+    // arena layout
+    struct Part { void **dst; size_t bytes; };
+    float *temp = nullptr;
+    std::vector<Part> parts = {
+        {(void **)&P.u, N * 4}, {(void **)&P.v, N * 4}, {(void **)&P.w, N * 4}, {(void **)&P.p, N * 4},
+        {(void **)&temp, N * 4}, {(void **)&P.pc, N * 4},
+        {(void **)&P.uf, (nx + 1) * ny * nz * 4}, {(void **)&P.vf, nx * (ny + 1) * nz * 4},
+        {(void **)&P.wf, nx * ny * (nz + 1) * 4},
+        {(void **)&P.mcoef, ncoef * 4}, {(void **)&P.pcoef, ncoef * 4},
+        {(void **)&P.bu, N * 4}, {(void **)&P.bv, N * 4}, {(void **)&P.bw, N * 4}, {(void **)&P.bp, N * 4},
+        {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4}, {(void **)&P.jac, N * 4},
+        {(void **)&P.partials, (size_t)nblk * ifl::kPart * 8},
+        {(void **)&P.st, sizeof(ifl::CaseState)},
+        {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
+    };
+    size_t total = 0;
+    for (auto &p : parts) total += align256(p.bytes);
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->arena, total);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e != hipSuccess) {
+        std::string m = std::string("device allocation: ") + hipGetErrorString(e);
+        ifl_destroy(h);
+        return fail(IFL_ERR_DEVICE, m);
+    }
+    size_t off = 0;
+    for (auto &p : parts) {
+        *p.dst = h->arena + off;
+        off += align256(p.bytes);
+    }
+    h->temp = temp;
+    // Solver.cu:13-104: uniform initial u, v, w, p, temp; everything else zero.
+    e = hipMemsetAsync(h->arena, 0, total, h->stream);
+    std::vector<float> host(N);
+    float *initDst[5] = {P.u, P.v, P.w, P.p, temp};
+    for (int q = 0; q < 5 && e == hipSuccess; ++q) {
+        std::fill(host.begin(), host.end(), (float)cfg->init[q]);
+        e = hipMemcpyAsync(initDst[q], host.data(), N * 4, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (e != hipSuccess) {
+        std::string m = std::string("device initialisation: ") + hipGetErrorString(e);
+        ifl_destroy(h);
+        return fail(IFL_ERR_DEVICE, m);
+    }
+    *out = h;
+    return IFL_OK;
+}
+
+int ifl_destroy(ifl_solver *h) {
+    if (!h) return IFL_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    for (int f = 0; f < 6; ++f)
+        if (h->prof[f]) (void)hipFree(h->prof[f]);
+    if (h->arena) (void)hipFree(h->arena);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return IFL_OK;
+}
+
+int ifl_set_face_profile(ifl_solver *h, int face, const double *vals, size_t n) {
+    if (!h || face < 0 || face > 5) return fail(IFL_ERR_ARG, "bad handle or face");
+    const ifl_config &c = h->cfg;
+    size_t need = face < 2 ? (size_t)c.ny * c.nz : face < 4 ? (size_t)c.nx * c.nz : (size_t)c.nx * c.ny;
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    if (!vals) {
+        if (h->prof[face]) IFL_HIP(hipFree(h->prof[face]));
+        h->prof[face] = nullptr;
+    } else {
+        if (n != need) return fail(IFL_ERR_ARG, "profile length must equal the number of cells on the face");
+        if (!h->prof[face]) IFL_HIP(hipMalloc((void **)&h->prof[face], need * 4));
+        std::vector<float> tmp(n);
+        for (size_t q = 0; q < n; ++q) tmp[q] = (float)vals[q];
+        IFL_HIP(hipMemcpy(h->prof[face], tmp.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    h->P.prof[face] = h->prof[face];
+    h->graphDirty = true;
+    return IFL_OK;
+}
+
+size_t ifl_field_size(const ifl_solver *h, int field) { return h ? fieldSize(h->cfg, field) : 0; }
+
+int ifl_get_field(ifl_solver *h, int field, float *dst, size_t n) {
+    if (!h || !dst) return fail(IFL_ERR_ARG, "null argument");
+    float *src = fieldPtr(h, field);
+    if (!src) return fail(IFL_ERR_ARG, "bad field id");
+    if (n != fieldSize(h->cfg, field)) return fail(IFL_ERR_ARG, "size mismatch");
+    IFL_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToHost, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    return IFL_OK;
+}
+
+int ifl_set_field(ifl_solver *h, int field, const float *src, size_t n) {
+    if (!h || !src) return fail(IFL_ERR_ARG, "null argument");
+    float *dst = fieldPtr(h, field);
+    if (!dst) return fail(IFL_ERR_ARG, "bad field id");
+    if (n != fieldSize(h->cfg, field)) return fail(IFL_ERR_ARG, "size mismatch");
+    IFL_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    return IFL_OK;
+}
+
+int ifl_solve(ifl_solver *h, int max_iter, ifl_report *report, ifl_iter_record *history, size_t hist_cap) {
+    if (!h) return fail(IFL_ERR_ARG, "null handle");
+    if (max_iter <= 0) max_iter = h->cfg.num_outer_iter;
+    if (report) std::memset(report, 0, sizeof(*report));
+    int st = buildGraph(h);
+    if (st != IFL_OK) return st;
+    st = setDone(h, 0);
+    if (st != IFL_OK) return st;
+    const long long it0 = h->iter;
+    int chunk = h->cfg.check_every > 0 ? std::min(h->cfg.check_every, kHistCap) : 64;
+    const int logEvery = h->cfg.log_every;
+    std::vector<ifl_iter_record> ring;
+    IFL_HIP(hipEventRecord(h->ev0, h->stream));
+    if (h->iter == 0) ifl::launch_init_faces(h->P, h->stream);       // Solver.cu:162-164
+    long long launched = 0;
+    ifl::CaseState s{};
+    double sweepsTotal = 0;
+    while (launched < max_iter) {
+        int n = (int)std::min<long long>(chunk, max_iter - launched);
+        for (int q = 0; q < n; ++q) IFL_HIP(hipGraphLaunch(h->graph, h->stream));
+        launched += n;
+        st = readState(h, s);
+        if (st != IFL_OK) return st;
+        const long long a = h->iter, b = s.iter;
+        if (b > a) {
+            ring.resize((size_t)(b - a));
+            for (long long it = a; it < b; ++it)
+                IFL_HIP(hipMemcpyAsync(&ring[it - a], &h->P.hist[it % kHistCap], sizeof(ifl_iter_record),
+                                       hipMemcpyDeviceToHost, h->stream));
+            IFL_HIP(hipStreamSynchronize(h->stream));
+            for (long long it = a; it < b; ++it) {
+                const ifl_iter_record &r = ring[it - a];
+                long long local = it - it0;
+                if (history && (size_t)local < hist_cap) history[local] = r;
+                for (int q = 0; q < 4; ++q) sweepsTotal += (double)r.sweeps[q] * (double)h->P.N;
+                bool last = (local == max_iter - 1) || r.converged || (it == b - 1 && s.done);
+                if (logEvery > 0 && (local == 0 || (it + 1) % logEvery == 0 || last)) printLog(r, it, h->P.dim);
+            }
+        }
+        h->iter = b;
+        if (s.done) break;
+    }
+    IFL_HIP(hipEventRecord(h->ev1, h->stream));
+    IFL_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    IFL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (report) {
+        report->iterations = (int)(h->iter - it0);
+        report->converged = s.done == 1;
+        report->status = s.done == 2 ? IFL_ERR_DIVERGED : IFL_OK;
+        report->elapsed_ms = ms;
+        report->final_scaled = s.lastScaled;
+        report->sweeps_total = sweepsTotal;
+    }
+    if (s.done == 2) return fail(IFL_ERR_DIVERGED, "non-finite residual: the solve diverged");
+    return IFL_OK;
+}
+
+int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
+    if (!h) return fail(IFL_ERR_ARG, "null handle");
+    double tmp[8] = {0};
+    int st = setDone(h, 0);
+    if (st != IFL_OK) return st;
+    const ifl::CaseParams &P = h->P;
+    ifl::CaseState s0{}, s{};
+    st = readState(h, s0);
+    if (st != IFL_OK) return st;
+    switch (stage) {
+        case IFL_STAGE_INIT_FACES: ifl::launch_init_faces(P, h->stream); break;
+        case IFL_STAGE_FACE_BC: ifl::launch_face_bc(P, h->stream); break;
+        case IFL_STAGE_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
+        case IFL_STAGE_MOM_RESIDUAL: break;
+        case IFL_STAGE_SMOOTH_U: enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0); break;
+        case IFL_STAGE_SMOOTH_V: enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1); break;
+        case IFL_STAGE_SMOOTH_W:
+            if (P.dim != 3) return fail(IFL_ERR_ARG, "SMOOTH_W needs dim 3");
+            enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);
+            break;
+        case IFL_STAGE_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
+        case IFL_STAGE_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(P, h->stream); break;
+        case IFL_STAGE_SMOOTH_P: enqueuePressureCorrection(h); break;
+        case IFL_STAGE_UPDATE: ifl::launch_update(P, 0, h->stream); break;
+        case IFL_STAGE_OUTER:
+            if (h->iter == 0) ifl::launch_init_faces(P, h->stream);
+            enqueueOuter(h);
+            break;
+        default: return fail(IFL_ERR_ARG, "bad stage");
+    }
+    IFL_HIP(hipGetLastError());
+    st = readState(h, s);
+    if (st != IFL_OK) return st;
+    switch (stage) {
+        case IFL_STAGE_MOM_RESIDUAL:
+            for (int q = 0; q < 3; ++q) tmp[q] = std::sqrt(s.resSq[q]);
+            break;
+        case IFL_STAGE_SMOOTH_U:
+        case IFL_STAGE_SMOOTH_V:
+        case IFL_STAGE_SMOOTH_W: {
+            int slot = stage - IFL_STAGE_SMOOTH_U;
+            tmp[0] = s.sm[slot].sweeps;
+            tmp[1] = s.sm[slot].lastSum;
+            break;
+        }
+        case IFL_STAGE_PCORR_ASSEMBLE: tmp[0] = s.mass; break;
+        case IFL_STAGE_SMOOTH_P:
+            tmp[0] = s.sweepsIter[3] - s0.sweepsIter[3];
+            tmp[1] = h->cfg.p_refine > 0 ? s.sm[4].lastSum : s.sm[3].lastSum;
+            break;
+        case IFL_STAGE_UPDATE:
+            for (int q = 0; q < 7; ++q) tmp[q] = s.upd[q];
+            break;
+        case IFL_STAGE_OUTER:
+            tmp[0] = s.lastConverged;
+            tmp[1] = s.lastScaled;
+            h->iter = s.iter;
+            break;
+        default: break;
+    }
+    if (out)
+        for (size_t q = 0; q < n_out && q < 8; ++q) out[q] = tmp[q];
+    return IFL_OK;
+}
+
+int ifl_write_vtk(ifl_solver *h, const char *path) {
+    if (!h || !path) return fail(IFL_ERR_ARG, "null argument");
+    const ifl_config &c = h->cfg;
+    const size_t N = (size_t)h->P.N;
+    std::vector<float> u(N), v(N), w(N), p(N), temp(N);
+    int st;
+    if ((st = ifl_get_field(h, IFL_FIELD_U, u.data(), N)) || (st = ifl_get_field(h, IFL_FIELD_V, v.data(), N))
+        || (st = ifl_get_field(h, IFL_FIELD_W, w.data(), N)) || (st = ifl_get_field(h, IFL_FIELD_P, p.data(), N))
+        || (st = ifl_get_field(h, IFL_FIELD_TEMP, temp.data(), N)))
+        return st;
+    std::filesystem::path fp(path);
+    std::error_code ec;
+    if (fp.has_parent_path() && !std::filesystem::exists(fp.parent_path()))
+        std::filesystem::create_directories(fp.parent_path(), ec);
+    std::ofstream file(fp);
+    if (!file.is_open()) return fail(IFL_ERR_IO, std::string("Failed to open the file: ") + path);
+    // Solver.cu:317-364 (legacy VTK ASCII, float coordinates, default stream precision)
+    const float xmin = (float)c.xmin, ymin = (float)c.ymin, zmin = (float)c.zmin;
+    const float dx = h->P.dx, dy = h->P.dy, dz = h->P.dz;
+    file << "# vtk DataFile Version 3.0" << std::endl;
+    file << "flash 3d grid and solution" << std::endl;
+    file << "ASCII" << std::endl;
+    file << "DATASET RECTILINEAR_GRID" << std::endl;
+    file << "DIMENSIONS " << c.nx + 1 << " " << c.ny + 1 << " " << c.nz + 1 << std::endl;
+    file << "X_COORDINATES " << c.nx + 1 << " float" << std::endl;
+    for (int i = 0; i <= c.nx; ++i) file << xmin + (float)i * dx << " ";
+    file << std::endl;
+    file << "Y_COORDINATES " << c.ny + 1 << " float" << std::endl;
+    for (int i = 0; i <= c.ny; ++i) file << ymin + (float)i * dy << " ";
+    file << std::endl;
+    file << "Z_COORDINATES " << c.nz + 1 << " float" << std::endl;
+    for (int i = 0; i <= c.nz; ++i) file << zmin + (float)i * dz << " ";
+    file << std::endl;
+    const int ncell = c.nx * c.ny * c.nz;
+    file << "CELL_DATA " << ncell << std::endl;
+    file << "FIELD FieldData 3" << std::endl;
+    file << "temperature 1 " << ncell << " float" << std::endl;
+    for (const float &t : temp) file << t << " ";
+    file << std::endl;
+    file << "velocity 3 " << ncell << " float" << std::endl;
+    for (int idx = 0; idx < ncell; ++idx) file << u[idx] << " " << v[idx] << " " << w[idx] << " ";
+    file << std::endl;
+    file << "pressure 1 " << ncell << " float" << std::endl;
+    for (const float &x : p) file << x << " ";
+    file.close();
+    if (!file) return fail(IFL_ERR_IO, std::string("write failed: ") + path);
+    return IFL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,598 @@
+// kernels.hip - gfx950 kernels of one SIMPLE outer iteration on a collocated structured grid.
+//
+// Every kernel is one pass over the cells of a case (one thread per cell, 256-thread
+// workgroups of 4 wave64), does all of a cell's work for its stage in one thread (so the
+// boundary rows that the reference modifies from four concurrent streams, Solver.cu:583-597,
+// are applied race-free and in launch order E,W,N,S,T,B), and reduces its FP64 norms with
+// wave64 butterflies plus a last-workgroup gather in a fixed order (deterministic; replaces
+// the float atomicAdd of kernels.cuh:578 etc.).  Arithmetic follows kernels.cuh expression by
+// expression, with the same float/double promotions, and is compiled with -ffp-contract=off,
+// so a stage's output equals the FP32 CPU oracle's bit for bit.
+#include "ifl_device.hpp"
+#include "ifl_kernels.hpp"
+
+namespace ifl {
+
+__device__ __forceinline__ void cellOf(const CaseParams &P, long long t, int &i, int &j, int &k) {
+    long long r = t / P.nx;
+    i = (int)(t - r * P.nx);
+    k = (int)(r / P.ny);
+    j = (int)(r - (long long)k * P.ny);
+}
+
+// ---------------- initFaceVel, kernels.cuh:10-45 ----------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_init_faces(CaseParams P) {
+    long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P.N) return;
+    int i, j, k;
+    cellOf(P, t, i, j, k);
+    long long c = t;
+    if (i >= 1) P.uf[ufi(P, i, j, k)] = (P.u[c - 1] + P.u[c]) / 2;
+    if (j >= 1) P.vf[vfi(P, i, j, k)] = (P.v[c - P.nx] + P.v[c]) / 2;
+    if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = (P.w[c - (long long)P.nx * P.ny] + P.w[c]) / 2;
+}
+
+// ---------------- applyBCsToFaceVel, kernels.cuh:47-95 ----------------
+template <int DIM>
+__device__ __forceinline__ void applyFaceBCs(const CaseParams &P, int i, int j, int k, long long c) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    if (i == 0) P.uf[ufi(P, 0, j, k)] = faceBC(P, W, j + (long long)ny * k, P.u[c]);
+    if (i == nx - 1) P.uf[ufi(P, nx, j, k)] = faceBC(P, E, j + (long long)ny * k, P.u[c]);
+    if (j == 0) P.vf[vfi(P, i, 0, k)] = faceBC(P, S, i + (long long)nx * k, P.v[c]);
+    if (j == ny - 1) P.vf[vfi(P, i, ny, k)] = faceBC(P, N, i + (long long)nx * k, P.v[c]);
+    if (DIM == 3) {
+        if (k == 0) P.wf[wfi(P, i, j, 0)] = faceBC(P, B, i + (long long)nx * j, P.w[c]);
+        if (k == nz - 1) P.wf[wfi(P, i, j, nz)] = faceBC(P, T, i + (long long)nx * j, P.w[c]);
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_face_bc(CaseParams P) {
+    if (caseDone(P)) return;
+    long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P.N) return;
+    int i, j, k;
+    cellOf(P, t, i, j, k);
+    applyFaceBCs<DIM>(P, i, j, k, t);
+}
+
+// ---------------- momentum assembly ----------------
+// calcMomLinkCoefKernel (kernels.cuh:97-134), calcInteriorMomSrcTermKernel (:136-178),
+// calcBCMomSrcTermKernel (:180-250) and applyBCsToMomEqKernel (:252-334) for one cell, with the
+// start-of-iteration face BCs (Solver.cu:165) applied first.  One coefficient set serves u, v, w
+// (the reference's copies are bitwise equal, Solver.cu:167-171).  Also accumulates the FP64
+// momentum residual sum (b - A phi)^2 of each component before the sweeps.
+__device__ __forceinline__ float momSrc(const CaseParams &P, int d, int idx, int n, long long c, long long st) {
+    const int lo = 2 * d + 1, hi = 2 * d;
+    const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
+    float p_L, p_R;
+    if (idx == 0) {
+        p_L = ghostCopy(P, lo) ? P.p[c] : 0.0f;
+        p_R = P.p[c + st];
+    } else if (idx == n - 1) {
+        p_R = ghostCopy(P, hi) ? P.p[c] : 0.0f;
+        p_L = P.p[c - st];
+    } else {
+        p_L = P.p[c - st];
+        p_R = P.p[c + st];
+    }
+    return 0.5 * (p_L - p_R) * area;
+}
+
+template <int DIM>
+__device__ __forceinline__ void bcRow(const CaseParams &P, int f, long long q, float (&a)[7], float (&b)[3]) {
+    const int ex = f == E ? sE : f == W ? sW : f == N ? sN : f == S ? sS : f == T ? sT : sB;
+    const int in = f == E ? sW : f == W ? sE : f == N ? sS : f == S ? sN : f == T ? sB : sT;
+    const float aEx = a[ex];
+    if (ghostCopy(P, f)) {                       // kernels.cuh:304-320
+        a[sC] -= 2 * aEx;
+        a[in] += aEx / 3;
+#pragma unroll
+        for (int comp = 0; comp < DIM; ++comp) b[comp] -= (float)8 / 3 * aEx * bcval(P, f, comp, q);
+        a[ex] = 0;
+    } else {                                     // :321-331 outlet
+        a[sC] += aEx;
+        a[ex] = 0;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ double momResidual(const CaseParams &P, const float *fld, const float (&a)[7], float b,
+                                              int i, int j, int k, long long c) {
+    const long long pl = (long long)P.nx * P.ny;
+    double r = (double)b;
+    r -= (double)a[sC] * (double)fld[c];
+    r -= (double)a[sE] * (double)((i != P.nx - 1) ? fld[c + 1] : 0.0f);
+    r -= (double)a[sW] * (double)((i != 0) ? fld[c - 1] : 0.0f);
+    r -= (double)a[sN] * (double)((j != P.ny - 1) ? fld[c + P.nx] : 0.0f);
+    r -= (double)a[sS] * (double)((j != 0) ? fld[c - P.nx] : 0.0f);
+    if (DIM == 3) {
+        r -= (double)a[sT] * (double)((k != P.nz - 1) ? fld[c + pl] : 0.0f);
+        r -= (double)a[sB] * (double)((k != 0) ? fld[c - pl] : 0.0f);
+    }
+    return r;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P) {
+    __shared__ double lds[kWaves * 3];
+    if (caseDone(P)) return;
+    const long long nblk = (P.N + kBlock - 1) / kBlock;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    double rs[3] = {0.0, 0.0, 0.0};
+    if (t < P.N) {
+        int i, j, k;
+        cellOf(P, t, i, j, k);
+        const long long c = t;
+        const int nx = P.nx, ny = P.ny, nz = P.nz;
+        const long long pl = (long long)nx * ny;
+        applyFaceBCs<DIM>(P, i, j, k, c);
+        // face velocities (boundary faces hold the values just applied)
+        const float ue = (i == nx - 1) ? faceBC(P, E, j + (long long)ny * k, P.u[c]) : P.uf[ufi(P, i + 1, j, k)];
+        const float uw = (i == 0) ? faceBC(P, W, j + (long long)ny * k, P.u[c]) : P.uf[ufi(P, i, j, k)];
+        const float vn = (j == ny - 1) ? faceBC(P, N, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j + 1, k)];
+        const float vs = (j == 0) ? faceBC(P, S, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j, k)];
+        const float rho = P.rho, mu = P.mu;
+        float a[7];
+        a[sE] = (rho * min0(ue) * P.Ax) - mu * P.Ax / P.dx;      // kernels.cuh:116-120
+        a[sW] = (rho * min0(-uw) * P.Ax) - mu * P.Ax / P.dx;
+        a[sN] = (rho * min0(vn) * P.Ay) - mu * P.Ay / P.dy;
+        a[sS] = (rho * min0(-vs) * P.Ay) - mu * P.Ay / P.dy;
+        a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
+        a[sT] = 0.0f;
+        a[sB] = 0.0f;
+        if (DIM == 3) {                                          // :122-132 (T2/T3 fixed)
+            const float wt = (k == nz - 1) ? faceBC(P, T, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)];
+            const float wb = (k == 0) ? faceBC(P, B, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)];
+            a[sT] = (rho * min0(wt) * P.Az) - mu * P.Az / P.dz;
+            a[sB] = (rho * min0(-wb) * P.Az) - mu * P.Az / P.dz;
+            a[sC] += -(a[sT] + a[sB]);
+        }
+        float b[3];
+        b[0] = momSrc(P, 0, i, nx, c, 1);
+        b[1] = momSrc(P, 1, j, ny, c, nx);
+        b[2] = DIM == 3 ? momSrc(P, 2, k, nz, c, pl) : 0.0f;
+        // BC rows in launch order E, W, N, S, T, B (Solver.cu:583-608)
+        if (i == nx - 1) bcRow<DIM>(P, E, j + (long long)ny * k, a, b);
+        if (i == 0) bcRow<DIM>(P, W, j + (long long)ny * k, a, b);
+        if (j == ny - 1) bcRow<DIM>(P, N, i + (long long)nx * k, a, b);
+        if (j == 0) bcRow<DIM>(P, S, i + (long long)nx * k, a, b);
+        if (DIM == 3) {
+            if (k == nz - 1) bcRow<DIM>(P, T, i + (long long)nx * j, a, b);
+            if (k == 0) bcRow<DIM>(P, B, i + (long long)nx * j, a, b);
+        }
+#pragma unroll
+        for (int s = 0; s < 1 + 2 * DIM; ++s) P.mcoef[c + P.N * s] = a[s];
+        P.bu[c] = b[0];
+        P.bv[c] = b[1];
+        if (DIM == 3) P.bw[c] = b[2];
+        double r0 = momResidual<DIM>(P, P.u, a, b[0], i, j, k, c);
+        double r1 = momResidual<DIM>(P, P.v, a, b[1], i, j, k, c);
+        rs[0] = r0 * r0;
+        rs[1] = r1 * r1;
+        if (DIM == 3) {
+            double r2 = momResidual<DIM>(P, P.w, a, b[2], i, j, k, c);
+            rs[2] = r2 * r2;
+        }
+    }
+    if (publishPartials<3>(P, rs, blockIdx.x, nblk, lds)) {
+        double tot[3];
+        gatherPartials<3>(P, nblk, tot, lds);
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < 3; ++q) P.st->resSq[q] = tot[q];
+            for (int q = 0; q < 4; ++q) P.st->sweepsIter[q] = 0;
+        }
+    }
+}
+
+// ---------------- red-black Gauss-Seidel half sweep ----------------
+// GaussSeidelIterateKernel (kernels.cuh:926-1005) restricted to cells of one colour
+// ((i+j+k)&1 == colour), so each half sweep is order independent and the full sweep
+// (red then black) is deterministic.  Neighbours outside the grid are 0 (:939-974).
+template <int DIM>
+__device__ __forceinline__ float gsUpdate(const CaseParams &P, const float *fld, const float *coef, const float *src,
+                                          int i, int j, int k, long long c, float rlx, float &dPhi) {
+    const long long N = P.N, pl = (long long)P.nx * P.ny;
+    const float phi_C = fld[c];
+    const float phi_E = (i != P.nx - 1) ? fld[c + 1] : 0.0f;
+    const float phi_W = (i != 0) ? fld[c - 1] : 0.0f;
+    const float phi_N = (j != P.ny - 1) ? fld[c + P.nx] : 0.0f;
+    const float phi_S = (j != 0) ? fld[c - P.nx] : 0.0f;
+    float newPhi = src[c] - coef[c + N * sE] * phi_E - coef[c + N * sW] * phi_W - coef[c + N * sN] * phi_N
+                   - coef[c + N * sS] * phi_S;
+    if (DIM == 3) {
+        const float phi_T = (k != P.nz - 1) ? fld[c + pl] : 0.0f;
+        const float phi_B = (k != 0) ? fld[c - pl] : 0.0f;
+        newPhi -= coef[c + N * sT] * phi_T + coef[c + N * sB] * phi_B;
+    }
+    newPhi /= coef[c + N * sC];
+    dPhi = rlx * (newPhi - phi_C);
+    return phi_C + dPhi;
+}
+
+// Full-sweep norm bookkeeping, Solver.cu:955-962 in FP64.
+__device__ __forceinline__ void finishSweep(const CaseParams &P, int slot, int eq, int sweep, double sum, double tol) {
+    SmoothState &sm = P.st->sm[slot];
+    double norm = sqrt(sum / (double)P.N);
+    double mx = sweep == 0 ? -1e20 : sm.maxNorm;
+    mx = fmax(norm, mx);
+    double rel = norm / (mx + 1e-20);
+    sm.maxNorm = mx;
+    sm.lastSum = sum;
+    sm.sweeps = sweep + 1;
+    sm.stop = (rel < tol) ? 1 : 0;
+    P.st->sweepsIter[eq] += 1;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_sweep_rb(CaseParams P, float *fld, const float *coef, const float *src,
+                                                     float rlx, double tol, int slot, int eq, int sweep, int color) {
+    __shared__ double lds[kWaves];
+    if (caseDone(P)) return;
+    if (sweep > 0 && P.st->sm[slot].stop) return;
+    const int hx = (P.nx + 1) >> 1;
+    const long long rows = (long long)P.ny * P.nz;
+    const long long nblk = (hx * rows + kBlock - 1) / kBlock;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    double s[1] = {0.0};
+    const long long r = t / hx;
+    if (r < rows) {
+        const int m = (int)(t - r * hx);
+        const int k = (int)(r / P.ny);
+        const int j = (int)(r - (long long)k * P.ny);
+        const int i = 2 * m + ((j + k + color) & 1);
+        if (i < P.nx) {
+            const long long c = i + (long long)P.nx * r;
+            float d;
+            fld[c] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, c, rlx, d);
+            s[0] = (double)d * (double)d;
+        }
+    }
+    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
+        double tot[1];
+        gatherPartials<1>(P, nblk, tot, lds);
+        if (threadIdx.x == 0) {
+            if (color == 0) P.st->sm[slot].redSum = tot[0];
+            else finishSweep(P, slot, eq, sweep, P.st->sm[slot].redSum + tot[0], tol);
+        }
+    }
+}
+
+// ---------------- point Jacobi sweep, kernels.cuh:845-924 ----------------
+// Reads fld, writes dst; k_copy_back then moves dst into fld when the sweep ran (the
+// reference's pointer swap, Solver.cu:894-896, with the result always in the caller's field).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const float *fld, float *dst, const float *coef,
+                                                         const float *src, float rlx, double tol, int slot, int eq,
+                                                         int sweep) {
+    __shared__ double lds[kWaves];
+    if (caseDone(P)) return;
+    if (sweep > 0 && P.st->sm[slot].stop) return;
+    const long long nblk = (P.N + kBlock - 1) / kBlock;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    double s[1] = {0.0};
+    if (t < P.N) {
+        int i, j, k;
+        cellOf(P, t, i, j, k);
+        float d;
+        dst[t] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, t, rlx, d);
+        s[0] = (double)d * (double)d;
+    }
+    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
+        double tot[1];
+        gatherPartials<1>(P, nblk, tot, lds);
+        if (threadIdx.x == 0) finishSweep(P, slot, eq, sweep, tot[0], tol);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P, float *fld, const float *src, int slot, int sweep) {
+    if (caseDone(P)) return;
+    if (P.st->sm[slot].sweeps != sweep + 1) return;   // sweep did not run
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t < P.N) fld[t] = src[t];
+}
+
+// ---------------- Rhie-Chow, kernels.cuh:336-500, then face BCs (Solver.cu:187) ----------------
+__device__ __forceinline__ float rhieChow(const CaseParams &P, int d, long long c, int idx, int n, long long st,
+                                          const float *vel, const float *aCp) {
+    const int lo = 2 * d + 1, hi = 2 * d;
+    const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
+    const long long LId = c - st, RId = c;
+    const float aC_L = aCp[LId];
+    const float aC_R = aCp[RId];
+    const float u_L = vel[LId];
+    const float u_R = vel[RId];
+    const float p_L = P.p[LId], p_R = P.p[RId];
+    float p_LL, p_RR;
+    if (idx == 1) {
+        p_LL = ghostCopy(P, lo) ? p_L : 0.0f;
+        p_RR = P.p[RId + st];
+    } else if (idx == n - 1) {
+        p_RR = ghostCopy(P, hi) ? p_R : 0.0f;
+        p_LL = P.p[LId - st];
+    } else {
+        p_LL = P.p[LId - st];
+        p_RR = P.p[RId + st];
+    }
+    return 0.5 * (u_L + u_R) + (p_R - p_LL) * area / (4 * aC_L) + (p_RR - p_L) * area / (4 * aC_R)
+           + 0.5 * (1 / aC_L + 1 / aC_R) * (p_L - p_R) * area;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P) {
+    if (caseDone(P)) return;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P.N) return;
+    int i, j, k;
+    cellOf(P, t, i, j, k);
+    const long long c = t;
+    const float *aC = P.mcoef;   // plane sC
+    if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
+    if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, c, j, P.ny, P.nx, P.v, aC);
+    if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, c, k, P.nz, (long long)P.nx * P.ny, P.w, aC);
+    applyFaceBCs<DIM>(P, i, j, k, c);
+}
+
+// ---------------- pressure-correction assembly, kernels.cuh:502-581, and p' = 0 ----------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
+    __shared__ double lds[kWaves];
+    if (caseDone(P)) return;
+    const long long nblk = (P.N + kBlock - 1) / kBlock;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    double s[1] = {0.0};
+    if (t < P.N) {
+        int i, j, k;
+        cellOf(P, t, i, j, k);
+        const long long c = t, N = P.N, pl = (long long)P.nx * P.ny;
+        const float *uC = P.mcoef;
+        const float rho = P.rho, Ax = P.Ax, Ay = P.Ay, Az = P.Az;
+        float aE = 0.0f, aW = 0.0f, aN = 0.0f, aS = 0.0f, aT = 0.0f, aB = 0.0f;
+        if (i < P.nx - 1) aE = -0.5 * rho * Ax * Ax * (1 / uC[c] + 1 / uC[c + 1]);
+        if (i > 0) aW = -0.5 * rho * Ax * Ax * (1 / uC[c] + 1 / uC[c - 1]);
+        if (j < P.ny - 1) aN = -0.5 * rho * Ay * Ay * (1 / uC[c] + 1 / uC[c + P.nx]);
+        if (j > 0) aS = -0.5 * rho * Ay * Ay * (1 / uC[c] + 1 / uC[c - P.nx]);
+        float aCc = -(aE + aW + aN + aS);
+        if (DIM == 3) {
+            if (k < P.nz - 1) aT = -0.5 * rho * Az * Az * (1 / uC[c] + 1 / uC[c + pl]);
+            if (k > 0) aB = -0.5 * rho * Az * Az * (1 / uC[c] + 1 / uC[c - pl]);
+            aCc += -(aT + aB);
+        }
+        float *pcf = P.pcoef;
+        pcf[c] = aCc;
+        pcf[c + N * sE] = aE;
+        pcf[c + N * sW] = aW;
+        pcf[c + N * sN] = aN;
+        pcf[c + N * sS] = aS;
+        if (DIM == 3) {
+            pcf[c + N * sT] = aT;
+            pcf[c + N * sB] = aB;
+        }
+        float b = rho * (Ax * (P.uf[ufi(P, i, j, k)] - P.uf[ufi(P, i + 1, j, k)])
+                         + Ay * (P.vf[vfi(P, i, j, k)] - P.vf[vfi(P, i, j + 1, k)]));
+        if (DIM == 3) b += rho * Az * (P.wf[wfi(P, i, j, k)] - P.wf[wfi(P, i, j, k + 1)]);
+        P.bp[c] = b;
+        P.pc[c] = 0.0f;                       // Solver.cu:193
+        s[0] = (double)b * (double)b;
+    }
+    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
+        double tot[1];
+        gatherPartials<1>(P, nblk, tot, lds);
+        if (threadIdx.x == 0) P.st->mass = tot[0];
+    }
+}
+
+// ---------------- FP64 defect of the p' system (refinement), delta = 0 ----------------
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P) {
+    if (caseDone(P)) return;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P.N) return;
+    int i, j, k;
+    cellOf(P, t, i, j, k);
+    float a[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[t + P.N * s] : 0.0f;
+    P.tmp[t] = (float)momResidual<DIM>(P, P.pc, a, P.bp[t], i, j, k, t);
+    P.tmp2[t] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcorr_accumulate(CaseParams P) {
+    if (caseDone(P)) return;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t < P.N) P.pc[t] += P.tmp2[t];
+}
+
+// ---------------- updateField, kernels.cuh:583-843, and the outer-iteration monitor ----------------
+__device__ __forceinline__ float velCorr(const CaseParams &P, int d, int idx, int n, long long c, long long st) {
+    const int lo = 2 * d + 1, hi = 2 * d;
+    const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
+    float pc_L, pc_R;
+    if (idx == 0) {
+        pc_L = ghostCopy(P, lo) ? P.pc[c] : 0.0f;
+        pc_R = P.pc[c + st];
+    } else if (idx == n - 1) {
+        pc_R = ghostCopy(P, hi) ? P.pc[c] : 0.0f;
+        pc_L = P.pc[c - st];
+    } else {
+        pc_L = P.pc[c - st];
+        pc_R = P.pc[c + st];
+    }
+    const float aC_C = P.mcoef[c];
+    return P.relax[d] * 0.5 * (pc_L - pc_R) * area / aC_C;
+}
+__device__ __forceinline__ float faceCorr(const CaseParams &P, int d, long long c, long long st) {
+    const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
+    const float aC_L = P.mcoef[c - st];
+    const float aC_R = P.mcoef[c];
+    return P.relax[d] * 0.5 * (1 / aC_L + 1 / aC_R) * (P.pc[c - st] - P.pc[c]) * area;
+}
+
+// Solver.cu:226-268 (FP64) + scaled residual; one thread.
+__device__ void finishOuter(const CaseParams &P, const double (&u)[7]) {
+    CaseState &S = *P.st;
+    const double Nd = (double)P.N;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    ifl_iter_record rec;
+    rec.p_norm = sqrt(u[0] / Nd);
+    rec.u_norm = sqrt(u[1] / Nd);
+    rec.v_norm = sqrt(u[2] / Nd);
+    rec.w_norm = 0.0;
+    rec.uf_norm = sqrt(u[4] / ((double)(nx - 1) * ny * nz));
+    rec.vf_norm = sqrt(u[5] / ((double)nx * (ny - 1) * nz));
+    rec.wf_norm = 0.0;
+    rec.mass_norm = sqrt(S.mass / Nd);
+    if (P.dim == 3) {
+        rec.w_norm = sqrt(u[3] / Nd);
+        rec.wf_norm = sqrt(u[6] / ((double)nx * ny * (nz - 1)));
+    }
+    rec.res_cont = sqrt(S.mass);
+    rec.res_u = sqrt(S.resSq[0]);
+    rec.res_v = sqrt(S.resSq[1]);
+    rec.res_w = sqrt(S.resSq[2]);
+    const double rs[4] = {rec.res_cont, rec.res_u, rec.res_v, rec.res_w};
+    if (S.iter < 5)
+        for (int q = 0; q < 4; ++q) S.resMax[q] = fmax(S.resMax[q], rs[q]);
+    double sc = 0.0;
+    for (int q = 0; q < (P.dim == 3 ? 4 : 3); ++q)
+        if (S.resMax[q] > 0) sc = fmax(sc, rs[q] / S.resMax[q]);
+    rec.scaled = sc;
+    S.pNormMax = fmax(S.pNormMax, rec.p_norm);
+    if (P.stopRule == IFL_STOP_FAITHFUL) {
+        double m = fmax(S.pNormMax, rec.p_norm);
+        S.uNormMax = S.vNormMax = S.ufNormMax = S.vfNormMax = S.wNormMax = S.wfNormMax = m;
+    } else {
+        S.uNormMax = fmax(S.uNormMax, rec.u_norm);
+        S.vNormMax = fmax(S.vNormMax, rec.v_norm);
+        S.wNormMax = fmax(S.wNormMax, rec.w_norm);
+        S.ufNormMax = fmax(S.ufNormMax, rec.uf_norm);
+        S.vfNormMax = fmax(S.vfNormMax, rec.vf_norm);
+        S.wfNormMax = fmax(S.wfNormMax, rec.wf_norm);
+    }
+    rec.rel_p = rec.p_norm / S.pNormMax;
+    rec.rel_u = rec.u_norm / S.uNormMax;
+    rec.rel_v = rec.v_norm / S.vNormMax;
+    rec.rel_uf = rec.uf_norm / S.ufNormMax;
+    rec.rel_vf = rec.vf_norm / S.vfNormMax;
+    rec.rel_w = 0.0;
+    rec.rel_wf = 0.0;
+    if (P.dim == 3) {
+        rec.rel_w = rec.w_norm / S.wNormMax;
+        rec.rel_wf = rec.wf_norm / S.wfNormMax;
+    }
+    bool conv = false;
+    if (P.stopRule == IFL_STOP_FAITHFUL || P.stopRule == IFL_STOP_FIXED) {
+        conv = rec.rel_p < P.tol[3] && rec.rel_u < P.tol[0] && rec.rel_v < P.tol[1] && rec.rel_uf < P.tol[0]
+               && rec.rel_vf < P.tol[1];
+        if (P.dim == 3) conv = conv && rec.rel_w < P.tol[2] && rec.rel_wf < P.tol[2];
+        conv = conv || rec.mass_norm < P.tolMass;
+    } else if (P.stopRule == IFL_STOP_SCALED) {
+        conv = sc < P.tolRes;
+    }
+    for (int q = 0; q < 4; ++q) rec.sweeps[q] = S.sweepsIter[q];
+    rec.converged = conv ? 1 : 0;
+    rec.pad = 0;
+    S.lastScaled = sc;
+    S.lastConverged = rec.converged;
+    if (P.hist) P.hist[S.iter % P.histCap] = rec;
+    const double chk[] = {rec.p_norm, rec.u_norm, rec.v_norm, rec.w_norm, rec.uf_norm, rec.vf_norm,
+                          rec.wf_norm, rec.mass_norm, rec.res_cont, rec.res_u, rec.res_v, rec.res_w};
+    bool finite = true;
+    for (double x : chk) finite = finite && isfinite(x);
+    S.iter += 1;
+    if (!finite) S.done = 2;
+    else if (conv) S.done = 1;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_update(CaseParams P, int finish) {
+    __shared__ double lds[kWaves * 7];
+    if (caseDone(P)) return;
+    const long long nblk = (P.N + kBlock - 1) / kBlock;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (t < P.N) {
+        int i, j, k;
+        cellOf(P, t, i, j, k);
+        const long long c = t, pl = (long long)P.nx * P.ny;
+        const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
+        P.p[c] += dp;
+        s[0] = (double)dp * dp;
+        const float du = velCorr(P, 0, i, P.nx, c, 1);
+        P.u[c] += du;
+        s[1] = (double)du * du;
+        const float dv = velCorr(P, 1, j, P.ny, c, P.nx);
+        P.v[c] += dv;
+        s[2] = (double)dv * dv;
+        if (DIM == 3) {
+            const float dw = velCorr(P, 2, k, P.nz, c, pl);
+            P.w[c] += dw;
+            s[3] = (double)dw * dw;
+        }
+        if (i >= 1) {
+            const float d = faceCorr(P, 0, c, 1);
+            P.uf[ufi(P, i, j, k)] += d;
+            s[4] = (double)d * d;
+        }
+        if (j >= 1) {
+            const float d = faceCorr(P, 1, c, P.nx);
+            P.vf[vfi(P, i, j, k)] += d;
+            s[5] = (double)d * d;
+        }
+        if (DIM == 3 && k >= 1) {
+            const float d = faceCorr(P, 2, c, pl);
+            P.wf[wfi(P, i, j, k)] += d;
+            s[6] = (double)d * d;
+        }
+    }
+    if (publishPartials<7>(P, s, blockIdx.x, nblk, lds)) {
+        double tot[7];
+        gatherPartials<7>(P, nblk, tot, lds);
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < 7; ++q) P.st->upd[q] = tot[q];
+            if (finish) finishOuter(P, tot);
+        }
+    }
+}
+
+// ---------------- host-side launchers ----------------
+static inline unsigned cellBlocks(const CaseParams &P) { return (unsigned)((P.N + kBlock - 1) / kBlock); }
+static inline unsigned rbBlocks(const CaseParams &P) {
+    long long hx = (P.nx + 1) / 2;
+    return (unsigned)((hx * P.ny * P.nz + kBlock - 1) / kBlock);
+}
+
+#define IFL_DISPATCH(DIMV, KERNEL, GRID, STREAM, ...)                                   \
+    do {                                                                               \
+        if ((DIMV) == 3) KERNEL<3><<<(GRID), kBlock, 0, (STREAM)>>>(__VA_ARGS__);      \
+        else KERNEL<2><<<(GRID), kBlock, 0, (STREAM)>>>(__VA_ARGS__);                  \
+    } while (0)
+
+void launch_init_faces(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_init_faces, cellBlocks(P), s, P); }
+void launch_face_bc(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_face_bc, cellBlocks(P), s, P); }
+void launch_mom_assemble(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_mom_assemble, cellBlocks(P), s, P); }
+void launch_rhie_chow(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_rhie_chow, cellBlocks(P), s, P); }
+void launch_pcorr_assemble(const CaseParams &P, hipStream_t s) {
+    IFL_DISPATCH(P.dim, k_pcorr_assemble, cellBlocks(P), s, P);
+}
+void launch_update(const CaseParams &P, int finish, hipStream_t s) {
+    IFL_DISPATCH(P.dim, k_update, cellBlocks(P), s, P, finish);
+}
+void launch_pcorr_defect(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_pcorr_defect, cellBlocks(P), s, P); }
+void launch_pcorr_accumulate(const CaseParams &P, hipStream_t s) {
+    k_pcorr_accumulate<<<cellBlocks(P), kBlock, 0, s>>>(P);
+}
+
+void launch_sweep(const CaseParams &P, int smoother, float *fld, const float *coef, const float *src, float rlx,
+                  double tol, int slot, int eq, int sweep, hipStream_t s) {
+    if (smoother == IFL_SMOOTHER_JACOBI) {
+        IFL_DISPATCH(P.dim, k_sweep_jacobi, cellBlocks(P), s, P, fld, P.jac, coef, src, rlx, tol, slot, eq, sweep);
+        k_copy_back<<<cellBlocks(P), kBlock, 0, s>>>(P, fld, P.jac, slot, sweep);
+        return;
+    }
+    for (int color = 0; color < 2; ++color)
+        IFL_DISPATCH(P.dim, k_sweep_rb, rbBlocks(P), s, P, fld, coef, src, rlx, tol, slot, eq, sweep, color);
+}
+
+}  // namespace ifl

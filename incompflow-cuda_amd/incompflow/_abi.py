@@ -1,0 +1,202 @@
+"""ctypes mirror of include/incompflow.h (the C-ABI of the solver library).
+
+Only data layouts and constants live here; the library binding is in ``_lib.py``.
+Field-for-field correspondence with the reference's compile-time contract:
+``ifl_config`` <-> include/config.h:6-81, face/slot/BC codes <-> include/constants.h:7-36.
+"""
+import ctypes as C
+
+IFL_ABI_VERSION = 1
+
+# status codes
+IFL_OK, IFL_ERR_ARG, IFL_ERR_DEVICE, IFL_ERR_DIVERGED, IFL_ERR_NOT_CONVERGED, IFL_ERR_IO, IFL_ERR_NOMEM = range(7)
+
+# faces (constants.h:7-12)
+EAST, WEST, NORTH, SOUTH, TOP, BOTTOM = range(6)
+# coefficient slots (constants.h:14-20)
+SLOT_C, SLOT_E, SLOT_W, SLOT_N, SLOT_S, SLOT_T, SLOT_B = range(7)
+# BC codes (constants.h:34-36)
+WALL, INLET, OUTLET = range(3)
+
+SMOOTHER_RBGS, SMOOTHER_JACOBI, SMOOTHER_LEX = range(3)
+STOP_NONE, STOP_FAITHFUL, STOP_FIXED, STOP_SCALED = range(4)
+
+(FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_PCORR, FIELD_UF, FIELD_VF, FIELD_WF,
+ FIELD_MCOEF, FIELD_PCOEF, FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP) = range(15)
+FIELD_NAMES = ["u", "v", "w", "p", "temp", "pcorr", "uf", "vf", "wf", "mcoef", "pcoef",
+               "bu", "bv", "bw", "bp"]
+
+(STAGE_INIT_FACES, STAGE_FACE_BC, STAGE_MOM_ASSEMBLE, STAGE_MOM_RESIDUAL, STAGE_SMOOTH_U,
+ STAGE_SMOOTH_V, STAGE_SMOOTH_W, STAGE_RHIE_CHOW, STAGE_PCORR_ASSEMBLE, STAGE_SMOOTH_P,
+ STAGE_UPDATE, STAGE_OUTER) = range(12)
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("xmin", C.c_double), ("xmax", C.c_double),
+        ("ymin", C.c_double), ("ymax", C.c_double),
+        ("zmin", C.c_double), ("zmax", C.c_double),
+        ("density", C.c_double),
+        ("viscosity", C.c_double),
+        ("bc_type", C.c_int32 * 6),
+        ("bc_val", (C.c_double * 3) * 6),
+        ("num_outer_iter", C.c_int32),
+        ("n_iter", C.c_int32 * 4),
+        ("relax", C.c_double * 4),
+        ("tol", C.c_double * 4),
+        ("tol_mass", C.c_double),
+        ("init", C.c_double * 5),
+        ("smoother", C.c_int32),
+        ("stop_rule", C.c_int32),
+        ("tol_residual", C.c_double),
+        ("p_refine", C.c_int32),
+        ("log_every", C.c_int32),
+        ("check_every", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+    def copy(self):
+        c = Config()
+        C.pointer(c)[0] = self
+        return c
+
+    def field_size(self, field):
+        nx, ny, nz, dim = self.nx, self.ny, self.nz, self.dim
+        n = nx * ny * nz
+        if field in (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_PCORR,
+                     FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP):
+            return n
+        if field == FIELD_UF:
+            return (nx + 1) * ny * nz
+        if field == FIELD_VF:
+            return nx * (ny + 1) * nz
+        if field == FIELD_WF:
+            return nx * ny * (nz + 1)
+        if field in (FIELD_MCOEF, FIELD_PCOEF):
+            return n * (1 + 2 * dim)
+        raise ValueError(field)
+
+    def shape(self, field):
+        """numpy shape (slowest first) of a field."""
+        nx, ny, nz, dim = self.nx, self.ny, self.nz, self.dim
+        if field == FIELD_UF:
+            return (nz, ny, nx + 1)
+        if field == FIELD_VF:
+            return (nz, ny + 1, nx)
+        if field == FIELD_WF:
+            return (nz + 1, ny, nx)
+        if field in (FIELD_MCOEF, FIELD_PCOEF):
+            return (1 + 2 * dim, nz, ny, nx)
+        return (nz, ny, nx)
+
+
+class IterRecord(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "p_norm", "u_norm", "v_norm", "w_norm", "uf_norm", "vf_norm", "wf_norm", "mass_norm",
+        "res_cont", "res_u", "res_v", "res_w", "scaled",
+        "rel_p", "rel_u", "rel_v", "rel_w", "rel_uf", "rel_vf", "rel_wf")] + [
+        ("sweeps", C.c_int32 * 4), ("converged", C.c_int32), ("pad", C.c_int32)]
+
+
+class Report(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("status", C.c_int32),
+                ("pad", C.c_int32), ("elapsed_ms", C.c_double), ("final_scaled", C.c_double),
+                ("sweeps_total", C.c_double)]
+
+
+def default_config():
+    """The reference's shipped include/config.h (1024x256 channel on [-2,2]x[-0.5,0.5])."""
+    c = Config()
+    c.dim = 2
+    c.nx, c.ny, c.nz = 1024, 256, 1
+    c.xmin, c.xmax, c.ymin, c.ymax, c.zmin, c.zmax = -2.0, 2.0, -0.5, 0.5, 0.0, 1.0
+    c.density, c.viscosity = 50.0, 1.0
+    for f, t in enumerate((OUTLET, INLET, WALL, WALL, WALL, WALL)):
+        c.bc_type[f] = t
+        for q in range(3):
+            c.bc_val[f][q] = 0.0
+    c.bc_val[WEST][0] = 1.0
+    c.num_outer_iter = 20000
+    for q, (n, r, t) in enumerate(((10, 0.75, 1e-5), (10, 0.75, 1e-5), (10, 0.75, 1e-5), (20, 0.25, 1e-3))):
+        c.n_iter[q], c.relax[q], c.tol[q] = n, r, t
+    c.tol_mass = 1e-9
+    for q, val in enumerate((0.0, 0.0, 0.0, 0.0, 273.0)):
+        c.init[q] = val
+    c.smoother = SMOOTHER_RBGS
+    c.stop_rule = STOP_FAITHFUL
+    c.tol_residual = 1e-5
+    c.p_refine = 0
+    c.log_every = 0
+    c.check_every = 0
+    return c
+
+
+def _set_relax(c, alpha_u, alpha_p):
+    for q in range(3):
+        c.relax[q] = alpha_u
+    c.relax[3] = alpha_p
+
+
+def channel_config(nx=512, ny=128, iters=3000):
+    """Plain channel: README defaults grid/physics (README.md:187-280) with alpha 0.7/0.3.
+
+    512x128 on x in [-0.0625, 0.0625], y in [-0.0125, 0.0125]; rho=998.2, mu=1.002e-3;
+    west inlet u=0.001, east outlet, north/south walls; fixed outer iteration count.
+    """
+    c = default_config()
+    c.nx, c.ny = nx, ny
+    c.xmin, c.xmax, c.ymin, c.ymax = -0.0625, 0.0625, -0.0125, 0.0125
+    c.density, c.viscosity = 998.2, 1.002e-3
+    c.bc_val[WEST][0] = 0.001
+    _set_relax(c, 0.7, 0.3)
+    c.num_outer_iter = iters
+    c.stop_rule = STOP_NONE
+    c.init[4] = 0.0
+    return c
+
+
+def cavity_config(n=1024, re=100.0, p_refine=1, tol_residual=1e-5, max_iter=200000):
+    """2D lid-driven cavity on the unit square, rho=1, mu=1/Re, lid u=1 on the north wall."""
+    c = default_config()
+    c.nx, c.ny = n, n
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 1.0, 0.0, 1.0
+    c.density, c.viscosity = 1.0, 1.0 / re
+    for f in range(6):
+        c.bc_type[f] = WALL
+        for q in range(3):
+            c.bc_val[f][q] = 0.0
+    c.bc_val[NORTH][0] = 1.0
+    _set_relax(c, 0.7, 0.3)
+    c.stop_rule = STOP_SCALED
+    c.tol_residual = tol_residual
+    c.num_outer_iter = max_iter
+    c.p_refine = p_refine
+    c.init[4] = 0.0
+    return c
+
+
+def cavity3d_config(n=256, re=400.0, tol_residual=1e-5, max_iter=200000):
+    """3D lid-driven cavity on the unit cube, rho=1, mu=1/Re, lid u=1 on the top (z=max) face."""
+    c = cavity_config(n=n, re=re, p_refine=0, tol_residual=tol_residual, max_iter=max_iter)
+    c.dim = 3
+    c.nz = n
+    c.bc_val[NORTH][0] = 0.0
+    c.bc_val[TOP][0] = 1.0
+    return c
+
+
+def adjoint_channel_config(nx=1024, ny=256):
+    """Adjoint channel: [0,0.5]x[0,0.05], rho=1, mu=1e-3, west inlet profile, east outlet."""
+    c = default_config()
+    c.nx, c.ny = nx, ny
+    c.xmin, c.xmax, c.ymin, c.ymax = 0.0, 0.5, 0.0, 0.05
+    c.density, c.viscosity = 1.0, 1e-3
+    c.bc_val[WEST][0] = 0.02
+    _set_relax(c, 0.7, 0.3)
+    c.stop_rule = STOP_SCALED
+    c.tol_residual = 1e-6
+    c.num_outer_iter = 400000
+    c.init[4] = 0.0
+    return c

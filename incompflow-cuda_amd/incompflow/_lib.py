@@ -1,0 +1,127 @@
+"""ctypes binding of libincompflow.so (include/incompflow.h).
+
+The library is the product: it must be built (``make -C incompflow-cuda_amd/csrc``) and a GPU
+must be present; there is no CPU fallback, and every failing call raises IflError with the
+library's own message.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libincompflow.so")
+
+# (name, restype, argtypes) for every entry point declared in include/incompflow.h
+EXPORTS = [
+    ("ifl_last_error", C.c_char_p, []),
+    ("ifl_abi_version", C.c_int, []),
+    ("ifl_config_default", None, [C.POINTER(_abi.Config)]),
+    ("ifl_create", C.c_int, [C.POINTER(_abi.Config), C.POINTER(C.c_void_p)]),
+    ("ifl_destroy", C.c_int, [C.c_void_p]),
+    ("ifl_set_face_profile", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
+    ("ifl_solve", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.Report), C.POINTER(_abi.IterRecord), C.c_size_t]),
+    ("ifl_get_field", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
+    ("ifl_set_field", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
+    ("ifl_field_size", C.c_size_t, [C.c_void_p, C.c_int]),
+    ("ifl_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
+]
+
+_lib = None
+
+
+class IflError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"incompflow status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IflError(-1, f"{LIB_PATH} is missing: build it with `make -C incompflow-cuda_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in EXPORTS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        if L.ifl_abi_version() != _abi.IFL_ABI_VERSION:
+            raise IflError(-1, "ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise IflError(st, lib().ifl_last_error().decode())
+    return st
+
+
+class Solver:
+    """One case on the GPU: the reference's Solver (include/Solver.h:46-52) over the C-ABI."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg.copy()
+        self.h = C.c_void_p()
+        _check(lib().ifl_create(C.byref(self.cfg), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ifl_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, max_iter=0, history=True):
+        """Run outer iterations; returns dict(iterations, converged, status, elapsed_ms, history)."""
+        n = max_iter if max_iter > 0 else self.cfg.num_outer_iter
+        cap = n if history else 0
+        hist = (_abi.IterRecord * max(cap, 1))()
+        rep = _abi.Report()
+        st = lib().ifl_solve(self.h, max_iter, C.byref(rep), hist, cap)
+        if st not in (0, _abi.IFL_ERR_DIVERGED):
+            _check(st)
+        recs = [hist[i] for i in range(min(rep.iterations, cap))]
+        return {"iterations": rep.iterations, "converged": bool(rep.converged), "status": st,
+                "elapsed_ms": rep.elapsed_ms, "final_scaled": rep.final_scaled,
+                "sweeps_total": rep.sweeps_total, "history": recs}
+
+    def get(self, field, shaped=True):
+        n = self.cfg.field_size(field)
+        out = np.empty(n, dtype=np.float32)
+        _check(lib().ifl_get_field(self.h, field, out.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return out.reshape(self.cfg.shape(field)) if shaped else out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).ravel())
+        _check(lib().ifl_set_field(self.h, field, a.ctypes.data_as(C.POINTER(C.c_float)), a.size))
+
+    def set_face_profile(self, face, vals):
+        if vals is None:
+            _check(lib().ifl_set_face_profile(self.h, face, None, 0))
+            return
+        a = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).ravel())
+        _check(lib().ifl_set_face_profile(self.h, face, a.ctypes.data_as(C.POINTER(C.c_double)), a.size))
+
+    def stage(self, stage):
+        out = np.zeros(8, dtype=np.float64)
+        _check(lib().ifl_run_stage(self.h, stage, out.ctypes.data_as(C.POINTER(C.c_double)), 8))
+        return out
+
+    def write_vtk(self, path):
+        _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))

@@ -1,0 +1,125 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Imported only by tests, the smoke check and the CPU-baseline leg of bench.py, as the checker.
+The library is built from oracle/simple_oracle.hpp + oracle/oracle_capi.cpp by oracle/Makefile.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_HERE)
+sys.path.insert(0, os.path.join(_ROOT, "incompflow-cuda_amd"))
+from incompflow import _abi  # noqa: E402
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(_abi.Config), C.c_int, C.POINTER(vp)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_field_size.argtypes = [vp, C.c_int]
+        L.orc_field_size.restype = C.c_size_t
+        L.orc_get_field.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_set_field.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_set_face_profile.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_run_stage.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_solve.argtypes = [vp, C.c_int, C.POINTER(_abi.IterRecord), C.c_size_t,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_set_iter.argtypes = [vp, C.c_longlong]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_config_size.restype = C.c_size_t
+        L.orc_record_size.restype = C.c_size_t
+        L.orc_set_threads.argtypes = [C.c_int]
+        assert L.orc_config_size() == C.sizeof(_abi.Config), "ifl_config layout mismatch"
+        assert L.orc_record_size() == C.sizeof(_abi.IterRecord), "ifl_iter_record layout mismatch"
+        _lib = L
+    return _lib
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """One oracle case (FP32 by default, FP64 with fp64=True)."""
+
+    def __init__(self, cfg, fp64=False):
+        self.cfg = cfg.copy()
+        self.fp64 = fp64
+        self.h = C.c_void_p()
+        self._check(lib().orc_create(C.byref(self.cfg), 1 if fp64 else 0, C.byref(self.h)))
+
+    def _check(self, st):
+        if st != 0:
+            raise OracleError(f"oracle status {st}: {lib().orc_last_error().decode()}")
+        return st
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, field, shaped=True):
+        n = lib().orc_field_size(self.h, field)
+        out = np.empty(n, dtype=np.float64)
+        self._check(lib().orc_get_field(self.h, field, out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        if not self.fp64:
+            out = out.astype(np.float32)
+        return out.reshape(self.cfg.shape(field)) if shaped else out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+        self._check(lib().orc_set_field(self.h, field, a.ctypes.data_as(C.POINTER(C.c_double)), a.size))
+
+    def set_face_profile(self, face, vals):
+        if vals is None:
+            self._check(lib().orc_set_face_profile(self.h, face, None, 0))
+            return
+        a = np.ascontiguousarray(np.asarray(vals, dtype=np.float64).ravel())
+        self._check(lib().orc_set_face_profile(self.h, face, a.ctypes.data_as(C.POINTER(C.c_double)), a.size))
+
+    def stage(self, stage):
+        out = np.zeros(8, dtype=np.float64)
+        self._check(lib().orc_run_stage(self.h, stage, out.ctypes.data_as(C.POINTER(C.c_double)), 8))
+        return out
+
+    def solve(self, max_iter, history=True):
+        cap = max_iter if history else 0
+        hist = (_abi.IterRecord * max(cap, 1))()
+        n = C.c_int()
+        conv = C.c_int()
+        st = lib().orc_solve(self.h, max_iter, hist, cap, C.byref(n), C.byref(conv))
+        recs = [hist[i] for i in range(min(n.value, cap))]
+        return {"iterations": n.value, "converged": bool(conv.value), "status": st, "history": recs}
+
+    def set_iter(self, it):
+        lib().orc_set_iter(self.h, int(it))

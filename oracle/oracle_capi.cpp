@@ -1,0 +1,174 @@
+// oracle_capi.cpp - extern "C" surface of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+// Loaded with ctypes by the tests, the smoke check and the CPU-baseline leg of the benchmark.
+// Values cross as double (exact for FP32 state), so float results compare bit for bit.
+#include <omp.h>
+
+#include <cstring>
+#include <new>
+
+#include "simple_oracle.hpp"
+
+namespace {
+
+struct Handle {
+    int fp64;
+    oracle::Simple<float> *f;
+    oracle::Simple<double> *d;
+};
+
+thread_local std::string g_err;
+
+template <class Real>
+int get_field(oracle::Simple<Real> *s, int id, double *dst, size_t n) {
+    auto *v = s->field(id);
+    if (!v) { g_err = "bad field id"; return IFL_ERR_ARG; }
+    if (n != v->size()) { g_err = "size mismatch"; return IFL_ERR_ARG; }
+    for (size_t q = 0; q < n; ++q) dst[q] = (double)(*v)[q];
+    return IFL_OK;
+}
+
+template <class Real>
+int set_field(oracle::Simple<Real> *s, int id, const double *src, size_t n) {
+    auto *v = s->field(id);
+    if (!v) { g_err = "bad field id"; return IFL_ERR_ARG; }
+    if (n != v->size()) { g_err = "size mismatch"; return IFL_ERR_ARG; }
+    for (size_t q = 0; q < n; ++q) (*v)[q] = (Real)src[q];
+    // the oracle keeps separate u/v/w momentum matrices (Solver.h:28-30); setting MCOEF sets all
+    if (id == IFL_FIELD_MCOEF) { s->vcoef = s->ucoef; s->wcoef = s->ucoef; }
+    return IFL_OK;
+}
+
+template <class Real>
+int run_stage(oracle::Simple<Real> *s, int stage, double *out, size_t n_out) {
+    double tmp[8] = {0};
+    switch (stage) {
+        case IFL_STAGE_INIT_FACES: s->initFaceVel(); break;
+        case IFL_STAGE_FACE_BC: s->applyBCsToFaceVel(); break;
+        case IFL_STAGE_MOM_ASSEMBLE:   // Solver.cu:165-175: face BCs, coefficients, sources, BC rows
+            s->applyBCsToFaceVel();
+            s->assembleMomentum();
+            break;
+        case IFL_STAGE_MOM_RESIDUAL: s->momentumResidual(tmp); break;
+        case IFL_STAGE_SMOOTH_U:
+        case IFL_STAGE_SMOOTH_V:
+        case IFL_STAGE_SMOOTH_W: {
+            int q = stage - IFL_STAGE_SMOOTH_U;
+            Real *fld = q == 0 ? s->u.data() : q == 1 ? s->v.data() : s->w.data();
+            const Real *cf = q == 0 ? s->ucoef.data() : q == 1 ? s->vcoef.data() : s->wcoef.data();
+            const Real *src = q == 0 ? s->bu.data() : q == 1 ? s->bv.data() : s->bw.data();
+            tmp[0] = s->iterate(fld, cf, src, s->nIter[q], s->relax[q], s->tol[q]);
+            tmp[1] = s->lastSweepSum;
+            break;
+        }
+        case IFL_STAGE_RHIE_CHOW: s->RhieChowInterpolate(); break;
+        case IFL_STAGE_PCORR_ASSEMBLE:
+            s->calcPresCorrLinkCoef();
+            tmp[0] = s->calcPresCorrSrcTerm();
+            std::fill(s->pc.begin(), s->pc.end(), (Real)0);
+            break;
+        case IFL_STAGE_SMOOTH_P:
+            tmp[0] = s->solvePressureCorrection();
+            tmp[1] = s->lastSweepSum;
+            break;
+        case IFL_STAGE_UPDATE: s->updateField(tmp); break;
+        case IFL_STAGE_OUTER: {
+            ifl_iter_record rec;
+            tmp[0] = s->outerIteration(rec) ? 1 : 0;
+            tmp[1] = rec.scaled;
+            break;
+        }
+        default: g_err = "bad stage"; return IFL_ERR_ARG;
+    }
+    if (out)
+        for (size_t q = 0; q < n_out && q < 8; ++q) out[q] = tmp[q];
+    return IFL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
+int orc_create(const ifl_config *cfg, int fp64, void **out) {
+    if (!cfg || !out) { g_err = "null argument"; return IFL_ERR_ARG; }
+    Handle *h = new (std::nothrow) Handle{fp64, nullptr, nullptr};
+    if (!h) return IFL_ERR_NOMEM;
+    bool ok;
+    if (fp64) { h->d = new oracle::Simple<double>(); ok = h->d->init(*cfg); if (!ok) g_err = h->d->error; }
+    else { h->f = new oracle::Simple<float>(); ok = h->f->init(*cfg); if (!ok) g_err = h->f->error; }
+    if (!ok) { delete h->f; delete h->d; delete h; return IFL_ERR_ARG; }
+    *out = h;
+    return IFL_OK;
+}
+
+int orc_destroy(void *hv) {
+    Handle *h = (Handle *)hv;
+    if (!h) return IFL_OK;
+    delete h->f;
+    delete h->d;
+    delete h;
+    return IFL_OK;
+}
+
+size_t orc_field_size(void *hv, int id) {
+    Handle *h = (Handle *)hv;
+    if (h->fp64) { auto *v = h->d->field(id); return v ? v->size() : 0; }
+    auto *v = h->f->field(id);
+    return v ? v->size() : 0;
+}
+
+int orc_get_field(void *hv, int id, double *dst, size_t n) {
+    Handle *h = (Handle *)hv;
+    return h->fp64 ? get_field(h->d, id, dst, n) : get_field(h->f, id, dst, n);
+}
+
+int orc_set_field(void *hv, int id, const double *src, size_t n) {
+    Handle *h = (Handle *)hv;
+    return h->fp64 ? set_field(h->d, id, src, n) : set_field(h->f, id, src, n);
+}
+
+int orc_set_face_profile(void *hv, int face, const double *vals, size_t n) {
+    Handle *h = (Handle *)hv;
+    if (face < 0 || face > 5) { g_err = "bad face"; return IFL_ERR_ARG; }
+    auto setp = [&](auto *s) {
+        using R = typename std::remove_reference<decltype(s->u[0])>::type;
+        size_t need = (face < 2) ? (size_t)s->ny * s->nz : (face < 4) ? (size_t)s->nx * s->nz : (size_t)s->nx * s->ny;
+        if (!vals) { s->prof[face].clear(); return IFL_OK; }
+        if (n != need) { g_err = "profile size mismatch"; return IFL_ERR_ARG; }
+        s->prof[face].assign(n, (R)0);
+        for (size_t q = 0; q < n; ++q) s->prof[face][q] = (R)vals[q];
+        return IFL_OK;
+    };
+    return h->fp64 ? setp(h->d) : setp(h->f);
+}
+
+int orc_run_stage(void *hv, int stage, double *out, size_t n_out) {
+    Handle *h = (Handle *)hv;
+    return h->fp64 ? run_stage(h->d, stage, out, n_out) : run_stage(h->f, stage, out, n_out);
+}
+
+int orc_solve(void *hv, int max_iter, ifl_iter_record *hist, size_t cap, int *iters, int *converged) {
+    Handle *h = (Handle *)hv;
+    int st = IFL_OK;
+    int n = h->fp64 ? h->d->solve(max_iter, hist, cap, converged, &st) : h->f->solve(max_iter, hist, cap, converged, &st);
+    if (iters) *iters = n;
+    return st;
+}
+
+// Overwrite the oracle's iteration counter / residual normalisation (warm starts).
+int orc_set_iter(void *hv, long long it) {
+    Handle *h = (Handle *)hv;
+    if (h->fp64) h->d->iter = it; else h->f->iter = it;
+    return IFL_OK;
+}
+
+size_t orc_config_size(void) { return sizeof(ifl_config); }
+size_t orc_record_size(void) { return sizeof(ifl_iter_record); }
+
+}  // extern "C"
