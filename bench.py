@@ -1,0 +1,175 @@
+"""Benchmark: steady SIMPLE loop, 2D lid-driven cavity 1024x1024 (Re=100), FP32, on one GPU.
+
+One step = one SIMPLE outer iteration (momentum assembly, u/v smoother sweeps, Rhie-Chow,
+p' assembly, p' sweeps + one FP64-defect refinement cycle, correction, FP64 norms and stop test)
+over the whole grid, run through the library's captured hipGraph.  Prints one JSON line.
+
+value      : smoother cell-updates per second over the timed steps (cells x sweeps executed / s,
+             the whole outer iteration's time in the denominator)
+roofline   : the red-black sweep kernel, timed with HIP events on the library's stream
+cpu_baseline: the FP32 CPU oracle (OpenMP, host cores) on a bounded number of outer iterations
+time_to_residual_s: a separate solve from rest to the 1e-5 scaled residual (unless --no-converge)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "incompflow-cuda_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time to 1e-5 residual & cell-updates/s (%HBM peak), plain vs batched+adjoint"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def sweep_bytes_per_cell(dim):
+    # compulsory FP32 traffic of one smoother sweep: (1+2dim) coefficients + source + phi read/write
+    return 4 * ((1 + 2 * dim) + 1 + 2)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--converge-budget-s", type=float, default=150.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=0)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU slab decomposition is a design, not an implementation.", file=sys.stderr)
+        sys.exit(2)
+
+    import incompflow
+    from incompflow import _abi
+
+    n = args.n
+    cfg = _abi.cavity_config(n=n, p_refine=1)
+    cells = cfg.nx * cfg.ny * cfg.nz
+
+    # ---- timed steps ----
+    run_cfg = cfg.copy()
+    run_cfg.stop_rule = _abi.STOP_NONE
+    s = incompflow.Solver(run_cfg)
+    log(f"[bench] cavity {n}x{n}: warmup {args.warmup} outer iterations")
+    s.solve(args.warmup, history=False)
+    device_sync()
+    log(f"[bench] timing {args.steps} outer iterations")
+    t0 = time.perf_counter()
+    r = s.solve(args.steps, history=True)
+    device_sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    sweeps = sum(sum(rec.sweeps) for rec in r["history"])
+    ms_per_step = 1000.0 * wall / args.steps
+    value = cells * sweeps / wall
+    log(f"[bench] {ms_per_step:.4f} ms/step (device {r['elapsed_ms'] / args.steps:.4f} ms), "
+        f"{sweeps / args.steps:.1f} sweeps/step")
+
+    # ---- roofline of the dominant kernel (red-black half sweep) ----
+    us, per = s.time_kernel(_abi.KERNEL_SWEEP, reps=args.kernel_reps)
+    alg_bytes = sweep_bytes_per_cell(cfg.dim) * cells / per     # one launch updates 1/per of the cells
+    achieved = alg_bytes / (us * 1e-6) / 1e9
+    other = {}
+    for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
+                      ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE)):
+        other[name] = round(s.time_kernel(kid, reps=max(10, args.kernel_reps // 4))[0], 3)
+    s.close()
+    log(f"[bench] sweep launch {us:.3f} us -> {achieved:.1f} GB/s algorithmic")
+
+    # ---- time to residual ----
+    ttr = None
+    iters_to_res = None
+    converged = None
+    if not args.no_converge:
+        c = incompflow.Solver(cfg)
+        log(f"[bench] solving from rest to scaled residual {cfg.tol_residual:g} (budget {args.converge_budget_s:.0f} s)")
+        device_sync()
+        t0 = time.perf_counter()
+        total = 0
+        converged = False
+        last = None
+        while True:
+            rr = c.solve(5000, history=False)
+            total += rr["iterations"]
+            last = rr["final_scaled"]
+            log(f"[bench]   {total} iterations, scaled residual {last:.3e}, {time.perf_counter() - t0:.1f} s")
+            if rr["converged"] or rr["status"] != 0:
+                converged = rr["converged"]
+                break
+            if time.perf_counter() - t0 > args.converge_budget_s or total >= cfg.num_outer_iter:
+                break
+        device_sync()
+        ttr = time.perf_counter() - t0
+        iters_to_res = total
+        c.close()
+
+    # ---- CPU baseline: FP32 oracle on the host cores, bounded sample ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        threads = orc.get_threads()
+        o = orc.Oracle(cfg)
+        it_cpu = args.cpu_iters or 8
+        o.solve(1, history=False)   # first iteration (face initialisation) outside the sample
+        t0 = time.perf_counter()
+        ro = o.solve(it_cpu, history=True)
+        tc = time.perf_counter() - t0
+        csweeps = sum(sum(rec.sweeps) for rec in ro["history"])
+        cpu = {"value": cells * csweeps / tc, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{it_cpu} outer iterations of the {n}x{n} cavity (FP32 oracle, OpenMP), "
+                         f"{1000 * tc / it_cpu:.1f} ms/iteration"}
+        o.close()
+        log(f"[bench] cpu oracle {1000 * tc / it_cpu:.1f} ms/iteration on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (fields start at rest)",
+        "config": {"workload": f"2D lid-driven cavity {n}x{n}, Re=100, FP32 RB-GS 10/10/20 sweeps, "
+                               f"alpha 0.7/0.3, FP64 reductions, 1 FP64-defect p' refinement",
+                   "cells": cells, "sweeps_per_step": sweeps / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweep_rb (red-black half sweep)", "avg_launch_us": us,
+                     "alg_bytes_per_launch": alg_bytes},
+        "kernel_us": other,
+        "time_to_residual_s": ttr,
+        "iterations_to_residual": iters_to_res,
+        "converged": converged,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

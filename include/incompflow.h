@@ -189,6 +189,22 @@ int ifl_write_vtk(ifl_solver *h, const char *path);
  *   OUTER          : [0] converged flag, [1] scaled residual */
 int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out);
 
+/* Kernels timed by ifl_time_kernel. */
+enum {
+    IFL_KERNEL_SWEEP = 0,         /* one full red-black sweep of the p' system (both colours) */
+    IFL_KERNEL_MOM_ASSEMBLE = 1,
+    IFL_KERNEL_RHIE_CHOW = 2,
+    IFL_KERNEL_PCORR_ASSEMBLE = 3,
+    IFL_KERNEL_UPDATE = 4,
+    IFL_KERNEL_COUNT = 5
+};
+
+/* Measurement hook (no reference counterpart): launch one kernel `reps` times back to back on
+ * the handle's stream and return the mean time per launch in microseconds from HIP events
+ * (for IFL_KERNEL_SWEEP, per half-sweep launch; *launches_per_rep receives the launches per
+ * repetition).  Overwrites the handle's device state of that stage. */
+int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *launches_per_rep);
+
 #ifdef __cplusplus
 }
 #endif

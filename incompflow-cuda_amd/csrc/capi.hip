@@ -500,6 +500,39 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
     return IFL_OK;
 }
 
+int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *launches_per_rep) {
+    if (!h || reps <= 0 || !avg_us) return fail(IFL_ERR_ARG, "bad argument");
+    int st = setDone(h, 0);
+    if (st != IFL_OK) return st;
+    const ifl::CaseParams &P = h->P;
+    int per = 1;
+    auto once = [&]() {
+        switch (kernel) {
+            case IFL_KERNEL_SWEEP:
+                ifl::launch_sweep(P, h->cfg.smoother, P.pc, P.pcoef, P.bp, P.relax[3], P.tol[3], 3, 3, 0, h->stream);
+                break;
+            case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
+            case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
+            case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(P, h->stream); break;
+            case IFL_KERNEL_UPDATE: ifl::launch_update(P, 0, h->stream); break;
+            default: break;
+        }
+    };
+    if (kernel < 0 || kernel >= IFL_KERNEL_COUNT) return fail(IFL_ERR_ARG, "bad kernel id");
+    if (kernel == IFL_KERNEL_SWEEP) per = 2;
+    once();   // warm-up
+    IFL_HIP(hipGetLastError());
+    IFL_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int r = 0; r < reps; ++r) once();
+    IFL_HIP(hipEventRecord(h->ev1, h->stream));
+    IFL_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    IFL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *avg_us = 1000.0 * ms / ((double)reps * per);
+    if (launches_per_rep) *launches_per_rep = per;
+    return IFL_OK;
+}
+
 int ifl_write_vtk(ifl_solver *h, const char *path) {
     if (!h || !path) return fail(IFL_ERR_ARG, "null argument");
     const ifl_config &c = h->cfg;

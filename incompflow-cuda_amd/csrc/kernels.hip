@@ -10,6 +10,7 @@
 // so a stage's output equals the FP32 CPU oracle's bit for bit.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
+#include "ifl_stencil.hpp"
 
 namespace ifl {
 
@@ -82,19 +83,12 @@ __device__ __forceinline__ float momSrc(const CaseParams &P, int d, int idx, int
 
 template <int DIM>
 __device__ __forceinline__ void bcRow(const CaseParams &P, int f, long long q, float (&a)[7], float (&b)[3]) {
-    const int ex = f == E ? sE : f == W ? sW : f == N ? sN : f == S ? sS : f == T ? sT : sB;
-    const int in = f == E ? sW : f == W ? sE : f == N ? sS : f == S ? sN : f == T ? sB : sT;
-    const float aEx = a[ex];
-    if (ghostCopy(P, f)) {                       // kernels.cuh:304-320
-        a[sC] -= 2 * aEx;
-        a[in] += aEx / 3;
+    const float aEx = a[bcExSlot(f)];
+    if (ghostCopy(P, f)) {                       // kernels.cuh:307, 312, 318
 #pragma unroll
         for (int comp = 0; comp < DIM; ++comp) b[comp] -= (float)8 / 3 * aEx * bcval(P, f, comp, q);
-        a[ex] = 0;
-    } else {                                     // :321-331 outlet
-        a[sC] += aEx;
-        a[ex] = 0;
     }
+    bcRowCoef(P, f, a);
 }
 
 template <int DIM>
@@ -133,22 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P) {
         const float uw = (i == 0) ? faceBC(P, W, j + (long long)ny * k, P.u[c]) : P.uf[ufi(P, i, j, k)];
         const float vn = (j == ny - 1) ? faceBC(P, N, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j + 1, k)];
         const float vs = (j == 0) ? faceBC(P, S, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j, k)];
-        const float rho = P.rho, mu = P.mu;
+        const float wt = DIM == 3 ? ((k == nz - 1) ? faceBC(P, T, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)]) : 0.0f;
+        const float wb = DIM == 3 ? ((k == 0) ? faceBC(P, B, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)]) : 0.0f;
         float a[7];
-        a[sE] = (rho * min0(ue) * P.Ax) - mu * P.Ax / P.dx;      // kernels.cuh:116-120
-        a[sW] = (rho * min0(-uw) * P.Ax) - mu * P.Ax / P.dx;
-        a[sN] = (rho * min0(vn) * P.Ay) - mu * P.Ay / P.dy;
-        a[sS] = (rho * min0(-vs) * P.Ay) - mu * P.Ay / P.dy;
-        a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
-        a[sT] = 0.0f;
-        a[sB] = 0.0f;
-        if (DIM == 3) {                                          // :122-132 (T2/T3 fixed)
-            const float wt = (k == nz - 1) ? faceBC(P, T, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)];
-            const float wb = (k == 0) ? faceBC(P, B, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)];
-            a[sT] = (rho * min0(wt) * P.Az) - mu * P.Az / P.dz;
-            a[sB] = (rho * min0(-wb) * P.Az) - mu * P.Az / P.dz;
-            a[sC] += -(a[sT] + a[sB]);
-        }
+        momLinkCoef<DIM>(P, ue, uw, vn, vs, wt, wb, a);
         float b[3];
         b[0] = momSrc(P, 0, i, nx, c, 1);
         b[1] = momSrc(P, 1, j, ny, c, nx);

@@ -28,6 +28,7 @@ EXPORTS = [
     ("ifl_field_size", C.c_size_t, [C.c_void_p, C.c_int]),
     ("ifl_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
+    ("ifl_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
 ]
 
 _lib = None
@@ -122,6 +123,13 @@ class Solver:
         out = np.zeros(8, dtype=np.float64)
         _check(lib().ifl_run_stage(self.h, stage, out.ctypes.data_as(C.POINTER(C.c_double)), 8))
         return out
+
+    def time_kernel(self, kernel, reps=50):
+        """Mean device time per launch (microseconds) and launches per repetition."""
+        us = C.c_double()
+        per = C.c_int()
+        _check(lib().ifl_time_kernel(self.h, kernel, reps, C.byref(us), C.byref(per)))
+        return us.value, per.value
 
     def write_vtk(self, path):
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))

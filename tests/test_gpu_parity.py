@@ -1,0 +1,208 @@
+"""Parity of the gfx950 library with the FP32 CPU oracle (runs on an MI355X).
+
+Stage outputs must be bit-identical to the oracle's: both sides evaluate kernels.cuh's
+expressions with the same float/double promotions and no fused multiply-add, and red-black
+ordering makes each half sweep order independent.  FP64 norms are compared at 1e-12 relative
+(their summation order differs).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import CASES, STATE_FIELDS, assert_bitwise, load_state, max_rel, random_state
+from incompflow import _abi
+from incompflow import Solver, IflError
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+OUT_FIELDS = STATE_FIELDS + (_abi.FIELD_MCOEF, _abi.FIELD_PCOEF, _abi.FIELD_BU, _abi.FIELD_BV, _abi.FIELD_BP)
+
+STAGES = [
+    ("FACE_BC", _abi.STAGE_FACE_BC),
+    ("MOM_ASSEMBLE", _abi.STAGE_MOM_ASSEMBLE),
+    ("MOM_RESIDUAL", _abi.STAGE_MOM_RESIDUAL),
+    ("SMOOTH_U", _abi.STAGE_SMOOTH_U),
+    ("SMOOTH_V", _abi.STAGE_SMOOTH_V),
+    ("SMOOTH_W", _abi.STAGE_SMOOTH_W),
+    ("RHIE_CHOW", _abi.STAGE_RHIE_CHOW),
+    ("PCORR_ASSEMBLE", _abi.STAGE_PCORR_ASSEMBLE),
+    ("SMOOTH_P", _abi.STAGE_SMOOTH_P),
+    ("UPDATE", _abi.STAGE_UPDATE),
+]
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare_all(g, o, cfg, what):
+    for f in OUT_FIELDS:
+        if cfg.dim == 2 and f in (_abi.FIELD_W, _abi.FIELD_WF):
+            continue
+        assert_bitwise(g.get(f, shaped=False), o.get(f, shaped=False), f"{what}: field {_abi.FIELD_NAMES[f]}")
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("p_refine", [0, 1])
+def test_stage_by_stage_bitwise(name, p_refine):
+    cfg = CASES[name]()
+    cfg.p_refine = p_refine
+    _log(f"[stage parity] case={name} p_refine={p_refine} grid={cfg.nx}x{cfg.ny}x{cfg.nz}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    st = random_state(cfg, seed=7)
+    load_state(g, st)
+    load_state(o, st)
+    g.stage(_abi.STAGE_INIT_FACES)
+    o.stage(_abi.STAGE_INIT_FACES)
+    _compare_all(g, o, cfg, "INIT_FACES")
+    for label, sid in STAGES:
+        if sid == _abi.STAGE_SMOOTH_W and cfg.dim == 2:
+            continue
+        og = o.stage(sid)
+        gg = g.stage(sid)
+        _compare_all(g, o, cfg, label)
+        if sid in (_abi.STAGE_SMOOTH_U, _abi.STAGE_SMOOTH_V, _abi.STAGE_SMOOTH_W, _abi.STAGE_SMOOTH_P):
+            assert gg[0] == og[0], f"{label}: sweeps {gg[0]} vs {og[0]}"
+            assert gg[1] == pytest.approx(og[1], rel=1e-12, abs=0)
+        elif sid in (_abi.STAGE_MOM_RESIDUAL, _abi.STAGE_PCORR_ASSEMBLE, _abi.STAGE_UPDATE):
+            np.testing.assert_allclose(gg, og, rtol=1e-12, atol=0, err_msg=label)
+
+
+def _rec_array(history, keys=("p_norm", "u_norm", "v_norm", "w_norm", "uf_norm", "vf_norm", "wf_norm",
+                              "mass_norm", "res_cont", "res_u", "res_v", "res_w", "scaled")):
+    return np.array([[getattr(r, k) for k in keys] for r in history])
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_outer_iterations_graph_bitwise(name):
+    """Five outer iterations through the captured hipGraph equal the oracle's loop."""
+    cfg = CASES[name]()
+    cfg.stop_rule = _abi.STOP_NONE
+    _log(f"[outer parity] case={name}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(5)
+    ro = o.solve(5)
+    assert rg["iterations"] == ro["iterations"] == 5
+    _compare_all(g, o, cfg, f"{name} after 5 iterations")
+    np.testing.assert_allclose(_rec_array(rg["history"]), _rec_array(ro["history"]), rtol=1e-11, atol=1e-300)
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+
+
+def test_graph_equals_stage_path():
+    cfg = CASES["cavity2d"]()
+    a = Solver(cfg)
+    b = Solver(cfg)
+    a.solve(4)
+    for _ in range(4):
+        b.stage(_abi.STAGE_OUTER)
+    for f in (_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_P, _abi.FIELD_UF, _abi.FIELD_VF):
+        assert_bitwise(a.get(f), b.get(f), _abi.FIELD_NAMES[f])
+
+
+def test_determinism_two_runs():
+    cfg = CASES["channel2d"]()
+    runs = []
+    for _ in range(2):
+        g = Solver(cfg)
+        g.solve(20)
+        runs.append([g.get(f) for f in (_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_P)])
+    for a, b in zip(*runs):
+        assert_bitwise(a, b, "repeat run")
+
+
+def test_jacobi_smoother_parity():
+    cfg = CASES["cavity2d"]()
+    cfg.smoother = _abi.SMOOTHER_JACOBI
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    g.solve(4)
+    o.solve(4)
+    _compare_all(g, o, cfg, "jacobi")
+
+
+def test_inlet_profile_parity():
+    cfg = CASES["channel2d"]()
+    prof = 0.001 * (1.0 + 0.3 * np.sin(np.arange(cfg.ny)))
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    g.set_face_profile(_abi.WEST, prof)
+    o.set_face_profile(_abi.WEST, prof)
+    g.solve(6)
+    o.solve(6)
+    _compare_all(g, o, cfg, "profile")
+
+
+def test_converged_cavity_same_iterations():
+    """Scaled-residual stop: same iteration count and bit-identical converged fields."""
+    cfg = _abi.cavity_config(n=32, p_refine=0, tol_residual=1e-5, max_iter=5000)
+    _log("[converged] cavity 32x32 to 1e-5")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(5000, history=False)
+    ro = o.solve(5000, history=False)
+    assert rg["converged"] and ro["converged"]
+    assert rg["iterations"] == ro["iterations"]
+    _compare_all(g, o, cfg, "converged cavity")
+
+
+def test_channel_c1_3000_iterations():
+    """Plain method, 512x128 channel, 3000 outer iterations, field by field vs the FP32 oracle."""
+    cfg = _abi.channel_config(512, 128, 3000)
+    _log("[C1] channel 512x128, 3000 iterations")
+    g = Solver(cfg)
+    rg = g.solve(3000, history=False)
+    assert rg["iterations"] == 3000
+    o = Oracle(cfg)
+    ro = o.solve(3000, history=False)
+    assert ro["iterations"] == 3000
+    for f in (_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_P):
+        a, b = g.get(f), o.get(f)
+        assert max_rel(a, b) <= 1e-4, _abi.FIELD_NAMES[f]
+        assert_bitwise(a, b, f"C1 {_abi.FIELD_NAMES[f]}")
+
+
+def test_nan_initial_field_reports_divergence():
+    cfg = CASES["cavity2d"]()
+    g = Solver(cfg)
+    u = g.get(_abi.FIELD_U)
+    u[3, 4] = np.nan
+    g.set(_abi.FIELD_U, u)
+    r = g.solve(10)
+    assert r["status"] == _abi.IFL_ERR_DIVERGED
+    assert r["iterations"] == 1
+
+
+def test_vtk_output_format(tmp_path):
+    cfg = CASES["cavity2d"]()
+    g = Solver(cfg)
+    g.solve(3)
+    path = tmp_path / "out" / "case.vtk"
+    g.write_vtk(str(path))
+    text = path.read_text()
+    lines = text.split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0"
+    assert lines[3] == "DATASET RECTILINEAR_GRID"
+    assert lines[4] == f"DIMENSIONS {cfg.nx + 1} {cfg.ny + 1} {cfg.nz + 1}"
+    assert lines[5] == f"X_COORDINATES {cfg.nx + 1} float"
+    xs = np.array(lines[6].split(), dtype=np.float64)
+    np.testing.assert_allclose(xs, np.linspace(cfg.xmin, cfg.xmax, cfg.nx + 1), atol=1e-6)
+    ncell = cfg.nx * cfg.ny * cfg.nz
+    assert f"CELL_DATA {ncell}" in lines
+    i = lines.index(f"velocity 3 {ncell} float")
+    vel = np.array(lines[i + 1].split(), dtype=np.float64).reshape(-1, 3)
+    u = g.get(_abi.FIELD_U).ravel()
+    np.testing.assert_allclose(vel[:, 0], u, rtol=1e-5, atol=1e-6)
+    assert not text.endswith("\n")   # Solver.cu:361-366 writes no final newline
+
+
+def test_bad_config_rejected():
+    cfg = CASES["cavity2d"]()
+    cfg.nx = 2
+    with pytest.raises(IflError):
+        Solver(cfg)
