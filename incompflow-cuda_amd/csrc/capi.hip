@@ -57,6 +57,7 @@ struct ifl_solver {
     long long iter = 0;   // host mirror of CaseState::iter
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float tolf[4];
+    int tile = 0;     // 2D sweep tile variant
 };
 
 namespace {
@@ -77,6 +78,7 @@ int validate(const ifl_config &c) {
         return fail(IFL_ERR_ARG, "smoother must be IFL_SMOOTHER_RBGS or IFL_SMOOTHER_JACOBI on the device");
     if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
     if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
+    if (c.tile < 0 || c.tile > 4) return fail(IFL_ERR_ARG, "tile must be 0..4");
     return IFL_OK;
 }
 
@@ -112,18 +114,92 @@ void fillParams(ifl_solver *h) {
     P.histCap = kHistCap;
 }
 
-unsigned maxBlocks(const ifl::CaseParams &P) {
-    long long cells = (P.N + ifl::kBlock - 1) / ifl::kBlock;
-    long long hx = (P.nx + 1) / 2;
-    long long rb = (hx * P.ny * P.nz + ifl::kBlock - 1) / ifl::kBlock;
-    return (unsigned)std::max(cells, rb);
+// Records per (slot, parity) sweep-partial array: one per workgroup of the largest sweep launch
+// (fused 2D tiles: the smallest tile gives the most; red+black half sweeps; Jacobi cell grid).
+int partStride(const ifl::CaseParams &P) {
+    int n = std::max(ifl::cell_blocks(P), 2 * ifl::rb_blocks(P));
+    if (P.dim == 2)
+        for (int t = ifl::TILE_64x16; t <= ifl::TILE_32x4; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
+    return n;
 }
 
-// Solver.cu:160-290 stage order, all on the handle's stream.
+// Smoother calls of Solver.cu:178-196.  In 2D with red-black ordering every sweep is one
+// launch of the fused tile kernel, reading one buffer of a ping-pong pair and writing the other;
+// an odd sweep count ends with a copy back.  Equations listed together share the matrix and
+// are swept in the same launches (u and v); each keeps its own norms and stop flag.
+struct EqRef {
+    float *fld, *alt;
+    const float *src;
+    int q, slot, n;
+};
+
+void enqueueFused2d(ifl_solver *h, int kind, const float *coef, EqRef *e, int neq) {
+    const ifl::CaseParams &P = h->P;
+    int nmax = 0;
+    for (int k = 0; k < neq; ++k) nmax = std::max(nmax, e[k].n);
+    for (int s = 0; s < nmax; ++s) {
+        ifl::SweepEq se[2];
+        int m = 0;
+        for (int k = 0; k < neq; ++k) {
+            if (s >= e[k].n) continue;
+            const bool even = (s % 2) == 0;
+            se[m++] = ifl::SweepEq{even ? e[k].fld : e[k].alt, even ? e[k].alt : e[k].fld, e[k].src,
+                                   P.relax[e[k].q], P.tol[e[k].q], e[k].slot, e[k].q};
+        }
+        if (m == 1) se[1] = se[0];
+        ifl::launch_rb_sweep2d(P, kind, coef, se[0], se[1], m, s, h->tile, h->stream);
+    }
+    for (int k = 0; k < neq; ++k)
+        if (e[k].n % 2 == 1) ifl::launch_copy(P, e[k].fld, e[k].alt, h->stream);
+}
+
+bool fused2d(const ifl_solver *h) { return h->P.dim == 2 && h->cfg.smoother == IFL_SMOOTHER_RBGS; }
+
+// Workgroups of one sweep launch (the records its successor adds up).
+int sweepRecords(const ifl_solver *h) {
+    if (fused2d(h)) return ifl::tile_count(h->P.nx, h->P.ny, h->tile);
+    if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) return ifl::cell_blocks(h->P);
+    return 2 * ifl::rb_blocks(h->P);
+}
+
+void enqueueFinish(ifl_solver *h, int q, int slot) {
+    const int n = h->cfg.n_iter[q];
+    if (n > 0) ifl::launch_smooth_finish(h->P, slot, q, n, sweepRecords(h), h->P.tol[q], h->stream);
+}
+
 void enqueueSmoother(ifl_solver *h, float *fld, const float *coef, const float *src, int q, int slot) {
     const ifl::CaseParams &P = h->P;
-    for (int s = 0; s < h->cfg.n_iter[q]; ++s)
-        ifl::launch_sweep(P, h->cfg.smoother, fld, coef, src, P.relax[q], P.tol[q], slot, q, s, h->stream);
+    const int n = h->cfg.n_iter[q];
+    if (fused2d(h)) {
+        const bool mom = slot < 3;
+        EqRef e{fld, P.alt[slot], src, q, slot, n};
+        enqueueFused2d(h, mom ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR, mom ? nullptr : P.rinv, &e, 1);
+    } else if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) {
+        for (int s = 0; s < n; ++s) {
+            const bool even = (s % 2) == 0;
+            ifl::launch_sweep_jacobi(P, even ? fld : P.alt[slot], even ? P.alt[slot] : fld, coef, src, P.relax[q],
+                                     P.tol[q], slot, s, h->stream);
+        }
+        if (n % 2 == 1) ifl::launch_copy(P, fld, P.alt[slot], h->stream);
+    } else {
+        for (int s = 0; s < n; ++s)
+            ifl::launch_sweep_rb_half(P, fld, coef, src, P.relax[q], P.tol[q], slot, s, h->stream);
+    }
+    enqueueFinish(h, q, slot);
+}
+
+void enqueueMomentumSmoothers(ifl_solver *h) {
+    const ifl::CaseParams &P = h->P;
+    if (fused2d(h)) {                                                 // u and v share the matrix
+        EqRef e[2] = {{P.u, P.alt[0], P.bu, 0, 0, h->cfg.n_iter[0]}, {P.v, P.alt[1], P.bv, 1, 1, h->cfg.n_iter[1]}};
+        enqueueFused2d(h, ifl::COEF_MOMENTUM, nullptr, e, 2);
+        enqueueFinish(h, 0, 0);
+        enqueueFinish(h, 1, 1);
+        return;
+    }
+    enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0);                     // :178
+    enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1);                     // :180
+    if (P.dim == 3) enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);     // :183
 }
 
 void enqueuePressureCorrection(ifl_solver *h) {
@@ -139,13 +215,12 @@ void enqueuePressureCorrection(ifl_solver *h) {
 void enqueueOuter(ifl_solver *h) {
     const ifl::CaseParams &P = h->P;
     ifl::launch_mom_assemble(P, h->stream);                           // :165-175
-    enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0);                     // :178
-    enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1);                     // :180
-    if (P.dim == 3) enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);     // :183
+    enqueueMomentumSmoothers(h);                                      // :178-184
     ifl::launch_rhie_chow(P, h->stream);                              // :186-187
     ifl::launch_pcorr_assemble(P, h->stream);                         // :189-193
     enqueuePressureCorrection(h);                                     // :196
-    ifl::launch_update(P, 1, h->stream);                              // :198-268
+    ifl::launch_update(P, h->stream);                                 // :198
+    ifl::launch_reduce_cells(P, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH, h->stream);  // :200-268
 }
 
 int buildGraph(ifl_solver *h) {
@@ -267,10 +342,12 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     if (!h) return fail(IFL_ERR_NOMEM, "out of host memory");
     h->cfg = *cfg;
     fillParams(h);
+    h->tile = (cfg->dim == 2) ? (cfg->tile ? cfg->tile : ifl::default_tile(cfg->nx, cfg->ny)) : 0;
     ifl::CaseParams &P = h->P;
     const size_t N = (size_t)P.N, nx = P.nx, ny = P.ny, nz = P.nz;
     const size_t ncoef = N * (1 + 2 * P.dim);
-    const unsigned nblk = maxBlocks(P);
+    const size_t pstride = (size_t)partStride(P), ncb = (size_t)ifl::cell_blocks(P);
+    P.partStride = (int)pstride;
     // arena layout
     struct Part { void **dst; size_t bytes; };
     float *temp = nullptr;
@@ -281,8 +358,11 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
         {(void **)&P.wf, nx * ny * (nz + 1) * 4},
         {(void **)&P.mcoef, ncoef * 4}, {(void **)&P.pcoef, ncoef * 4},
         {(void **)&P.bu, N * 4}, {(void **)&P.bv, N * 4}, {(void **)&P.bw, N * 4}, {(void **)&P.bp, N * 4},
-        {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4}, {(void **)&P.jac, N * 4},
-        {(void **)&P.partials, (size_t)nblk * ifl::kPart * 8},
+        {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
+        {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
+        {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
+        {(void **)&P.spart, (size_t)ifl::kSlots * 2 * pstride * 8},
+        {(void **)&P.partMom, 3 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
         {(void **)&P.st, sizeof(ifl::CaseState)},
         {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
     };
@@ -447,7 +527,10 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
     switch (stage) {
         case IFL_STAGE_INIT_FACES: ifl::launch_init_faces(P, h->stream); break;
         case IFL_STAGE_FACE_BC: ifl::launch_face_bc(P, h->stream); break;
-        case IFL_STAGE_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
+        case IFL_STAGE_MOM_ASSEMBLE:
+            ifl::launch_mom_assemble(P, h->stream);
+            ifl::launch_reduce_cells(P, ifl::RED_MOM, h->stream);
+            break;
         case IFL_STAGE_MOM_RESIDUAL: break;
         case IFL_STAGE_SMOOTH_U: enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0); break;
         case IFL_STAGE_SMOOTH_V: enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1); break;
@@ -456,9 +539,15 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
             enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);
             break;
         case IFL_STAGE_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
-        case IFL_STAGE_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(P, h->stream); break;
+        case IFL_STAGE_PCORR_ASSEMBLE:
+            ifl::launch_pcorr_assemble(P, h->stream);
+            ifl::launch_reduce_cells(P, ifl::RED_MASS, h->stream);
+            break;
         case IFL_STAGE_SMOOTH_P: enqueuePressureCorrection(h); break;
-        case IFL_STAGE_UPDATE: ifl::launch_update(P, 0, h->stream); break;
+        case IFL_STAGE_UPDATE:
+            ifl::launch_update(P, h->stream);
+            ifl::launch_reduce_cells(P, ifl::RED_UPD, h->stream);
+            break;
         case IFL_STAGE_OUTER:
             if (h->iter == 0) ifl::launch_init_faces(P, h->stream);
             enqueueOuter(h);
@@ -509,17 +598,22 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
     auto once = [&]() {
         switch (kernel) {
             case IFL_KERNEL_SWEEP:
-                ifl::launch_sweep(P, h->cfg.smoother, P.pc, P.pcoef, P.bp, P.relax[3], P.tol[3], 3, 3, 0, h->stream);
+                if (fused2d(h)) {
+                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3};
+                    ifl::launch_rb_sweep2d(P, ifl::COEF_PCORR, P.rinv, e, e, 1, 0, h->tile, h->stream);
+                } else {
+                    ifl::launch_sweep_rb_half(P, P.pc, P.pcoef, P.bp, P.relax[3], P.tol[3], 3, 0, h->stream);
+                }
                 break;
             case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
             case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
             case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(P, h->stream); break;
-            case IFL_KERNEL_UPDATE: ifl::launch_update(P, 0, h->stream); break;
+            case IFL_KERNEL_UPDATE: ifl::launch_update(P, h->stream); break;
             default: break;
         }
     };
     if (kernel < 0 || kernel >= IFL_KERNEL_COUNT) return fail(IFL_ERR_ARG, "bad kernel id");
-    if (kernel == IFL_KERNEL_SWEEP) per = 2;
+    if (kernel == IFL_KERNEL_SWEEP && !(fused2d(h))) per = 2;
     once();   // warm-up
     IFL_HIP(hipGetLastError());
     IFL_HIP(hipEventRecord(h->ev0, h->stream));

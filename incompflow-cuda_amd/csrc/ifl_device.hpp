@@ -1,8 +1,16 @@
 // ifl_device.hpp - device-side data structures and helpers of the gfx950 SIMPLE solver.
 //
-// One case (grid) is described by CaseParams (passed by value to every kernel, or read from a
-// device table in the batched path) and owns one CaseState in device memory that carries the
-// FP64 reduction results, the smoother stop flags and the outer-iteration monitor.
+// One case (grid) is described by CaseParams (passed by value to every kernel) and owns one
+// CaseState in device memory that carries the FP64 reduction results, the smoother stop state
+// and the outer-iteration monitor.
+//
+// Reductions never synchronise workgroups inside a kernel (an agent-scope fence per workgroup
+// writes back the XCD's L2 on gfx950).  Each workgroup stores its FP64 partial sums with plain
+// stores; a later kernel (after the kernel boundary has made them visible) adds them up in a fixed
+// order, so every total is deterministic:
+//   * smoother sweeps: every workgroup of sweep s re-adds the partials of sweep s-1 to decide,
+//     identically in all workgroups, whether sweep s still runs (Solver.cu:955-962);
+//   * assembly/update sums: one single-workgroup kernel per outer iteration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,23 +22,25 @@ namespace ifl {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kPart = 8;               // FP64 partial sums per workgroup
 constexpr int kSlots = 5;              // smoother call slots: u, v, w, p', p' refinement
+constexpr int kRows = 4;               // rows per thread in the cell kernels (64 x 16 cells per workgroup)
 
 enum { E = 0, W = 1, N = 2, S = 3, T = 4, B = 5 };                // constants.h:7-12
 enum { sC = 0, sE = 1, sW = 2, sN = 3, sS = 4, sT = 5, sB = 6 };  // constants.h:14-20
 
+// State of one smoother call after sweep k, double-buffered by the parity of k so that the
+// workgroups of sweep k+1 can read the state after sweep k-1 while workgroup 0 publishes k.
 struct SmoothState {
-    double maxNorm;   // running max of the sweep RMS increment (Solver.cu:941, 957)
-    double redSum;    // red half-sweep sum of squared increments
-    double lastSum;   // last full-sweep sum
-    int sweeps;       // sweeps executed in the current call
-    int stop;         // relNorm < tol seen: remaining sweeps of the call are skipped
+    double maxNorm[2];  // running max of the sweep RMS increment (Solver.cu:941, 957)
+    int stopped[2];     // relNorm < tol seen (remaining sweeps of the call are skipped)
+    double lastSum;     // sum of squared increments of the last sweep that ran
+    int sweeps;         // sweeps that ran in the current call
+    int pad;
 };
 
 struct CaseState {
-    unsigned counter;       // last-workgroup counter of the running reduction
     int done;               // 0 running, 1 converged, 2 diverged: every kernel returns at once
+    int pad0;
     long long iter;         // outer iterations executed
     SmoothState sm[kSlots];
     int sweepsIter[4];      // sweeps executed in the current outer iteration (u, v, w, p')
@@ -46,7 +56,7 @@ struct CaseState {
 
 struct CaseParams {
     int dim, nx, ny, nz;
-    long long N;
+    int N;                   // cells (< 2^28, checked on the host)
     float dx, dy, dz, Ax, Ay, Az, rho, mu;
     int bct[6];
     float bcv[6][3];
@@ -59,33 +69,30 @@ struct CaseParams {
     float *u, *v, *w, *p, *pc, *uf, *vf, *wf;
     float *mcoef, *pcoef, *bu, *bv, *bw, *bp;
     float *tmp, *tmp2;       // p' defect and refinement correction
-    float *jac;              // Jacobi destination
-    double *partials;        // kPart doubles per workgroup
+    float *alt[5];           // ping-pong partners of u, v, w, p', refinement correction
+    float *rinv;             // 1/aC of the momentum matrix (p' coefficients are rebuilt from it)
+    double *spart;           // sweep partials: [slot][parity][partStride]
+    double *partMom;         // 3 per cell workgroup: momentum residual sums
+    double *partMass;        // 1 per cell workgroup: b'^2
+    double *partUpd;         // 7 per cell workgroup: update increments
+    int partStride;          // doubles per (slot, parity) sweep-partial array
     CaseState *st;
     ifl_iter_record *hist;   // ring of histCap records indexed by iter % histCap
 };
 
 // ---------------- indexing (kernels.cuh:30-38, 105-114) ----------------
-__device__ __forceinline__ long long cid(const CaseParams &P, int i, int j, int k) {
-    return i + (long long)P.nx * (j + (long long)P.ny * k);
-}
-__device__ __forceinline__ long long ufi(const CaseParams &P, int i, int j, int k) {
-    return i + (long long)(P.nx + 1) * (j + (long long)P.ny * k);
-}
-__device__ __forceinline__ long long vfi(const CaseParams &P, int i, int j, int k) {
-    return i + (long long)P.nx * (j + (long long)(P.ny + 1) * k);
-}
-__device__ __forceinline__ long long wfi(const CaseParams &P, int i, int j, int k) {
-    return i + (long long)P.nx * (j + (long long)P.ny * k);
-}
+__device__ __forceinline__ int cid(const CaseParams &P, int i, int j, int k) { return i + P.nx * (j + P.ny * k); }
+__device__ __forceinline__ int ufi(const CaseParams &P, int i, int j, int k) { return i + (P.nx + 1) * (j + P.ny * k); }
+__device__ __forceinline__ int vfi(const CaseParams &P, int i, int j, int k) { return i + P.nx * (j + (P.ny + 1) * k); }
+__device__ __forceinline__ int wfi(const CaseParams &P, int i, int j, int k) { return i + P.nx * (j + P.ny * k); }
 
 // velBCs::val[f][comp] (config.h:35-42) or the per-face profile of the normal component.
-__device__ __forceinline__ float bcval(const CaseParams &P, int f, int comp, long long q) {
+__device__ __forceinline__ float bcval(const CaseParams &P, int f, int comp, int q) {
     if (comp == (f >> 1) && P.prof[f] != nullptr) return P.prof[f][q];
     return P.bcv[f][comp];
 }
 // Boundary face velocity, kernels.cuh:87-93: wall 0, inlet value, outlet adjacent cell.
-__device__ __forceinline__ float faceBC(const CaseParams &P, int f, long long q, float adjacent) {
+__device__ __forceinline__ float faceBC(const CaseParams &P, int f, int q, float adjacent) {
     int t = P.bct[f];
     if (t == IFL_WALL) return 0.0f;
     if (t == IFL_INLET) return bcval(P, f, f >> 1, q);
@@ -104,73 +111,92 @@ __device__ __forceinline__ double waveSum(double x) {
     return x;
 }
 
-// Reduce K per-thread values over the workgroup; thread t < K returns the total of value t.
+// Reduce K per-thread values over the workgroup (all threads must call); every thread returns
+// the K totals.
 template <int K>
-__device__ __forceinline__ double blockSum(const double (&v)[K], double *lds /* kWaves*K */) {
+__device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {
+    __shared__ double lds[kWaves][K];
+    __shared__ double bc[K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
         double w = waveSum(v[s]);
-        if (lane == 0) lds[wave * K + s] = w;
+        if (lane == 0) lds[wave][s] = w;
     }
     __syncthreads();
-    double r = 0.0;
     if (threadIdx.x < K) {
-        r = lds[threadIdx.x];
+        double r = lds[0][threadIdx.x];
 #pragma unroll
-        for (int q = 1; q < kWaves; ++q) r += lds[q * K + threadIdx.x];
+        for (int q = 1; q < kWaves; ++q) r += lds[q][threadIdx.x];
+        bc[threadIdx.x] = r;
     }
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ void storeAgent(double *p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double loadAgent(const double *p) {
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publish this workgroup's K partials and return true in the last workgroup to finish.
-template <int K>
-__device__ __forceinline__ bool publishPartials(const CaseParams &P, const double (&v)[K], long long blk,
-                                                long long nblk, double *lds) {
-    double tot = blockSum<K>(v, lds);
-    if (threadIdx.x < K) storeAgent(&P.partials[blk * kPart + threadIdx.x], tot);
-    __shared__ int amLast;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned prev = __hip_atomic_fetch_add(&P.st->counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        amLast = (prev == (unsigned)(nblk - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    if (amLast) __threadfence();
-    return amLast != 0;
-}
-
-// In the last workgroup: total of partial s over all nblk workgroups, in a fixed order
-// (thread t adds workgroups t, t+256, ... in sequence; the 256 results are combined by the
-// fixed wave/LDS tree), so the result does not depend on which workgroup finished last.
-template <int K>
-__device__ __forceinline__ void gatherPartials(const CaseParams &P, long long nblk, double (&out)[K], double *lds) {
-    double acc[K];
-#pragma unroll
-    for (int s = 0; s < K; ++s) acc[s] = 0.0;
-    for (long long b = threadIdx.x; b < nblk; b += kBlock)
-#pragma unroll
-        for (int s = 0; s < K; ++s) acc[s] += loadAgent(&P.partials[b * kPart + s]);
-    double tot = blockSum<K>(acc, lds);
-    __shared__ double bc[K];
-    if (threadIdx.x < K) bc[threadIdx.x] = tot;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < K; ++s) out[s] = bc[s];
-    if (threadIdx.x == 0) __hip_atomic_store(&P.st->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
 }
 
-__device__ __forceinline__ int caseDone(const CaseParams &P) {
-    return __hip_atomic_load(&P.st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Store this workgroup's K totals at out[0..K).
+template <int K>
+__device__ __forceinline__ void storeBlockSums(const double (&v)[K], double *out) {
+    double tot[K];
+    blockSum<K>(v, tot);
+    if (threadIdx.x < K) out[threadIdx.x] = tot[threadIdx.x];
+}
+
+// Totals of n partial records of K doubles (record b at part[b*K]); thread t adds records
+// t, t+256, ... in order, then the fixed workgroup tree: the same result in every workgroup.
+template <int K>
+__device__ __forceinline__ void sumPartials(const double *part, int n, double (&out)[K]) {
+    double acc[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) acc[s] = 0.0;
+    for (int b = threadIdx.x; b < n; b += kBlock)
+#pragma unroll
+        for (int s = 0; s < K; ++s) acc[s] += part[b * K + s];
+    blockSum<K>(acc, out);
+}
+
+__device__ __forceinline__ int caseDone(const CaseParams &P) { return P.st->done; }
+
+__device__ __forceinline__ int flatBlock() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ __forceinline__ bool leader() { return flatBlock() == 0 && threadIdx.x == 0; }
+
+__device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int sweep) {
+    return P.spart + (size_t)(slot * 2 + (sweep & 1)) * P.partStride;
+}
+
+// Whether sweep `sweep` of the smoother call in `slot` runs.  Every workgroup evaluates the
+// relative norm of sweep-1 from its partials (nprev records) exactly as the host loop of
+// Solver.cu:955-962 does after each sweep (in FP64); workgroup 0 publishes the state after
+// sweep-1.  With sweep == nIter this closes the call (sweeps count, last sum).
+__device__ __forceinline__ bool sweepRuns(const CaseParams &P, int slot, int sweep, int nprev, double tol) {
+    if (sweep == 0) return true;
+    SmoothState &c = P.st->sm[slot];
+    const int pp = sweep & 1;                      // parity of the state after sweep-2
+    const bool stoppedBefore = (sweep >= 2) && c.stopped[pp];
+    const double mxPrev = (sweep >= 2) ? c.maxNorm[pp] : -1e20;
+    bool stopped = true;
+    double mx = mxPrev, T = 0.0;
+    if (!stoppedBefore) {
+        double t[1];
+        sumPartials<1>(sweepPart(P, slot, sweep - 1), nprev, t);
+        T = t[0];
+        const double norm = sqrt(T / (double)P.N);
+        mx = fmax(norm, mxPrev);
+        const double rel = norm / (mx + 1e-20);
+        stopped = rel < tol;
+    }
+    if (leader()) {
+        const int q = (sweep - 1) & 1;
+        c.stopped[q] = stopped ? 1 : 0;
+        c.maxNorm[q] = mx;
+        if (!stoppedBefore) {
+            c.lastSum = T;
+            c.sweeps = sweep;
+        }
+    }
+    return !stopped;
 }
 
 }  // namespace ifl

@@ -1,61 +1,65 @@
 // kernels.hip - gfx950 kernels of one SIMPLE outer iteration on a collocated structured grid.
 //
-// Every kernel is one pass over the cells of a case (one thread per cell, 256-thread
-// workgroups of 4 wave64), does all of a cell's work for its stage in one thread (so the
-// boundary rows that the reference modifies from four concurrent streams, Solver.cu:583-597,
-// are applied race-free and in launch order E,W,N,S,T,B), and reduces its FP64 norms with
-// wave64 butterflies plus a last-workgroup gather in a fixed order (deterministic; replaces
-// the float atomicAdd of kernels.cuh:578 etc.).  Arithmetic follows kernels.cuh expression by
-// expression, with the same float/double promotions, and is compiled with -ffp-contract=off,
-// so a stage's output equals the FP32 CPU oracle's bit for bit.
+// Cell kernels: a workgroup of 256 threads (4 wave64) covers 64 cells along x and 16 rows
+// (4 rows per thread), so every wave reads and writes 256 contiguous bytes per row.  Each cell
+// is done by one thread for all of its stage's work, so the boundary rows that the reference
+// modifies from four concurrent streams (Solver.cu:583-597) are applied race-free and in launch
+// order E,W,N,S,T,B.  FP64 norms are reduced with wave64 butterflies into one record per
+// workgroup, and added up in a fixed order by k_reduce_cells (deterministic; replaces the float
+// atomicAdd of kernels.cuh:578 etc.).  Arithmetic follows kernels.cuh expression by expression,
+// with the same float/double promotions, and is compiled with -ffp-contract=off, so a stage's
+// output equals the FP32 CPU oracle's bit for bit.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
 
 namespace ifl {
 
-__device__ __forceinline__ void cellOf(const CaseParams &P, long long t, int &i, int &j, int &k) {
-    long long r = t / P.nx;
-    i = (int)(t - r * P.nx);
-    k = (int)(r / P.ny);
-    j = (int)(r - (long long)k * P.ny);
+// Visit the cells of this workgroup: f(i, j, k, c).
+template <class F>
+__device__ __forceinline__ void forCells(const CaseParams &P, F &&f) {
+    const int rows = P.ny * P.nz;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+        const int r = r0 + 4 * q;
+        if (i < P.nx && r < rows) {
+            const int k = r / P.ny;
+            const int j = r - k * P.ny;
+            f(i, j, k, i + P.nx * r);
+        }
+    }
 }
 
 // ---------------- initFaceVel, kernels.cuh:10-45 ----------------
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_init_faces(CaseParams P) {
-    long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= P.N) return;
-    int i, j, k;
-    cellOf(P, t, i, j, k);
-    long long c = t;
-    if (i >= 1) P.uf[ufi(P, i, j, k)] = (P.u[c - 1] + P.u[c]) / 2;
-    if (j >= 1) P.vf[vfi(P, i, j, k)] = (P.v[c - P.nx] + P.v[c]) / 2;
-    if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = (P.w[c - (long long)P.nx * P.ny] + P.w[c]) / 2;
+    forCells(P, [&](int i, int j, int k, int c) {
+        if (i >= 1) P.uf[ufi(P, i, j, k)] = (P.u[c - 1] + P.u[c]) / 2;
+        if (j >= 1) P.vf[vfi(P, i, j, k)] = (P.v[c - P.nx] + P.v[c]) / 2;
+        if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = (P.w[c - P.nx * P.ny] + P.w[c]) / 2;
+    });
 }
 
 // ---------------- applyBCsToFaceVel, kernels.cuh:47-95 ----------------
 template <int DIM>
-__device__ __forceinline__ void applyFaceBCs(const CaseParams &P, int i, int j, int k, long long c) {
+__device__ __forceinline__ void applyFaceBCs(const CaseParams &P, int i, int j, int k, int c) {
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    if (i == 0) P.uf[ufi(P, 0, j, k)] = faceBC(P, W, j + (long long)ny * k, P.u[c]);
-    if (i == nx - 1) P.uf[ufi(P, nx, j, k)] = faceBC(P, E, j + (long long)ny * k, P.u[c]);
-    if (j == 0) P.vf[vfi(P, i, 0, k)] = faceBC(P, S, i + (long long)nx * k, P.v[c]);
-    if (j == ny - 1) P.vf[vfi(P, i, ny, k)] = faceBC(P, N, i + (long long)nx * k, P.v[c]);
+    if (i == 0) P.uf[ufi(P, 0, j, k)] = faceBC(P, W, j + ny * k, P.u[c]);
+    if (i == nx - 1) P.uf[ufi(P, nx, j, k)] = faceBC(P, E, j + ny * k, P.u[c]);
+    if (j == 0) P.vf[vfi(P, i, 0, k)] = faceBC(P, S, i + nx * k, P.v[c]);
+    if (j == ny - 1) P.vf[vfi(P, i, ny, k)] = faceBC(P, N, i + nx * k, P.v[c]);
     if (DIM == 3) {
-        if (k == 0) P.wf[wfi(P, i, j, 0)] = faceBC(P, B, i + (long long)nx * j, P.w[c]);
-        if (k == nz - 1) P.wf[wfi(P, i, j, nz)] = faceBC(P, T, i + (long long)nx * j, P.w[c]);
+        if (k == 0) P.wf[wfi(P, i, j, 0)] = faceBC(P, B, i + nx * j, P.w[c]);
+        if (k == nz - 1) P.wf[wfi(P, i, j, nz)] = faceBC(P, T, i + nx * j, P.w[c]);
     }
 }
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_face_bc(CaseParams P) {
     if (caseDone(P)) return;
-    long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= P.N) return;
-    int i, j, k;
-    cellOf(P, t, i, j, k);
-    applyFaceBCs<DIM>(P, i, j, k, t);
+    forCells(P, [&](int i, int j, int k, int c) { applyFaceBCs<DIM>(P, i, j, k, c); });
 }
 
 // ---------------- momentum assembly ----------------
@@ -64,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_face_bc(CaseParams P) {
 // start-of-iteration face BCs (Solver.cu:165) applied first.  One coefficient set serves u, v, w
 // (the reference's copies are bitwise equal, Solver.cu:167-171).  Also accumulates the FP64
 // momentum residual sum (b - A phi)^2 of each component before the sweeps.
-__device__ __forceinline__ float momSrc(const CaseParams &P, int d, int idx, int n, long long c, long long st) {
+__device__ __forceinline__ float momSrc(const CaseParams &P, int d, int idx, int n, int c, int st) {
     const int lo = 2 * d + 1, hi = 2 * d;
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
     float p_L, p_R;
@@ -82,7 +86,7 @@ __device__ __forceinline__ float momSrc(const CaseParams &P, int d, int idx, int
 }
 
 template <int DIM>
-__device__ __forceinline__ void bcRow(const CaseParams &P, int f, long long q, float (&a)[7], float (&b)[3]) {
+__device__ __forceinline__ void bcRow(const CaseParams &P, int f, int q, float (&a)[7], float (&b)[3]) {
     const float aEx = a[bcExSlot(f)];
     if (ghostCopy(P, f)) {                       // kernels.cuh:307, 312, 318
 #pragma unroll
@@ -93,8 +97,8 @@ __device__ __forceinline__ void bcRow(const CaseParams &P, int f, long long q, f
 
 template <int DIM>
 __device__ __forceinline__ double momResidual(const CaseParams &P, const float *fld, const float (&a)[7], float b,
-                                              int i, int j, int k, long long c) {
-    const long long pl = (long long)P.nx * P.ny;
+                                              int i, int j, int k, int c) {
+    const int pl = P.nx * P.ny;
     double r = (double)b;
     r -= (double)a[sC] * (double)fld[c];
     r -= (double)a[sE] * (double)((i != P.nx - 1) ? fld[c + 1] : 0.0f);
@@ -110,72 +114,60 @@ __device__ __forceinline__ double momResidual(const CaseParams &P, const float *
 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P) {
-    __shared__ double lds[kWaves * 3];
     if (caseDone(P)) return;
-    const long long nblk = (P.N + kBlock - 1) / kBlock;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (leader())
+        for (int q = 0; q < 4; ++q) P.st->sweepsIter[q] = 0;   // a new outer iteration starts
     double rs[3] = {0.0, 0.0, 0.0};
-    if (t < P.N) {
-        int i, j, k;
-        cellOf(P, t, i, j, k);
-        const long long c = t;
+    forCells(P, [&](int i, int j, int k, int c) {
         const int nx = P.nx, ny = P.ny, nz = P.nz;
-        const long long pl = (long long)nx * ny;
         applyFaceBCs<DIM>(P, i, j, k, c);
         // face velocities (boundary faces hold the values just applied)
-        const float ue = (i == nx - 1) ? faceBC(P, E, j + (long long)ny * k, P.u[c]) : P.uf[ufi(P, i + 1, j, k)];
-        const float uw = (i == 0) ? faceBC(P, W, j + (long long)ny * k, P.u[c]) : P.uf[ufi(P, i, j, k)];
-        const float vn = (j == ny - 1) ? faceBC(P, N, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j + 1, k)];
-        const float vs = (j == 0) ? faceBC(P, S, i + (long long)nx * k, P.v[c]) : P.vf[vfi(P, i, j, k)];
-        const float wt = DIM == 3 ? ((k == nz - 1) ? faceBC(P, T, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)]) : 0.0f;
-        const float wb = DIM == 3 ? ((k == 0) ? faceBC(P, B, i + (long long)nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)]) : 0.0f;
+        const float ue = (i == nx - 1) ? faceBC(P, E, j + ny * k, P.u[c]) : P.uf[ufi(P, i + 1, j, k)];
+        const float uw = (i == 0) ? faceBC(P, W, j + ny * k, P.u[c]) : P.uf[ufi(P, i, j, k)];
+        const float vn = (j == ny - 1) ? faceBC(P, N, i + nx * k, P.v[c]) : P.vf[vfi(P, i, j + 1, k)];
+        const float vs = (j == 0) ? faceBC(P, S, i + nx * k, P.v[c]) : P.vf[vfi(P, i, j, k)];
+        const float wt = DIM == 3 ? ((k == nz - 1) ? faceBC(P, T, i + nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)]) : 0.0f;
+        const float wb = DIM == 3 ? ((k == 0) ? faceBC(P, B, i + nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)]) : 0.0f;
         float a[7];
         momLinkCoef<DIM>(P, ue, uw, vn, vs, wt, wb, a);
         float b[3];
         b[0] = momSrc(P, 0, i, nx, c, 1);
         b[1] = momSrc(P, 1, j, ny, c, nx);
-        b[2] = DIM == 3 ? momSrc(P, 2, k, nz, c, pl) : 0.0f;
+        b[2] = DIM == 3 ? momSrc(P, 2, k, nz, c, nx * ny) : 0.0f;
         // BC rows in launch order E, W, N, S, T, B (Solver.cu:583-608)
-        if (i == nx - 1) bcRow<DIM>(P, E, j + (long long)ny * k, a, b);
-        if (i == 0) bcRow<DIM>(P, W, j + (long long)ny * k, a, b);
-        if (j == ny - 1) bcRow<DIM>(P, N, i + (long long)nx * k, a, b);
-        if (j == 0) bcRow<DIM>(P, S, i + (long long)nx * k, a, b);
+        if (i == nx - 1) bcRow<DIM>(P, E, j + ny * k, a, b);
+        if (i == 0) bcRow<DIM>(P, W, j + ny * k, a, b);
+        if (j == ny - 1) bcRow<DIM>(P, N, i + nx * k, a, b);
+        if (j == 0) bcRow<DIM>(P, S, i + nx * k, a, b);
         if (DIM == 3) {
-            if (k == nz - 1) bcRow<DIM>(P, T, i + (long long)nx * j, a, b);
-            if (k == 0) bcRow<DIM>(P, B, i + (long long)nx * j, a, b);
+            if (k == nz - 1) bcRow<DIM>(P, T, i + nx * j, a, b);
+            if (k == 0) bcRow<DIM>(P, B, i + nx * j, a, b);
         }
 #pragma unroll
         for (int s = 0; s < 1 + 2 * DIM; ++s) P.mcoef[c + P.N * s] = a[s];
         P.bu[c] = b[0];
         P.bv[c] = b[1];
         if (DIM == 3) P.bw[c] = b[2];
-        double r0 = momResidual<DIM>(P, P.u, a, b[0], i, j, k, c);
-        double r1 = momResidual<DIM>(P, P.v, a, b[1], i, j, k, c);
-        rs[0] = r0 * r0;
-        rs[1] = r1 * r1;
+        const double r0 = momResidual<DIM>(P, P.u, a, b[0], i, j, k, c);
+        const double r1 = momResidual<DIM>(P, P.v, a, b[1], i, j, k, c);
+        rs[0] += r0 * r0;
+        rs[1] += r1 * r1;
         if (DIM == 3) {
-            double r2 = momResidual<DIM>(P, P.w, a, b[2], i, j, k, c);
-            rs[2] = r2 * r2;
+            const double r2 = momResidual<DIM>(P, P.w, a, b[2], i, j, k, c);
+            rs[2] += r2 * r2;
         }
-    }
-    if (publishPartials<3>(P, rs, blockIdx.x, nblk, lds)) {
-        double tot[3];
-        gatherPartials<3>(P, nblk, tot, lds);
-        if (threadIdx.x == 0) {
-            for (int q = 0; q < 3; ++q) P.st->resSq[q] = tot[q];
-            for (int q = 0; q < 4; ++q) P.st->sweepsIter[q] = 0;
-        }
-    }
+    });
+    storeBlockSums<3>(rs, P.partMom + 3 * flatBlock());
 }
 
-// ---------------- red-black Gauss-Seidel half sweep ----------------
+// ---------------- red-black Gauss-Seidel half sweep (3D) ----------------
 // GaussSeidelIterateKernel (kernels.cuh:926-1005) restricted to cells of one colour
 // ((i+j+k)&1 == colour), so each half sweep is order independent and the full sweep
 // (red then black) is deterministic.  Neighbours outside the grid are 0 (:939-974).
 template <int DIM>
 __device__ __forceinline__ float gsUpdate(const CaseParams &P, const float *fld, const float *coef, const float *src,
-                                          int i, int j, int k, long long c, float rlx, float &dPhi) {
-    const long long N = P.N, pl = (long long)P.nx * P.ny;
+                                          int i, int j, int k, int c, float rlx, float &dPhi) {
+    const int N = P.N, pl = P.nx * P.ny;
     const float phi_C = fld[c];
     const float phi_E = (i != P.nx - 1) ? fld[c + 1] : 0.0f;
     const float phi_W = (i != 0) ? fld[c - 1] : 0.0f;
@@ -193,94 +185,73 @@ __device__ __forceinline__ float gsUpdate(const CaseParams &P, const float *fld,
     return phi_C + dPhi;
 }
 
-// Full-sweep norm bookkeeping, Solver.cu:955-962 in FP64.
-__device__ __forceinline__ void finishSweep(const CaseParams &P, int slot, int eq, int sweep, double sum, double tol) {
-    SmoothState &sm = P.st->sm[slot];
-    double norm = sqrt(sum / (double)P.N);
-    double mx = sweep == 0 ? -1e20 : sm.maxNorm;
-    mx = fmax(norm, mx);
-    double rel = norm / (mx + 1e-20);
-    sm.maxNorm = mx;
-    sm.lastSum = sum;
-    sm.sweeps = sweep + 1;
-    sm.stop = (rel < tol) ? 1 : 0;
-    P.st->sweepsIter[eq] += 1;
-}
-
+// Grid: x over (nx+1)/2 colour cells of a row (64 per workgroup), y over rows (16 per workgroup).
+// Partials of the red launch occupy records [0, nb), the black launch [nb, 2nb).
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_sweep_rb(CaseParams P, float *fld, const float *coef, const float *src,
-                                                     float rlx, double tol, int slot, int eq, int sweep, int color) {
-    __shared__ double lds[kWaves];
+                                                     float rlx, double tol, int slot, int sweep, int color) {
     if (caseDone(P)) return;
-    if (sweep > 0 && P.st->sm[slot].stop) return;
-    const int hx = (P.nx + 1) >> 1;
-    const long long rows = (long long)P.ny * P.nz;
-    const long long nblk = (hx * rows + kBlock - 1) / kBlock;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int nb = gridDim.x * gridDim.y;
+    if (!sweepRuns(P, slot, sweep, 2 * nb, tol)) return;
+    const int rows = P.ny * P.nz;
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
     double s[1] = {0.0};
-    const long long r = t / hx;
-    if (r < rows) {
-        const int m = (int)(t - r * hx);
-        const int k = (int)(r / P.ny);
-        const int j = (int)(r - (long long)k * P.ny);
-        const int i = 2 * m + ((j + k + color) & 1);
-        if (i < P.nx) {
-            const long long c = i + (long long)P.nx * r;
-            float d;
-            fld[c] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, c, rlx, d);
-            s[0] = (double)d * (double)d;
+#pragma unroll
+    for (int q = 0; q < kRows; ++q) {
+        const int r = r0 + 4 * q;
+        if (r < rows) {
+            const int k = r / P.ny;
+            const int j = r - k * P.ny;
+            const int i = 2 * m + ((j + k + color) & 1);
+            if (i < P.nx) {
+                const int c = i + P.nx * r;
+                float d;
+                fld[c] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, c, rlx, d);
+                s[0] += (double)d * (double)d;
+            }
         }
     }
-    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
-        double tot[1];
-        gatherPartials<1>(P, nblk, tot, lds);
-        if (threadIdx.x == 0) {
-            if (color == 0) P.st->sm[slot].redSum = tot[0];
-            else finishSweep(P, slot, eq, sweep, P.st->sm[slot].redSum + tot[0], tol);
-        }
-    }
+    storeBlockSums<1>(s, sweepPart(P, slot, sweep) + color * nb + flatBlock());
 }
 
 // ---------------- point Jacobi sweep, kernels.cuh:845-924 ----------------
-// Reads fld, writes dst; k_copy_back then moves dst into fld when the sweep ran (the
-// reference's pointer swap, Solver.cu:894-896, with the result always in the caller's field).
+// Reads `in`, writes `out` (ping-pong; a sweep that does not run copies), so after an even number
+// of launches the result is in the caller's field and an odd count ends with a copy back:
+// the reference's pointer swap (Solver.cu:894-896) with the result always in the caller's field.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const float *fld, float *dst, const float *coef,
-                                                         const float *src, float rlx, double tol, int slot, int eq,
-                                                         int sweep) {
-    __shared__ double lds[kWaves];
+__global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const float *in, float *out, const float *coef,
+                                                         const float *src, float rlx, double tol, int slot, int sweep) {
     if (caseDone(P)) return;
-    if (sweep > 0 && P.st->sm[slot].stop) return;
-    const long long nblk = (P.N + kBlock - 1) / kBlock;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int nb = gridDim.x * gridDim.y;
+    const bool run = sweepRuns(P, slot, sweep, nb, tol);
     double s[1] = {0.0};
-    if (t < P.N) {
-        int i, j, k;
-        cellOf(P, t, i, j, k);
-        float d;
-        dst[t] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, t, rlx, d);
-        s[0] = (double)d * (double)d;
-    }
-    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
-        double tot[1];
-        gatherPartials<1>(P, nblk, tot, lds);
-        if (threadIdx.x == 0) finishSweep(P, slot, eq, sweep, tot[0], tol);
-    }
+    forCells(P, [&](int i, int j, int k, int c) {
+        if (run) {
+            float d;
+            out[c] = gsUpdate<DIM>(P, in, coef, src, i, j, k, c, rlx, d);
+            s[0] += (double)d * (double)d;
+        } else {
+            out[c] = in[c];
+        }
+    });
+    if (run) storeBlockSums<1>(s, sweepPart(P, slot, sweep) + flatBlock());
 }
 
-__global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P, float *fld, const float *src, int slot, int sweep) {
+// Closes a smoother call of nIter sweeps: publishes the state after the last sweep and adds
+// the sweeps that ran to the outer iteration's count (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P, int slot, int eq, int n, int nprev, double tol) {
     if (caseDone(P)) return;
-    if (P.st->sm[slot].sweeps != sweep + 1) return;   // sweep did not run
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t < P.N) fld[t] = src[t];
+    sweepRuns(P, slot, n, nprev, tol);
+    if (threadIdx.x == 0) P.st->sweepsIter[eq] += P.st->sm[slot].sweeps;
 }
 
 // ---------------- Rhie-Chow, kernels.cuh:336-500, then face BCs (Solver.cu:187) ----------------
-__device__ __forceinline__ float rhieChow(const CaseParams &P, int d, long long c, int idx, int n, long long st,
+__device__ __forceinline__ float rhieChow(const CaseParams &P, int d, int c, int idx, int n, int st,
                                           const float *vel, const float *aCp) {
     const int lo = 2 * d + 1, hi = 2 * d;
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
-    const long long LId = c - st, RId = c;
+    const int LId = c - st, RId = c;
     const float aC_L = aCp[LId];
     const float aC_R = aCp[RId];
     const float u_L = vel[LId];
@@ -304,30 +275,22 @@ __device__ __forceinline__ float rhieChow(const CaseParams &P, int d, long long 
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P) {
     if (caseDone(P)) return;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= P.N) return;
-    int i, j, k;
-    cellOf(P, t, i, j, k);
-    const long long c = t;
     const float *aC = P.mcoef;   // plane sC
-    if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
-    if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, c, j, P.ny, P.nx, P.v, aC);
-    if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, c, k, P.nz, (long long)P.nx * P.ny, P.w, aC);
-    applyFaceBCs<DIM>(P, i, j, k, c);
+    forCells(P, [&](int i, int j, int k, int c) {
+        if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
+        if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, c, j, P.ny, P.nx, P.v, aC);
+        if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, c, k, P.nz, P.nx * P.ny, P.w, aC);
+        applyFaceBCs<DIM>(P, i, j, k, c);
+    });
 }
 
 // ---------------- pressure-correction assembly, kernels.cuh:502-581, and p' = 0 ----------------
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
-    __shared__ double lds[kWaves];
     if (caseDone(P)) return;
-    const long long nblk = (P.N + kBlock - 1) / kBlock;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
     double s[1] = {0.0};
-    if (t < P.N) {
-        int i, j, k;
-        cellOf(P, t, i, j, k);
-        const long long c = t, N = P.N, pl = (long long)P.nx * P.ny;
+    forCells(P, [&](int i, int j, int k, int c) {
+        const int N = P.N, pl = P.nx * P.ny;
         const float *uC = P.mcoef;
         const float rho = P.rho, Ax = P.Ax, Ay = P.Ay, Az = P.Az;
         float aE = 0.0f, aW = 0.0f, aN = 0.0f, aS = 0.0f, aT = 0.0f, aB = 0.0f;
@@ -342,6 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
             aCc += -(aT + aB);
         }
         float *pcf = P.pcoef;
+        P.rinv[c] = 1 / uC[c];
         pcf[c] = aCc;
         pcf[c + N * sE] = aE;
         pcf[c + N * sW] = aW;
@@ -356,38 +320,31 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
         if (DIM == 3) b += rho * Az * (P.wf[wfi(P, i, j, k)] - P.wf[wfi(P, i, j, k + 1)]);
         P.bp[c] = b;
         P.pc[c] = 0.0f;                       // Solver.cu:193
-        s[0] = (double)b * (double)b;
-    }
-    if (publishPartials<1>(P, s, blockIdx.x, nblk, lds)) {
-        double tot[1];
-        gatherPartials<1>(P, nblk, tot, lds);
-        if (threadIdx.x == 0) P.st->mass = tot[0];
-    }
+        s[0] += (double)b * (double)b;
+    });
+    storeBlockSums<1>(s, P.partMass + flatBlock());
 }
 
 // ---------------- FP64 defect of the p' system (refinement), delta = 0 ----------------
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P) {
     if (caseDone(P)) return;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= P.N) return;
-    int i, j, k;
-    cellOf(P, t, i, j, k);
-    float a[7];
+    forCells(P, [&](int i, int j, int k, int c) {
+        float a[7];
 #pragma unroll
-    for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[t + P.N * s] : 0.0f;
-    P.tmp[t] = (float)momResidual<DIM>(P, P.pc, a, P.bp[t], i, j, k, t);
-    P.tmp2[t] = 0.0f;
+        for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[c + P.N * s] : 0.0f;
+        P.tmp[c] = (float)momResidual<DIM>(P, P.pc, a, P.bp[c], i, j, k, c);
+        P.tmp2[c] = 0.0f;
+    });
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcorr_accumulate(CaseParams P) {
     if (caseDone(P)) return;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t < P.N) P.pc[t] += P.tmp2[t];
+    forCells(P, [&](int, int, int, int c) { P.pc[c] += P.tmp2[c]; });
 }
 
-// ---------------- updateField, kernels.cuh:583-843, and the outer-iteration monitor ----------------
-__device__ __forceinline__ float velCorr(const CaseParams &P, int d, int idx, int n, long long c, long long st) {
+// ---------------- updateField, kernels.cuh:583-843 ----------------
+__device__ __forceinline__ float velCorr(const CaseParams &P, int d, int idx, int n, int c, int st) {
     const int lo = 2 * d + 1, hi = 2 * d;
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
     float pc_L, pc_R;
@@ -404,16 +361,56 @@ __device__ __forceinline__ float velCorr(const CaseParams &P, int d, int idx, in
     const float aC_C = P.mcoef[c];
     return P.relax[d] * 0.5 * (pc_L - pc_R) * area / aC_C;
 }
-__device__ __forceinline__ float faceCorr(const CaseParams &P, int d, long long c, long long st) {
+__device__ __forceinline__ float faceCorr(const CaseParams &P, int d, int c, int st) {
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
     const float aC_L = P.mcoef[c - st];
     const float aC_R = P.mcoef[c];
     return P.relax[d] * 0.5 * (1 / aC_L + 1 / aC_R) * (P.pc[c - st] - P.pc[c]) * area;
 }
 
-// Solver.cu:226-268 (FP64) + scaled residual; one thread.
-__device__ void finishOuter(const CaseParams &P, const double (&u)[7]) {
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_update(CaseParams P) {
+    if (caseDone(P)) return;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    forCells(P, [&](int i, int j, int k, int c) {
+        const int pl = P.nx * P.ny;
+        const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
+        P.p[c] += dp;
+        s[0] += (double)dp * dp;
+        const float du = velCorr(P, 0, i, P.nx, c, 1);
+        P.u[c] += du;
+        s[1] += (double)du * du;
+        const float dv = velCorr(P, 1, j, P.ny, c, P.nx);
+        P.v[c] += dv;
+        s[2] += (double)dv * dv;
+        if (DIM == 3) {
+            const float dw = velCorr(P, 2, k, P.nz, c, pl);
+            P.w[c] += dw;
+            s[3] += (double)dw * dw;
+        }
+        if (i >= 1) {
+            const float d = faceCorr(P, 0, c, 1);
+            P.uf[ufi(P, i, j, k)] += d;
+            s[4] += (double)d * d;
+        }
+        if (j >= 1) {
+            const float d = faceCorr(P, 1, c, P.nx);
+            P.vf[vfi(P, i, j, k)] += d;
+            s[5] += (double)d * d;
+        }
+        if (DIM == 3 && k >= 1) {
+            const float d = faceCorr(P, 2, c, pl);
+            P.wf[wfi(P, i, j, k)] += d;
+            s[6] += (double)d * d;
+        }
+    });
+    storeBlockSums<7>(s, P.partUpd + 7 * flatBlock());
+}
+
+// ---------------- outer-iteration monitor: Solver.cu:226-268 (FP64) + scaled residual ----------------
+__device__ void finishOuter(const CaseParams &P) {
     CaseState &S = *P.st;
+    const double *u = S.upd;
     const double Nd = (double)P.N;
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     ifl_iter_record rec;
@@ -487,62 +484,45 @@ __device__ void finishOuter(const CaseParams &P, const double (&u)[7]) {
     else if (conv) S.done = 1;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_update(CaseParams P, int finish) {
-    __shared__ double lds[kWaves * 7];
+// One workgroup: totals of the cell-kernel partials (fixed order), then (optionally) the monitor.
+__global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P, int nblk, int mask) {
     if (caseDone(P)) return;
-    const long long nblk = (P.N + kBlock - 1) / kBlock;
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    double s[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (t < P.N) {
-        int i, j, k;
-        cellOf(P, t, i, j, k);
-        const long long c = t, pl = (long long)P.nx * P.ny;
-        const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
-        P.p[c] += dp;
-        s[0] = (double)dp * dp;
-        const float du = velCorr(P, 0, i, P.nx, c, 1);
-        P.u[c] += du;
-        s[1] = (double)du * du;
-        const float dv = velCorr(P, 1, j, P.ny, c, P.nx);
-        P.v[c] += dv;
-        s[2] = (double)dv * dv;
-        if (DIM == 3) {
-            const float dw = velCorr(P, 2, k, P.nz, c, pl);
-            P.w[c] += dw;
-            s[3] = (double)dw * dw;
-        }
-        if (i >= 1) {
-            const float d = faceCorr(P, 0, c, 1);
-            P.uf[ufi(P, i, j, k)] += d;
-            s[4] = (double)d * d;
-        }
-        if (j >= 1) {
-            const float d = faceCorr(P, 1, c, P.nx);
-            P.vf[vfi(P, i, j, k)] += d;
-            s[5] = (double)d * d;
-        }
-        if (DIM == 3 && k >= 1) {
-            const float d = faceCorr(P, 2, c, pl);
-            P.wf[wfi(P, i, j, k)] += d;
-            s[6] = (double)d * d;
-        }
+    CaseState &S = *P.st;
+    if (mask & RED_MOM) {
+        double t[3];
+        sumPartials<3>(P.partMom, nblk, t);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 3; ++q) S.resSq[q] = t[q];
     }
-    if (publishPartials<7>(P, s, blockIdx.x, nblk, lds)) {
-        double tot[7];
-        gatherPartials<7>(P, nblk, tot, lds);
-        if (threadIdx.x == 0) {
-            for (int q = 0; q < 7; ++q) P.st->upd[q] = tot[q];
-            if (finish) finishOuter(P, tot);
-        }
+    if (mask & RED_MASS) {
+        double t[1];
+        sumPartials<1>(P.partMass, nblk, t);
+        if (threadIdx.x == 0) S.mass = t[0];
     }
+    if (mask & RED_UPD) {
+        double t[7];
+        sumPartials<7>(P.partUpd, nblk, t);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 7; ++q) S.upd[q] = t[q];
+    }
+    if ((mask & RED_FINISH) && threadIdx.x == 0) finishOuter(P);
 }
 
 // ---------------- host-side launchers ----------------
-static inline unsigned cellBlocks(const CaseParams &P) { return (unsigned)((P.N + kBlock - 1) / kBlock); }
-static inline unsigned rbBlocks(const CaseParams &P) {
-    long long hx = (P.nx + 1) / 2;
-    return (unsigned)((hx * P.ny * P.nz + kBlock - 1) / kBlock);
+dim3 cellGrid(const CaseParams &P) {
+    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
+}
+int cell_blocks(const CaseParams &P) {
+    dim3 g = cellGrid(P);
+    return (int)(g.x * g.y);
+}
+static dim3 rbGrid(const CaseParams &P) {
+    const int hx = (P.nx + 1) / 2;
+    return dim3((unsigned)((hx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
+}
+int rb_blocks(const CaseParams &P) {
+    dim3 g = rbGrid(P);
+    return (int)(g.x * g.y);
 }
 
 #define IFL_DISPATCH(DIMV, KERNEL, GRID, STREAM, ...)                                   \
@@ -551,30 +531,33 @@ static inline unsigned rbBlocks(const CaseParams &P) {
         else KERNEL<2><<<(GRID), kBlock, 0, (STREAM)>>>(__VA_ARGS__);                  \
     } while (0)
 
-void launch_init_faces(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_init_faces, cellBlocks(P), s, P); }
-void launch_face_bc(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_face_bc, cellBlocks(P), s, P); }
-void launch_mom_assemble(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_mom_assemble, cellBlocks(P), s, P); }
-void launch_rhie_chow(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_rhie_chow, cellBlocks(P), s, P); }
+void launch_init_faces(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_init_faces, cellGrid(P), s, P); }
+void launch_face_bc(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_face_bc, cellGrid(P), s, P); }
+void launch_mom_assemble(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_mom_assemble, cellGrid(P), s, P); }
+void launch_rhie_chow(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_rhie_chow, cellGrid(P), s, P); }
 void launch_pcorr_assemble(const CaseParams &P, hipStream_t s) {
-    IFL_DISPATCH(P.dim, k_pcorr_assemble, cellBlocks(P), s, P);
+    IFL_DISPATCH(P.dim, k_pcorr_assemble, cellGrid(P), s, P);
 }
-void launch_update(const CaseParams &P, int finish, hipStream_t s) {
-    IFL_DISPATCH(P.dim, k_update, cellBlocks(P), s, P, finish);
+void launch_update(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_update, cellGrid(P), s, P); }
+void launch_reduce_cells(const CaseParams &P, int mask, hipStream_t s) {
+    k_reduce_cells<<<1, kBlock, 0, s>>>(P, cell_blocks(P), mask);
 }
-void launch_pcorr_defect(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_pcorr_defect, cellBlocks(P), s, P); }
+void launch_pcorr_defect(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_pcorr_defect, cellGrid(P), s, P); }
 void launch_pcorr_accumulate(const CaseParams &P, hipStream_t s) {
-    k_pcorr_accumulate<<<cellBlocks(P), kBlock, 0, s>>>(P);
+    k_pcorr_accumulate<<<cellGrid(P), kBlock, 0, s>>>(P);
 }
 
-void launch_sweep(const CaseParams &P, int smoother, float *fld, const float *coef, const float *src, float rlx,
-                  double tol, int slot, int eq, int sweep, hipStream_t s) {
-    if (smoother == IFL_SMOOTHER_JACOBI) {
-        IFL_DISPATCH(P.dim, k_sweep_jacobi, cellBlocks(P), s, P, fld, P.jac, coef, src, rlx, tol, slot, eq, sweep);
-        k_copy_back<<<cellBlocks(P), kBlock, 0, s>>>(P, fld, P.jac, slot, sweep);
-        return;
-    }
+void launch_sweep_rb_half(const CaseParams &P, float *fld, const float *coef, const float *src, float rlx, double tol,
+                          int slot, int sweep, hipStream_t s) {
     for (int color = 0; color < 2; ++color)
-        IFL_DISPATCH(P.dim, k_sweep_rb, rbBlocks(P), s, P, fld, coef, src, rlx, tol, slot, eq, sweep, color);
+        IFL_DISPATCH(P.dim, k_sweep_rb, rbGrid(P), s, P, fld, coef, src, rlx, tol, slot, sweep, color);
+}
+void launch_sweep_jacobi(const CaseParams &P, const float *in, float *out, const float *coef, const float *src,
+                         float rlx, double tol, int slot, int sweep, hipStream_t s) {
+    IFL_DISPATCH(P.dim, k_sweep_jacobi, cellGrid(P), s, P, in, out, coef, src, rlx, tol, slot, sweep);
+}
+void launch_smooth_finish(const CaseParams &P, int slot, int eq, int n, int nprev, double tol, hipStream_t s) {
+    k_smooth_finish<<<1, kBlock, 0, s>>>(P, slot, eq, n, nprev, tol);
 }
 
 }  // namespace ifl

@@ -30,6 +30,8 @@ FIELD_NAMES = ["u", "v", "w", "p", "temp", "pcorr", "uf", "vf", "wf", "mcoef", "
  STAGE_SMOOTH_V, STAGE_SMOOTH_W, STAGE_RHIE_CHOW, STAGE_PCORR_ASSEMBLE, STAGE_SMOOTH_P,
  STAGE_UPDATE, STAGE_OUTER) = range(12)
 
+TILE_AUTO, TILE_64x16, TILE_64x8, TILE_32x8, TILE_32x4 = range(5)
+
 (KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE) = range(5)
 
 
@@ -56,7 +58,8 @@ class Config(C.Structure):
         ("p_refine", C.c_int32),
         ("log_every", C.c_int32),
         ("check_every", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("tile", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
     def copy(self):

@@ -171,7 +171,7 @@ def test_nan_initial_field_reports_divergence():
     cfg = CASES["cavity2d"]()
     g = Solver(cfg)
     u = g.get(_abi.FIELD_U)
-    u[3, 4] = np.nan
+    u[0, 3, 4] = np.nan
     g.set(_abi.FIELD_U, u)
     r = g.solve(10)
     assert r["status"] == _abi.IFL_ERR_DIVERGED
@@ -206,3 +206,30 @@ def test_bad_config_rejected():
     cfg.nx = 2
     with pytest.raises(IflError):
         Solver(cfg)
+
+
+@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_64x8, _abi.TILE_32x8, _abi.TILE_32x4])
+@pytest.mark.parametrize("shape", [(70, 37), (33, 18)])
+def test_fused_sweep_tiles_ragged_grids(tile, shape):
+    """Every tile variant of the fused sweep on grids that are not tile multiples."""
+    cfg = _abi.channel_config(shape[0], shape[1], 6)
+    cfg.tile = tile
+    _log(f"[tiles] tile={tile} grid={shape}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    g.solve(6)
+    o.solve(6)
+    _compare_all(g, o, cfg, f"tile {tile} {shape}")
+
+
+def test_odd_and_unequal_sweep_counts():
+    """nIter_u != nIter_v and odd counts exercise the partially fused launches and copy-backs."""
+    cfg = CASES["cavity2d"]()
+    cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[3] = 3, 6, 7
+    cfg.p_refine = 1
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(5)
+    ro = o.solve(5)
+    _compare_all(g, o, cfg, "odd sweeps")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
