@@ -24,31 +24,115 @@ namespace ifl {
 template <int TX, int TY, int NEQ, int KIND>
 __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float *coef, SweepEq e0, SweepEq e1,
                                                       int sweep) {
-    constexpr int HX = TX + 4, HY = TY + 4;
-    constexpr bool PC = KIND == COEF_PCORR;
+    constexpr int HX = TX + 4, HY = TY + 4;                    // field + 2-cell halo
+    constexpr int NST = (HX * HY + kBlock - 1) / kBlock;       // staged values per thread
+    constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;         // red cells of tile + ring
+    constexpr int NRT = (NR + kBlock - 1) / kBlock;
+    constexpr int BW = TX / 2, NB = TY * BW;                   // black cells of the tile
+    constexpr int NBT = (NB + kBlock - 1) / kBlock;
+    constexpr bool PC = KIND == COEF_PCORR, MOM = KIND == COEF_MOMENTUM;
+    constexpr int UX = TX + 3, UY = TY + 2;                    // uf faces x0-1..x0+TX+1, rows y0-1..y0+TY
+    constexpr int VX = TX + 2, VY = TY + 3;                    // vf faces of cells x0-1..x0+TX, rows y0-1..y0+TY+1
+    constexpr int NUT = MOM ? (UX * UY + kBlock - 1) / kBlock : 1;
+    constexpr int NVT = MOM ? (VX * VY + kBlock - 1) / kBlock : 1;
     __shared__ float sphi[NEQ][HY][HX];
     __shared__ float srinv[PC ? HY : 1][PC ? HX : 1];
+    __shared__ float suf[MOM ? UY : 1][MOM ? UX : 1];
+    __shared__ float svf[MOM ? VY : 1][MOM ? VX : 1];
     if (caseDone(P)) return;
     const int nx = P.nx, ny = P.ny;
     const int tilesX = (nx + TX - 1) / TX;
     const int by = blockIdx.x / tilesX;
     const int bx = blockIdx.x - by * tilesX;
     const int x0 = bx * TX, y0 = by * TY;
-    const int nblk = gridDim.x;
     const SweepEq eqs[2] = {e0, e1};
-    bool act[2] = {false, false};
-#pragma unroll
-    for (int q = 0; q < NEQ; ++q) act[q] = sweepRuns(P, eqs[q].slot, sweep, nblk, eqs[q].tol);
+    const int tid = threadIdx.x;
 
-    // stage the old field (and 1/aC) on tile + 2-cell halo; outside the grid the value is 0
-    for (int e = threadIdx.x; e < HX * HY; e += kBlock) {
+    // ---- issue every global load of the tile first (latency overlaps the stop decision) ----
+    float rphi[NEQ][NST], rr[PC ? NST : 1];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int e = tid + m * kBlock;
         const int ly = e / HX, lx = e - ly * HX;
         const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-        const bool in = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool in = e < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         const int g = gy * nx + gx;
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) sphi[q][ly][lx] = in ? eqs[q].in[g] : 0.0f;
-        if constexpr (PC) srinv[ly][lx] = in ? coef[g] : 0.0f;
+        for (int q = 0; q < NEQ; ++q) rphi[q][m] = in ? eqs[q].in[g] : 0.0f;
+        if constexpr (PC) rr[m] = in ? coef[g] : 0.0f;
+    }
+    float ruf[NUT], rvf[NVT];
+    if constexpr (MOM) {
+#pragma unroll
+        for (int m = 0; m < NUT; ++m) {
+            const int e = tid + m * kBlock;
+            const int ly = e / UX, lx = e - ly * UX;
+            const int fx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            const bool in = e < UX * UY && fx >= 0 && fx <= nx && gy >= 0 && gy < ny;
+            ruf[m] = in ? P.uf[ufi(P, fx, gy, 0)] : 0.0f;
+        }
+#pragma unroll
+        for (int m = 0; m < NVT; ++m) {
+            const int e = tid + m * kBlock;
+            const int ly = e / VX, lx = e - ly * VX;
+            const int gx = x0 - 1 + lx, fy = y0 - 1 + ly;
+            const bool in = e < VX * VY && gx >= 0 && gx < nx && fy >= 0 && fy <= ny;
+            rvf[m] = in ? P.vf[vfi(P, gx, fy, 0)] : 0.0f;
+        }
+    }
+    // red cells ((gx+gy) even) of the tile and its 1-cell ring, black cells of the tile
+    int redG[NRT], blkG[NBT];
+    float srcR[NEQ][NRT], srcB[NEQ][NBT];
+#pragma unroll
+    for (int m = 0; m < NRT; ++m) {
+        const int e = tid + m * kBlock;
+        const int r = e / RW, kk = e - r * RW;
+        const int gy = y0 - 1 + r;
+        const int gx = x0 - 1 + 2 * kk + ((x0 - 1 + gy) & 1);
+        const bool in = e < NR && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        redG[m] = in ? ((gy - y0 + 2) << 16) | (gx - x0 + 2) : -1;
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q) srcR[q][m] = in ? eqs[q].src[gy * nx + gx] : 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < NBT; ++m) {
+        const int e = tid + m * kBlock;
+        const int r = e / BW, kk = e - r * BW;
+        const int gy = y0 + r;
+        const int gx = x0 + 2 * kk + ((x0 + gy + 1) & 1);
+        const bool in = e < NB && gx < nx && gy < ny;
+        blkG[m] = in ? ((gy - y0 + 2) << 16) | (gx - x0 + 2) : -1;
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q) srcB[q][m] = in ? eqs[q].src[gy * nx + gx] : 0.0f;
+    }
+
+    // ---- stop decision of this sweep from the previous sweep's partials ----
+    bool act[2] = {false, false};
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) act[q] = sweepRuns(P, eqs[q].slot, sweep, gridDim.x, eqs[q].tol);
+
+    // ---- stage to LDS ----
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int e = tid + m * kBlock;
+        if (e < HX * HY) {
+            const int ly = e / HX, lx = e - ly * HX;
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) sphi[q][ly][lx] = rphi[q][m];
+            if constexpr (PC) srinv[ly][lx] = rr[m];
+        }
+    }
+    if constexpr (MOM) {
+#pragma unroll
+        for (int m = 0; m < NUT; ++m) {
+            const int e = tid + m * kBlock;
+            if (e < UX * UY) suf[e / UX][e % UX] = ruf[m];
+        }
+#pragma unroll
+        for (int m = 0; m < NVT; ++m) {
+            const int e = tid + m * kBlock;
+            if (e < VX * VY) svf[e / VX][e % VX] = rvf[m];
+        }
     }
     __syncthreads();
 
@@ -56,18 +140,18 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
     const double Ky = -0.5 * P.rho * P.Ay * P.Ay;
     double acc[2] = {0.0, 0.0};
 
-    auto relax = [&](int gx, int gy) {
-        const int lx = gx - x0 + 2, ly = gy - y0 + 2;
-        const int g = gy * nx + gx;
+    // relax cell at LDS position (ly, lx) with sources src[q]
+    auto relax = [&](int ly, int lx, const float (&src)[2]) {
+        const int gx = x0 + lx - 2, gy = y0 + ly - 2;
         float a[7];
         if constexpr (KIND == COEF_STORED) {
+            const int g = gy * nx + gx;
 #pragma unroll
             for (int s = 0; s < 5; ++s) a[s] = coef[g + P.N * s];
-        } else if constexpr (KIND == COEF_MOMENTUM) {
-            const float ue = P.uf[ufi(P, gx + 1, gy, 0)];
-            const float uw = P.uf[ufi(P, gx, gy, 0)];
-            const float vn = P.vf[vfi(P, gx, gy + 1, 0)];
-            const float vs = P.vf[vfi(P, gx, gy, 0)];
+        } else if constexpr (MOM) {
+            // uf face of cell (gx,gy) west side sits at suf[ly-1][lx-1], vf south at svf[ly-1][lx-1]
+            const float uw = suf[ly - 1][lx - 1], ue = suf[ly - 1][lx];
+            const float vs = svf[ly - 1][lx - 1], vn = svf[ly][lx - 1];
             momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
             momBcRowsCoef<2>(P, gx, gy, 0, a);
         } else {
@@ -78,12 +162,12 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
             a[sS] = (gy > 0) ? (float)(Ky * (rC + srinv[ly - 1][lx])) : 0.0f;
             a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
         }
-        const bool owned = gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY;
+        const bool owned = lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2;
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) {
             if (!act[q]) continue;
             const float phi_C = sphi[q][ly][lx];
-            float newPhi = eqs[q].src[g] - a[sE] * sphi[q][ly][lx + 1] - a[sW] * sphi[q][ly][lx - 1]
+            float newPhi = src[q] - a[sE] * sphi[q][ly][lx + 1] - a[sW] * sphi[q][ly][lx - 1]
                            - a[sN] * sphi[q][ly + 1][lx] - a[sS] * sphi[q][ly - 1][lx];
             newPhi /= a[sC];
             const float d = eqs[q].rlx * (newPhi - phi_C);
@@ -93,27 +177,23 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
     };
 
     if (act[0] || act[1]) {
-        // red cells ((gx+gy) even) of the tile and its 1-cell ring
-        constexpr int RW = TX / 2 + 1;
-        for (int e = threadIdx.x; e < (TY + 2) * RW; e += kBlock) {
-            const int r = e / RW, kk = e - r * RW;
-            const int gy = y0 - 1 + r;
-            const int gx = x0 - 1 + 2 * kk + ((x0 - 1 + gy) & 1);
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) relax(gx, gy);
-        }
+#pragma unroll
+        for (int m = 0; m < NRT; ++m)
+            if (redG[m] >= 0) {
+                const float src[2] = {srcR[0][m], srcR[NEQ - 1][m]};
+                relax(redG[m] >> 16, redG[m] & 0xffff, src);
+            }
         __syncthreads();
-        // black cells of the tile
-        constexpr int BW = TX / 2;
-        for (int e = threadIdx.x; e < TY * BW; e += kBlock) {
-            const int r = e / BW, kk = e - r * BW;
-            const int gy = y0 + r;
-            const int gx = x0 + 2 * kk + ((x0 + gy + 1) & 1);
-            if (gx < nx && gy < ny) relax(gx, gy);
-        }
+#pragma unroll
+        for (int m = 0; m < NBT; ++m)
+            if (blkG[m] >= 0) {
+                const float src[2] = {srcB[0][m], srcB[NEQ - 1][m]};
+                relax(blkG[m] >> 16, blkG[m] & 0xffff, src);
+            }
         __syncthreads();
     }
     // write the tile (a stopped equation is copied through unchanged, keeping the ping-pong parity)
-    for (int e = threadIdx.x; e < TX * TY; e += kBlock) {
+    for (int e = tid; e < TX * TY; e += kBlock) {
         const int r = e / TX, xx = e - r * TX;
         const int gx = x0 + xx, gy = y0 + r;
         if (gx < nx && gy < ny) {
