@@ -79,6 +79,7 @@ int validate(const ifl_config &c) {
     if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
     if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
     if (c.tile < 0 || c.tile > 4) return fail(IFL_ERR_ARG, "tile must be 0..4");
+    if (c.sweep_block < 0 || c.sweep_block > 2) return fail(IFL_ERR_ARG, "sweep_block must be 0, 1 or 2");
     return IFL_OK;
 }
 
@@ -133,24 +134,31 @@ struct EqRef {
     int q, slot, n;
 };
 
+int sweepBlock(const ifl_solver *h) { return h->cfg.sweep_block == 2 ? 2 : 1; }
+
 void enqueueFused2d(ifl_solver *h, int kind, const float *coef, EqRef *e, int neq) {
     const ifl::CaseParams &P = h->P;
-    int nmax = 0;
-    for (int k = 0; k < neq; ++k) nmax = std::max(nmax, e[k].n);
-    for (int s = 0; s < nmax; ++s) {
+    const int K = sweepBlock(h);
+    int nL = 0;
+    for (int k = 0; k < neq; ++k) nL = std::max(nL, (e[k].n + K - 1) / K);
+    for (int L = 0; L < nL; ++L) {
         ifl::SweepEq se[2];
         int m = 0;
         for (int k = 0; k < neq; ++k) {
-            if (s >= e[k].n) continue;
-            const bool even = (s % 2) == 0;
+            const int kk = std::min(K, e[k].n - L * K);
+            if (kk <= 0) continue;
+            const int kp = L > 0 ? std::min(K, e[k].n - (L - 1) * K) : 0;
+            const bool even = (L % 2) == 0;
             se[m++] = ifl::SweepEq{even ? e[k].fld : e[k].alt, even ? e[k].alt : e[k].fld, e[k].src,
-                                   P.relax[e[k].q], P.tol[e[k].q], e[k].slot, e[k].q};
+                                   P.relax[e[k].q], P.tol[e[k].q], e[k].slot, e[k].q, kk, kp};
         }
         if (m == 1) se[1] = se[0];
-        ifl::launch_rb_sweep2d(P, kind, coef, se[0], se[1], m, s, h->tile, h->stream);
+        ifl::launch_rb_sweep2d(P, kind, coef, se[0], se[1], m, L, K, h->tile, h->stream);
     }
-    for (int k = 0; k < neq; ++k)
-        if (e[k].n % 2 == 1) ifl::launch_copy(P, e[k].fld, e[k].alt, h->stream);
+    for (int k = 0; k < neq; ++k) {
+        const int nLk = (e[k].n + K - 1) / K;
+        if (nLk % 2 == 1) ifl::launch_copy(P, e[k].fld, e[k].alt, h->stream);
+    }
 }
 
 bool fused2d(const ifl_solver *h) { return h->P.dim == 2 && h->cfg.smoother == IFL_SMOOTHER_RBGS; }
@@ -164,7 +172,10 @@ int sweepRecords(const ifl_solver *h) {
 
 void enqueueFinish(ifl_solver *h, int q, int slot) {
     const int n = h->cfg.n_iter[q];
-    if (n > 0) ifl::launch_smooth_finish(h->P, slot, q, n, sweepRecords(h), h->P.tol[q], h->stream);
+    if (n <= 0) return;
+    const int K = fused2d(h) ? sweepBlock(h) : 1;
+    const int nL = (n + K - 1) / K, kLast = n - (nL - 1) * K;
+    ifl::launch_smooth_finish(h->P, slot, q, nL, kLast, sweepRecords(h), h->P.tol[q], h->stream);
 }
 
 void enqueueSmoother(ifl_solver *h, float *fld, const float *coef, const float *src, int q, int slot) {
@@ -361,7 +372,7 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
         {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
         {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
         {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
-        {(void **)&P.spart, (size_t)ifl::kSlots * 2 * pstride * 8},
+        {(void **)&P.spart, (size_t)ifl::kSlots * 4 * pstride * 8},
         {(void **)&P.partMom, 3 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
         {(void **)&P.st, sizeof(ifl::CaseState)},
         {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
@@ -599,8 +610,9 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
         switch (kernel) {
             case IFL_KERNEL_SWEEP:
                 if (fused2d(h)) {
-                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3};
-                    ifl::launch_rb_sweep2d(P, ifl::COEF_PCORR, P.rinv, e, e, 1, 0, h->tile, h->stream);
+                    const int K = sweepBlock(h);
+                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3, K, 0};
+                    ifl::launch_rb_sweep2d(P, ifl::COEF_PCORR, P.rinv, e, e, 1, 0, K, h->tile, h->stream);
                 } else {
                     ifl::launch_sweep_rb_half(P, P.pc, P.pcoef, P.bp, P.relax[3], P.tol[3], 3, 0, h->stream);
                 }

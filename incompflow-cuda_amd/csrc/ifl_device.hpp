@@ -28,11 +28,13 @@ constexpr int kRows = 4;               // rows per thread in the cell kernels (6
 enum { E = 0, W = 1, N = 2, S = 3, T = 4, B = 5 };                // constants.h:7-12
 enum { sC = 0, sE = 1, sW = 2, sN = 3, sS = 4, sT = 5, sB = 6 };  // constants.h:14-20
 
-// State of one smoother call after sweep k, double-buffered by the parity of k so that the
-// workgroups of sweep k+1 can read the state after sweep k-1 while workgroup 0 publishes k.
+// State of one smoother call after launch L (a launch runs 1 or 2 sweeps), double-buffered by
+// the parity of L so that the workgroups of launch L+1 can read the state after launch L-1 while
+// workgroup 0 publishes the state after L.
 struct SmoothState {
     double maxNorm[2];  // running max of the sweep RMS increment (Solver.cu:941, 957)
-    int stopped[2];     // relNorm < tol seen (remaining sweeps of the call are skipped)
+    int stopped[2];     // relNorm < tol seen (the remaining sweeps of the call are skipped)
+    int swAt[2];        // sweeps that ran up to and including that launch
     double lastSum;     // sum of squared increments of the last sweep that ran
     int sweeps;         // sweeps that ran in the current call
     int pad;
@@ -71,7 +73,7 @@ struct CaseParams {
     float *tmp, *tmp2;       // p' defect and refinement correction
     float *alt[5];           // ping-pong partners of u, v, w, p', refinement correction
     float *rinv;             // 1/aC of the momentum matrix (p' coefficients are rebuilt from it)
-    double *spart;           // sweep partials: [slot][parity][partStride]
+    double *spart;           // sweep partials: [slot][launch parity][sweep in launch][partStride]
     double *partMom;         // 3 per cell workgroup: momentum residual sums
     double *partMass;        // 1 per cell workgroup: b'^2
     double *partUpd;         // 7 per cell workgroup: update increments
@@ -162,39 +164,44 @@ __device__ __forceinline__ int caseDone(const CaseParams &P) { return P.st->done
 __device__ __forceinline__ int flatBlock() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 __device__ __forceinline__ bool leader() { return flatBlock() == 0 && threadIdx.x == 0; }
 
-__device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int sweep) {
-    return P.spart + (size_t)(slot * 2 + (sweep & 1)) * P.partStride;
+__device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int launch, int kk) {
+    return P.spart + (size_t)((slot * 2 + (launch & 1)) * 2 + kk) * P.partStride;
 }
 
-// Whether sweep `sweep` of the smoother call in `slot` runs.  Every workgroup evaluates the
-// relative norm of sweep-1 from its partials (nprev records) exactly as the host loop of
-// Solver.cu:955-962 does after each sweep (in FP64); workgroup 0 publishes the state after
-// sweep-1.  With sweep == nIter this closes the call (sweeps count, last sum).
-__device__ __forceinline__ bool sweepRuns(const CaseParams &P, int slot, int sweep, int nprev, double tol) {
-    if (sweep == 0) return true;
+// Whether launch L of the smoother call in `slot` runs.  Every workgroup evaluates, from their
+// partials (nprev records each), the kPrev sweeps of launch L-1 in order exactly as the host loop
+// of Solver.cu:955-962 does after each sweep (in FP64): norm, running max, relative norm against
+// tol.  A met test stops the call after the launch that contains the sweep (with one sweep per
+// launch: right after that sweep, as in the reference).  Workgroup 0 publishes the state after
+// launch L-1.  With L == number of launches this closes the call (sweeps count, last sum).
+__device__ __forceinline__ bool launchRuns(const CaseParams &P, int slot, int L, int kPrev, int nprev, double tol) {
+    if (L == 0) return true;
     SmoothState &c = P.st->sm[slot];
-    const int pp = sweep & 1;                      // parity of the state after sweep-2
-    const bool stoppedBefore = (sweep >= 2) && c.stopped[pp];
-    const double mxPrev = (sweep >= 2) ? c.maxNorm[pp] : -1e20;
-    bool stopped = true;
-    double mx = mxPrev, T = 0.0;
+    const int pp = L & 1;                          // parity of the state after launch L-2
+    const bool stoppedBefore = (L >= 2) && c.stopped[pp];
+    double mx = (L >= 2) ? c.maxNorm[pp] : -1e20;
+    const int before = (L >= 2) ? c.swAt[pp] : 0;
+    bool stopped = stoppedBefore;
+    double T = 0.0;
+    int ran = before;
     if (!stoppedBefore) {
-        double t[1];
-        sumPartials<1>(sweepPart(P, slot, sweep - 1), nprev, t);
-        T = t[0];
-        const double norm = sqrt(T / (double)P.N);
-        mx = fmax(norm, mxPrev);
-        const double rel = norm / (mx + 1e-20);
-        stopped = rel < tol;
+        for (int kk = 0; kk < kPrev; ++kk) {
+            double t[1];
+            sumPartials<1>(sweepPart(P, slot, L - 1, kk), nprev, t);
+            T = t[0];
+            const double norm = sqrt(T / (double)P.N);
+            mx = fmax(norm, mx);
+            if (norm / (mx + 1e-20) < tol) stopped = true;
+        }
+        ran = before + kPrev;
     }
     if (leader()) {
-        const int q = (sweep - 1) & 1;
+        const int q = (L - 1) & 1;
         c.stopped[q] = stopped ? 1 : 0;
         c.maxNorm[q] = mx;
-        if (!stoppedBefore) {
-            c.lastSum = T;
-            c.sweeps = sweep;
-        }
+        c.swAt[q] = ran;
+        if (!stoppedBefore) c.lastSum = T;
+        c.sweeps = ran;
     }
     return !stopped;
 }

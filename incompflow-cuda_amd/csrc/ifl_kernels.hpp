@@ -14,6 +14,8 @@ struct SweepEq {
     double tol;
     int slot;
     int eq;
+    int k;       // sweeps of this equation in this launch (0..K)
+    int kPrev;   // sweeps of this equation in the previous launch of the call
 };
 
 enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2 };
@@ -38,9 +40,10 @@ void launch_sweep_rb_half(const CaseParams &P, float *fld, const float *coef, co
                           int slot, int sweep, hipStream_t s);
 void launch_sweep_jacobi(const CaseParams &P, const float *in, float *out, const float *coef, const float *src,
                          float rlx, double tol, int slot, int sweep, hipStream_t s);
-void launch_smooth_finish(const CaseParams &P, int slot, int eq, int n, int nprev, double tol, hipStream_t s);
+void launch_smooth_finish(const CaseParams &P, int slot, int eq, int nL, int kLast, int nprev, double tol,
+                          hipStream_t s);
 void launch_rb_sweep2d(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1, int neq,
-                       int sweep, int tile, hipStream_t s);
+                       int launch, int block, int tile, hipStream_t s);
 void launch_copy(const CaseParams &P, float *dst, const float *src, hipStream_t s);
 
 }  // namespace ifl

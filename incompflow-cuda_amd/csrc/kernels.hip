@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rb(CaseParams P, float *fld, c
                                                      float rlx, double tol, int slot, int sweep, int color) {
     if (caseDone(P)) return;
     const int nb = gridDim.x * gridDim.y;
-    if (!sweepRuns(P, slot, sweep, 2 * nb, tol)) return;
+    if (!launchRuns(P, slot, sweep, 1, 2 * nb, tol)) return;
     const int rows = P.ny * P.nz;
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rb(CaseParams P, float *fld, c
             }
         }
     }
-    storeBlockSums<1>(s, sweepPart(P, slot, sweep) + color * nb + flatBlock());
+    storeBlockSums<1>(s, sweepPart(P, slot, sweep, 0) + color * nb + flatBlock());
 }
 
 // ---------------- point Jacobi sweep, kernels.cuh:845-924 ----------------
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const flo
                                                          const float *src, float rlx, double tol, int slot, int sweep) {
     if (caseDone(P)) return;
     const int nb = gridDim.x * gridDim.y;
-    const bool run = sweepRuns(P, slot, sweep, nb, tol);
+    const bool run = launchRuns(P, slot, sweep, 1, nb, tol);
     double s[1] = {0.0};
     forCells(P, [&](int i, int j, int k, int c) {
         if (run) {
@@ -235,14 +235,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const flo
             out[c] = in[c];
         }
     });
-    if (run) storeBlockSums<1>(s, sweepPart(P, slot, sweep) + flatBlock());
+    if (run) storeBlockSums<1>(s, sweepPart(P, slot, sweep, 0) + flatBlock());
 }
 
-// Closes a smoother call of nIter sweeps: publishes the state after the last sweep and adds
-// the sweeps that ran to the outer iteration's count (one workgroup).
-__global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P, int slot, int eq, int n, int nprev, double tol) {
+// Closes a smoother call of nL launches (the last one ran kLast sweeps): publishes the state
+// after the last launch and adds the sweeps that ran to the outer iteration's count.
+__global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P, int slot, int eq, int nL, int kLast, int nprev,
+                                                          double tol) {
     if (caseDone(P)) return;
-    sweepRuns(P, slot, n, nprev, tol);
+    launchRuns(P, slot, nL, kLast, nprev, tol);
     if (threadIdx.x == 0) P.st->sweepsIter[eq] += P.st->sm[slot].sweeps;
 }
 
@@ -556,8 +557,9 @@ void launch_sweep_jacobi(const CaseParams &P, const float *in, float *out, const
                          float rlx, double tol, int slot, int sweep, hipStream_t s) {
     IFL_DISPATCH(P.dim, k_sweep_jacobi, cellGrid(P), s, P, in, out, coef, src, rlx, tol, slot, sweep);
 }
-void launch_smooth_finish(const CaseParams &P, int slot, int eq, int n, int nprev, double tol, hipStream_t s) {
-    k_smooth_finish<<<1, kBlock, 0, s>>>(P, slot, eq, n, nprev, tol);
+void launch_smooth_finish(const CaseParams &P, int slot, int eq, int nL, int kLast, int nprev, double tol,
+                          hipStream_t s) {
+    k_smooth_finish<<<1, kBlock, 0, s>>>(P, slot, eq, nL, kLast, nprev, tol);
 }
 
 }  // namespace ifl

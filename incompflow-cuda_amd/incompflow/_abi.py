@@ -59,7 +59,8 @@ class Config(C.Structure):
         ("log_every", C.c_int32),
         ("check_every", C.c_int32),
         ("tile", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("sweep_block", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
     def copy(self):

@@ -66,6 +66,7 @@ public:
     int stopRule = IFL_STOP_FAITHFUL;
     double tolRes = 1e-5;
     int pRefine = 0;
+    int sweepBlock = 1;      // sweeps per block: a met stop test ends the call at the block's end
     int numOuterIter = 1;
 
     // ---- state (Solver.h:10-45) ----
@@ -116,6 +117,7 @@ public:
         stopRule = c.stop_rule;
         tolRes = c.tol_residual;
         pRefine = c.p_refine;
+        sweepBlock = c.sweep_block > 1 ? c.sweep_block : 1;
         numOuterIter = c.num_outer_iter;
         // Solver.cu:13-104: uniform initial u,v,w,p,temp; everything else zero.
         u.assign(Nc, (Real)c.init[0]);
@@ -374,9 +376,12 @@ public:
     }
 
     // GaussSeidelIterate / pointJacobiIterate host loops, Solver.cu:868-966 (norms in FP64).
+    // With sweepBlock = 1 this is the reference loop; with sweepBlock = k the test still runs
+    // after every sweep but a met test ends the call only after the current block of k sweeps.
     int iterate(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl) {
         double maxNorm = -1e20;
         int done = 0;
+        bool stopPending = false;
         lastSweepSum = 0;
         for (int it = 0; it < n; ++it) {
             double s = sweep(fld, coef, src, rlx, smoother);
@@ -385,7 +390,8 @@ public:
             double norm = std::sqrt(s / (double)Nc);
             maxNorm = std::max(norm, maxNorm);
             double relNorm = norm / (maxNorm + 1e-20);
-            if (relNorm < (double)tl) break;
+            if (relNorm < (double)tl) stopPending = true;
+            if (stopPending && (it + 1) % sweepBlock == 0) break;
         }
         lastSweeps = done;
         return done;

@@ -233,3 +233,23 @@ def test_odd_and_unequal_sweep_counts():
     ro = o.solve(5)
     _compare_all(g, o, cfg, "odd sweeps")
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+
+
+@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_32x4])
+@pytest.mark.parametrize("name", ["channel2d", "cavity2d"])
+def test_two_sweeps_per_launch(tile, name):
+    """sweep_block=2 (two red-black sweeps per launch over a 4-cell halo) vs the oracle's blocked
+    stop rule, with odd and unequal sweep counts and the p' refinement."""
+    cfg = CASES[name]()
+    cfg.sweep_block = 2
+    cfg.tile = tile
+    cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[3] = 5, 4, 7
+    cfg.p_refine = 1
+    _log(f"[block2] case={name} tile={tile}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(6)
+    ro = o.solve(6)
+    _compare_all(g, o, cfg, f"block2 {name}")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+    np.testing.assert_allclose(_rec_array(rg["history"]), _rec_array(ro["history"]), rtol=1e-11, atol=1e-300)
