@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--sweep-block", type=int, default=1)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
+    ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -72,7 +73,13 @@ def main():
     from incompflow import _abi
 
     n = args.n
-    cfg = _abi.cavity_config(n=n, p_refine=args.p_refine, tol_residual=args.target)
+    if args.workload == "cavity3d":
+        if n == 1024:
+            n = 256
+        cfg = _abi.cavity3d_config(n=n, tol_residual=args.target)
+        cfg.p_refine = args.p_refine
+    else:
+        cfg = _abi.cavity_config(n=n, p_refine=args.p_refine, tol_residual=args.target)
     cfg.sweep_block = args.sweep_block
     cfg.tile = args.tile
     cells = cfg.nx * cfg.ny * cfg.nz
@@ -101,7 +108,9 @@ def main():
     kb = max(1, args.sweep_block)       # sweeps per launch of the fused kernel
     alg_bytes = kb * sweep_bytes_per_cell(cfg.dim) * cells / per   # one launch updates 1/per of the cells
     achieved = alg_bytes / (us * 1e-6) / 1e9
-    own_bytes = 16.0 * cells / per      # the fused kernel's own compulsory traffic: p' in/out, 1/aC, b'
+    # the kernel's own compulsory traffic: 2D p' in/out, 1/aC, b' (coefficients rebuilt from 1/aC);
+    # 3D field in/out, source and the 7 stored coefficient planes
+    own_bytes = (16.0 if cfg.dim == 2 else 40.0) * cells * kb / per
     own_gbs = own_bytes / (us * 1e-6) / 1e9
     other = {}
     for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
@@ -173,13 +182,17 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (fields start at rest)",
-        "config": {"workload": f"2D lid-driven cavity {n}x{n}, Re=100, FP32 RB-GS 10/10/20 sweeps, "
-                               f"alpha 0.7/0.3, FP64 reductions, {args.p_refine} FP64-defect p' refinement",
+        "config": {"workload": (f"2D lid-driven cavity {n}x{n}, Re=100, FP32 RB-GS 10/10/20 sweeps, "
+                                f"alpha 0.7/0.3, FP64 reductions, {args.p_refine} FP64-defect p' refinement")
+                   if cfg.dim == 2 else
+                   (f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
+                    f"FP64 reductions"),
                    "cells": cells, "sweeps_per_step": sweeps / args.steps, "sweep_block": kb,
                    "tile": cfg.tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_rb_tile2d (one fused red-black sweep of p')", "avg_launch_us": us,
+                     "kernel": ("k_rb_tile2d (fused red-black sweeps of p')" if cfg.dim == 2 else
+                                "k_rb_zmarch3d (one fused red-black sweep of p')"), "avg_launch_us": us,
                      "alg_bytes_per_launch": alg_bytes, "own_bytes_per_launch": own_bytes,
                      "own_gbs": own_gbs},
         "kernel_us": other,

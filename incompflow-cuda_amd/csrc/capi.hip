@@ -118,7 +118,7 @@ void fillParams(ifl_solver *h) {
 // Records per (slot, parity) sweep-partial array: one per workgroup of the largest sweep launch
 // (fused 2D tiles: the smallest tile gives the most; red+black half sweeps; Jacobi cell grid).
 int partStride(const ifl::CaseParams &P) {
-    int n = std::max(ifl::cell_blocks(P), 2 * ifl::rb_blocks(P));
+    int n = std::max(std::max(ifl::cell_blocks(P), 2 * ifl::rb_blocks(P)), ifl::zmarch_blocks(P));
     if (P.dim == 2)
         for (int t = ifl::TILE_64x16; t <= ifl::TILE_32x4; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
     return n;
@@ -167,7 +167,7 @@ bool fused2d(const ifl_solver *h) { return h->P.dim == 2 && h->cfg.smoother == I
 int sweepRecords(const ifl_solver *h) {
     if (fused2d(h)) return ifl::tile_count(h->P.nx, h->P.ny, h->tile);
     if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) return ifl::cell_blocks(h->P);
-    return 2 * ifl::rb_blocks(h->P);
+    return ifl::zmarch_blocks(h->P);
 }
 
 void enqueueFinish(ifl_solver *h, int q, int slot) {
@@ -192,9 +192,14 @@ void enqueueSmoother(ifl_solver *h, float *fld, const float *coef, const float *
                                      P.tol[q], slot, s, h->stream);
         }
         if (n % 2 == 1) ifl::launch_copy(P, fld, P.alt[slot], h->stream);
-    } else {
-        for (int s = 0; s < n; ++s)
-            ifl::launch_sweep_rb_half(P, fld, coef, src, P.relax[q], P.tol[q], slot, s, h->stream);
+    } else {                                     // 3D red-black: z-marching fused sweep
+        for (int s = 0; s < n; ++s) {
+            const bool even = (s % 2) == 0;
+            ifl::SweepEq e{even ? fld : P.alt[slot], even ? P.alt[slot] : fld, src, P.relax[q], P.tol[q], slot, q, 1,
+                           s > 0 ? 1 : 0};
+            ifl::launch_rb_sweep3d(P, coef, e, s, h->stream);
+        }
+        if (n % 2 == 1) ifl::launch_copy(P, fld, P.alt[slot], h->stream);
     }
     enqueueFinish(h, q, slot);
 }
@@ -614,7 +619,8 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
                     ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3, K, 0};
                     ifl::launch_rb_sweep2d(P, ifl::COEF_PCORR, P.rinv, e, e, 1, 0, K, h->tile, h->stream);
                 } else {
-                    ifl::launch_sweep_rb_half(P, P.pc, P.pcoef, P.bp, P.relax[3], P.tol[3], 3, 0, h->stream);
+                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3, 1, 0};
+                    ifl::launch_rb_sweep3d(P, P.pcoef, e, 0, h->stream);
                 }
                 break;
             case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
@@ -625,7 +631,7 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
         }
     };
     if (kernel < 0 || kernel >= IFL_KERNEL_COUNT) return fail(IFL_ERR_ARG, "bad kernel id");
-    if (kernel == IFL_KERNEL_SWEEP && !(fused2d(h))) per = 2;
+    if (kernel == IFL_KERNEL_SWEEP && h->cfg.smoother == IFL_SMOOTHER_JACOBI) per = 1;
     once();   // warm-up
     IFL_HIP(hipGetLastError());
     IFL_HIP(hipEventRecord(h->ev0, h->stream));

@@ -45,5 +45,7 @@ void launch_smooth_finish(const CaseParams &P, int slot, int eq, int nL, int kLa
 void launch_rb_sweep2d(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1, int neq,
                        int launch, int block, int tile, hipStream_t s);
 void launch_copy(const CaseParams &P, float *dst, const float *src, hipStream_t s);
+int zmarch_blocks(const CaseParams &P);
+void launch_rb_sweep3d(const CaseParams &P, const float *coef, const SweepEq &e, int launch, hipStream_t s);
 
 }  // namespace ifl

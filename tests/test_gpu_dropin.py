@@ -49,4 +49,4 @@ def test_dropin_solver_example(tmp_path):
     lines = vtk.split("\n")
     k = lines.index("pressure 1 480 float")
     p_vtk = np.array(lines[k + 1].split(), dtype=np.float64)
-    np.testing.assert_allclose(p_vtk, s.get(_abi.FIELD_P).ravel(), rtol=2e-6, atol=1e-9)
+    np.testing.assert_allclose(p_vtk, s.get(_abi.FIELD_P).ravel(), rtol=1e-5, atol=1e-6)   # 6 digits

@@ -253,3 +253,29 @@ def test_two_sweeps_per_launch(tile, name):
     _compare_all(g, o, cfg, f"block2 {name}")
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
     np.testing.assert_allclose(_rec_array(rg["history"]), _rec_array(ro["history"]), rtol=1e-11, atol=1e-300)
+
+
+@pytest.mark.parametrize("shape", [(70, 20, 40), (9, 17, 33)])
+def test_3d_zmarch_multiple_tiles_and_chunks(shape):
+    """The 3D fused sweep with several x/y tiles and z chunks (chunk edges recompute red planes)."""
+    cfg = CASES["duct3d"]()
+    cfg.nx, cfg.ny, cfg.nz = shape
+    cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[2], cfg.n_iter[3] = 3, 2, 5, 7
+    _log(f"[3d zmarch] grid={shape}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(3)
+    ro = o.solve(3)
+    _compare_all(g, o, cfg, f"3d {shape}")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+
+
+def test_3d_jacobi_parity():
+    cfg = CASES["cavity3d"]()
+    cfg.smoother = _abi.SMOOTHER_JACOBI
+    cfg.n_iter[3] = 5
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    g.solve(3)
+    o.solve(3)
+    _compare_all(g, o, cfg, "3d jacobi")
