@@ -634,13 +634,18 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
     if (kernel == IFL_KERNEL_SWEEP && h->cfg.smoother == IFL_SMOOTHER_JACOBI) per = 1;
     once();   // warm-up
     IFL_HIP(hipGetLastError());
-    IFL_HIP(hipEventRecord(h->ev0, h->stream));
-    for (int r = 0; r < reps; ++r) once();
-    IFL_HIP(hipEventRecord(h->ev1, h->stream));
-    IFL_HIP(hipEventSynchronize(h->ev1));
-    float ms = 0;
-    IFL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    *avg_us = 1000.0 * ms / ((double)reps * per);
+    // one event pair per launch: the per-dispatch device duration, as a kernel trace reports it
+    double total = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        IFL_HIP(hipEventRecord(h->ev0, h->stream));
+        once();
+        IFL_HIP(hipEventRecord(h->ev1, h->stream));
+        IFL_HIP(hipEventSynchronize(h->ev1));
+        float ms = 0;
+        IFL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        total += ms;
+    }
+    *avg_us = 1000.0 * total / ((double)reps * per);
     if (launches_per_rep) *launches_per_rep = per;
     return IFL_OK;
 }
