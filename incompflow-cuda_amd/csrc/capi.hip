@@ -115,128 +115,139 @@ void fillParams(ifl_solver *h) {
     P.histCap = kHistCap;
 }
 
-// Records per (slot, parity) sweep-partial array: one per workgroup of the largest sweep launch
-// (fused 2D tiles: the smallest tile gives the most; red+black half sweeps; Jacobi cell grid).
+// Records per (slot, launch parity, sweep) partial array: one per workgroup of the largest sweep
+// launch (fused 2D tiles: the smallest tile gives the most; 3D z-march; Jacobi cell grid).
 int partStride(const ifl::CaseParams &P) {
-    int n = std::max(std::max(ifl::cell_blocks(P), 2 * ifl::rb_blocks(P)), ifl::zmarch_blocks(P));
+    int n = std::max(ifl::cell_blocks(P), ifl::zmarch_blocks(P));
     if (P.dim == 2)
         for (int t = ifl::TILE_64x16; t <= ifl::TILE_32x4; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
     return n;
 }
 
-// Smoother calls of Solver.cu:178-196.  In 2D with red-black ordering every sweep is one
-// launch of the fused tile kernel, reading one buffer of a ping-pong pair and writing the other;
-// an odd sweep count ends with a copy back.  Equations listed together share the matrix and
-// are swept in the same launches (u and v); each keeps its own norms and stop flag.
-struct EqRef {
-    float *fld, *alt;
-    const float *src;
-    int q, slot, n;
+// What one outer iteration launches, for one case or a batch of cases with the same settings.
+struct Plan {
+    ifl::Lc L;
+    const ifl_config *cfg;   // sweep counts, smoother, refinement, sweep block (shared by a batch)
+    int dim;
 };
 
-int sweepBlock(const ifl_solver *h) { return h->cfg.sweep_block == 2 ? 2 : 1; }
+int sweepBlockOf(const ifl_config &c, int dim) {
+    return (dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block == 2) ? 2 : 1;
+}
 
-void enqueueFused2d(ifl_solver *h, int kind, const float *coef, EqRef *e, int neq) {
-    const ifl::CaseParams &P = h->P;
-    const int K = sweepBlock(h);
-    int nL = 0;
-    for (int k = 0; k < neq; ++k) nL = std::max(nL, (e[k].n + K - 1) / K);
-    for (int L = 0; L < nL; ++L) {
-        ifl::SweepEq se[2];
-        int m = 0;
-        for (int k = 0; k < neq; ++k) {
-            const int kk = std::min(K, e[k].n - L * K);
-            if (kk <= 0) continue;
-            const int kp = L > 0 ? std::min(K, e[k].n - (L - 1) * K) : 0;
-            const bool even = (L % 2) == 0;
-            se[m++] = ifl::SweepEq{even ? e[k].fld : e[k].alt, even ? e[k].alt : e[k].fld, e[k].src,
-                                   P.relax[e[k].q], P.tol[e[k].q], e[k].slot, e[k].q, kk, kp};
-        }
-        if (m == 1) se[1] = se[0];
-        ifl::launch_rb_sweep2d(P, kind, coef, se[0], se[1], m, L, K, h->tile, h->stream);
+// Workgroup kind of a smoother launch and, for one case, its workgroup count.
+int sweepKind(const Plan &pl) {
+    if (pl.cfg->smoother == IFL_SMOOTHER_JACOBI) return ifl::MAP_CELL;
+    return pl.dim == 2 ? ifl::MAP_TILE : ifl::MAP_ZMARCH;
+}
+int sweepRecordsSingle(const Plan &pl) {
+    const ifl::CaseParams &P = *pl.L.P;
+    switch (sweepKind(pl)) {
+        case ifl::MAP_CELL: return ifl::cell_blocks(P);
+        case ifl::MAP_TILE: return ifl::tile_count(P.nx, P.ny, pl.L.tile);
+        default: return ifl::zmarch_blocks(P);
     }
+}
+
+// Smoother calls of Solver.cu:178-196.  Every launch reads one buffer of a ping-pong pair and
+// writes the other; an odd launch count ends with a copy back.  Equations listed together share
+// the matrix and are swept in the same launches (u and v in 2D); each keeps its own norms and
+// stop state.  A call ends with k_smooth_finish.
+void enqueueSmoothers(const Plan &pl, const int *slots, int neq) {
+    const ifl_config &c = *pl.cfg;
+    const int K = sweepBlockOf(c, pl.dim);
+    const bool fused = pl.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS;
+    int n[2], nL[2], nLmax = 0;
     for (int k = 0; k < neq; ++k) {
-        const int nLk = (e[k].n + K - 1) / K;
-        if (nLk % 2 == 1) ifl::launch_copy(P, e[k].fld, e[k].alt, h->stream);
+        const int q = slots[k] < 4 ? slots[k] : 3;
+        n[k] = c.n_iter[q];
+        nL[k] = (n[k] + K - 1) / K;
+        nLmax = std::max(nLmax, nL[k]);
     }
-}
-
-bool fused2d(const ifl_solver *h) { return h->P.dim == 2 && h->cfg.smoother == IFL_SMOOTHER_RBGS; }
-
-// Workgroups of one sweep launch (the records its successor adds up).
-int sweepRecords(const ifl_solver *h) {
-    if (fused2d(h)) return ifl::tile_count(h->P.nx, h->P.ny, h->tile);
-    if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) return ifl::cell_blocks(h->P);
-    return ifl::zmarch_blocks(h->P);
-}
-
-void enqueueFinish(ifl_solver *h, int q, int slot) {
-    const int n = h->cfg.n_iter[q];
-    if (n <= 0) return;
-    const int K = fused2d(h) ? sweepBlock(h) : 1;
-    const int nL = (n + K - 1) / K, kLast = n - (nL - 1) * K;
-    ifl::launch_smooth_finish(h->P, slot, q, nL, kLast, sweepRecords(h), h->P.tol[q], h->stream);
-}
-
-void enqueueSmoother(ifl_solver *h, float *fld, const float *coef, const float *src, int q, int slot) {
-    const ifl::CaseParams &P = h->P;
-    const int n = h->cfg.n_iter[q];
-    if (fused2d(h)) {
-        const bool mom = slot < 3;
-        EqRef e{fld, P.alt[slot], src, q, slot, n};
-        enqueueFused2d(h, mom ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR, mom ? nullptr : P.rinv, &e, 1);
-    } else if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) {
-        for (int s = 0; s < n; ++s) {
-            const bool even = (s % 2) == 0;
-            ifl::launch_sweep_jacobi(P, even ? fld : P.alt[slot], even ? P.alt[slot] : fld, coef, src, P.relax[q],
-                                     P.tol[q], slot, s, h->stream);
+    if (fused || neq == 1) {
+        for (int L = 0; L < nLmax; ++L) {
+            ifl::SweepSel se[2];
+            int m = 0;
+            for (int k = 0; k < neq; ++k) {
+                const int kk = std::min(K, n[k] - L * K);
+                if (kk <= 0) continue;
+                const int kp = L > 0 ? std::min(K, n[k] - (L - 1) * K) : 0;
+                se[m++] = ifl::SweepSel{slots[k], slots[k] < 4 ? slots[k] : 3, kk, kp};
+            }
+            if (m == 1) se[1] = se[0];
+            if (fused) {
+                const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR;
+                ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
+            } else {
+                for (int k = 0; k < m; ++k) {
+                    if (c.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(pl.L, se[k], L);
+                    else ifl::launch_rb_sweep3d(pl.L, se[k], L);
+                }
+            }
         }
-        if (n % 2 == 1) ifl::launch_copy(P, fld, P.alt[slot], h->stream);
-    } else {                                     // 3D red-black: z-marching fused sweep
-        for (int s = 0; s < n; ++s) {
-            const bool even = (s % 2) == 0;
-            ifl::SweepEq e{even ? fld : P.alt[slot], even ? P.alt[slot] : fld, src, P.relax[q], P.tol[q], slot, q, 1,
-                           s > 0 ? 1 : 0};
-            ifl::launch_rb_sweep3d(P, coef, e, s, h->stream);
-        }
-        if (n % 2 == 1) ifl::launch_copy(P, fld, P.alt[slot], h->stream);
-    }
-    enqueueFinish(h, q, slot);
-}
-
-void enqueueMomentumSmoothers(ifl_solver *h) {
-    const ifl::CaseParams &P = h->P;
-    if (fused2d(h)) {                                                 // u and v share the matrix
-        EqRef e[2] = {{P.u, P.alt[0], P.bu, 0, 0, h->cfg.n_iter[0]}, {P.v, P.alt[1], P.bv, 1, 1, h->cfg.n_iter[1]}};
-        enqueueFused2d(h, ifl::COEF_MOMENTUM, nullptr, e, 2);
-        enqueueFinish(h, 0, 0);
-        enqueueFinish(h, 1, 1);
+    } else {
+        for (int k = 0; k < neq; ++k) enqueueSmoothers(pl, &slots[k], 1);
         return;
     }
-    enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0);                     // :178
-    enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1);                     // :180
-    if (P.dim == 3) enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);     // :183
-}
-
-void enqueuePressureCorrection(ifl_solver *h) {
-    const ifl::CaseParams &P = h->P;
-    enqueueSmoother(h, P.pc, P.pcoef, P.bp, 3, 3);                     // Solver.cu:196
-    for (int r = 0; r < h->cfg.p_refine; ++r) {                       // FP64-defect refinement
-        ifl::launch_pcorr_defect(P, h->stream);
-        enqueueSmoother(h, P.tmp2, P.pcoef, P.tmp, 3, 4);
-        ifl::launch_pcorr_accumulate(P, h->stream);
+    for (int k = 0; k < neq; ++k) {
+        if (nL[k] % 2 == 1) ifl::launch_copy_back(pl.L, slots[k]);
+        if (n[k] > 0) {
+            const int q = slots[k] < 4 ? slots[k] : 3;
+            const int kLast = n[k] - (nL[k] - 1) * K;
+            ifl::launch_smooth_finish(pl.L, slots[k], q, nL[k], kLast, sweepKind(pl), sweepRecordsSingle(pl));
+        }
     }
 }
 
-void enqueueOuter(ifl_solver *h) {
-    const ifl::CaseParams &P = h->P;
-    ifl::launch_mom_assemble(P, h->stream);                           // :165-175
-    enqueueMomentumSmoothers(h);                                      // :178-184
-    ifl::launch_rhie_chow(P, h->stream);                              // :186-187
-    ifl::launch_pcorr_assemble(P, h->stream);                         // :189-193
-    enqueuePressureCorrection(h);                                     // :196
-    ifl::launch_update(P, h->stream);                                 // :198
-    ifl::launch_reduce_cells(P, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH, h->stream);  // :200-268
+void enqueueMomentumSmoothers(const Plan &pl) {
+    if (pl.dim == 2 && pl.cfg->smoother == IFL_SMOOTHER_RBGS) {      // u and v share the matrix
+        const int slots[2] = {0, 1};
+        enqueueSmoothers(pl, slots, 2);
+        return;
+    }
+    for (int q = 0; q < pl.dim; ++q) enqueueSmoothers(pl, &q, 1);   // :178, :180, :183
+}
+
+void enqueuePressureCorrection(const Plan &pl) {
+    const int sp = 3, sd = 4;
+    enqueueSmoothers(pl, &sp, 1);                                    // Solver.cu:196
+    for (int r = 0; r < pl.cfg->p_refine; ++r) {                     // FP64-defect refinement
+        ifl::launch_pcorr_defect(pl.L);
+        enqueueSmoothers(pl, &sd, 1);
+        ifl::launch_pcorr_accumulate(pl.L);
+    }
+}
+
+void enqueueOuter(const Plan &pl) {
+    ifl::launch_mom_assemble(pl.L);                                  // :165-175
+    enqueueMomentumSmoothers(pl);                                    // :178-184
+    ifl::launch_rhie_chow(pl.L);                                     // :186-187
+    ifl::launch_pcorr_assemble(pl.L);                                // :189-193
+    enqueuePressureCorrection(pl);                                   // :196
+    ifl::launch_update(pl.L);                                        // :198
+    ifl::launch_reduce_cells(pl.L, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH);  // :200-268
+}
+
+Plan planOf(ifl_solver *h) {
+    Plan pl;
+    pl.L.P = &h->P;
+    pl.L.batch = false;
+    pl.L.tile = h->tile ? h->tile : ifl::TILE_64x16;
+    pl.L.s = h->stream;
+    pl.cfg = &h->cfg;
+    pl.dim = h->P.dim;
+    return pl;
+}
+
+int captureGraph(hipStream_t stream, const Plan &pl, hipGraphExec_t *out) {
+    hipGraph_t g = nullptr;
+    IFL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    enqueueOuter(pl);
+    IFL_HIP(hipStreamEndCapture(stream, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(IFL_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return IFL_OK;
 }
 
 int buildGraph(ifl_solver *h) {
@@ -245,13 +256,8 @@ int buildGraph(ifl_solver *h) {
         IFL_HIP(hipGraphExecDestroy(h->graph));
         h->graph = nullptr;
     }
-    hipGraph_t g = nullptr;
-    IFL_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueueOuter(h);
-    IFL_HIP(hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(IFL_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    const int st = captureGraph(h->stream, planOf(h), &h->graph);
+    if (st != IFL_OK) return st;
     h->graphDirty = false;
     return IFL_OK;
 }
@@ -486,7 +492,7 @@ int ifl_solve(ifl_solver *h, int max_iter, ifl_report *report, ifl_iter_record *
     const int logEvery = h->cfg.log_every;
     std::vector<ifl_iter_record> ring;
     IFL_HIP(hipEventRecord(h->ev0, h->stream));
-    if (h->iter == 0) ifl::launch_init_faces(h->P, h->stream);       // Solver.cu:162-164
+    if (h->iter == 0) ifl::launch_init_faces(planOf(h).L);           // Solver.cu:162-164
     long long launched = 0;
     ifl::CaseState s{};
     double sweepsTotal = 0;
@@ -537,36 +543,40 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
     int st = setDone(h, 0);
     if (st != IFL_OK) return st;
     const ifl::CaseParams &P = h->P;
+    const Plan pl = planOf(h);
+    const ifl::Lc &L = pl.L;
     ifl::CaseState s0{}, s{};
     st = readState(h, s0);
     if (st != IFL_OK) return st;
     switch (stage) {
-        case IFL_STAGE_INIT_FACES: ifl::launch_init_faces(P, h->stream); break;
-        case IFL_STAGE_FACE_BC: ifl::launch_face_bc(P, h->stream); break;
+        case IFL_STAGE_INIT_FACES: ifl::launch_init_faces(L); break;
+        case IFL_STAGE_FACE_BC: ifl::launch_face_bc(L); break;
         case IFL_STAGE_MOM_ASSEMBLE:
-            ifl::launch_mom_assemble(P, h->stream);
-            ifl::launch_reduce_cells(P, ifl::RED_MOM, h->stream);
+            ifl::launch_mom_assemble(L);
+            ifl::launch_reduce_cells(L, ifl::RED_MOM);
             break;
         case IFL_STAGE_MOM_RESIDUAL: break;
-        case IFL_STAGE_SMOOTH_U: enqueueSmoother(h, P.u, P.mcoef, P.bu, 0, 0); break;
-        case IFL_STAGE_SMOOTH_V: enqueueSmoother(h, P.v, P.mcoef, P.bv, 1, 1); break;
-        case IFL_STAGE_SMOOTH_W:
-            if (P.dim != 3) return fail(IFL_ERR_ARG, "SMOOTH_W needs dim 3");
-            enqueueSmoother(h, P.w, P.mcoef, P.bw, 2, 2);
+        case IFL_STAGE_SMOOTH_U:
+        case IFL_STAGE_SMOOTH_V:
+        case IFL_STAGE_SMOOTH_W: {
+            const int slot = stage - IFL_STAGE_SMOOTH_U;
+            if (slot == 2 && P.dim != 3) return fail(IFL_ERR_ARG, "SMOOTH_W needs dim 3");
+            enqueueSmoothers(pl, &slot, 1);
             break;
-        case IFL_STAGE_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
+        }
+        case IFL_STAGE_RHIE_CHOW: ifl::launch_rhie_chow(L); break;
         case IFL_STAGE_PCORR_ASSEMBLE:
-            ifl::launch_pcorr_assemble(P, h->stream);
-            ifl::launch_reduce_cells(P, ifl::RED_MASS, h->stream);
+            ifl::launch_pcorr_assemble(L);
+            ifl::launch_reduce_cells(L, ifl::RED_MASS);
             break;
-        case IFL_STAGE_SMOOTH_P: enqueuePressureCorrection(h); break;
+        case IFL_STAGE_SMOOTH_P: enqueuePressureCorrection(pl); break;
         case IFL_STAGE_UPDATE:
-            ifl::launch_update(P, h->stream);
-            ifl::launch_reduce_cells(P, ifl::RED_UPD, h->stream);
+            ifl::launch_update(L);
+            ifl::launch_reduce_cells(L, ifl::RED_UPD);
             break;
         case IFL_STAGE_OUTER:
-            if (h->iter == 0) ifl::launch_init_faces(P, h->stream);
-            enqueueOuter(h);
+            if (h->iter == 0) ifl::launch_init_faces(L);
+            enqueueOuter(pl);
             break;
         default: return fail(IFL_ERR_ARG, "bad stage");
     }
@@ -609,29 +619,28 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
     if (!h || reps <= 0 || !avg_us) return fail(IFL_ERR_ARG, "bad argument");
     int st = setDone(h, 0);
     if (st != IFL_OK) return st;
-    const ifl::CaseParams &P = h->P;
-    int per = 1;
+    const Plan pl = planOf(h);
+    const ifl::Lc &L = pl.L;
+    const int per = 1;
     auto once = [&]() {
         switch (kernel) {
-            case IFL_KERNEL_SWEEP:
-                if (fused2d(h)) {
-                    const int K = sweepBlock(h);
-                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3, K, 0};
-                    ifl::launch_rb_sweep2d(P, ifl::COEF_PCORR, P.rinv, e, e, 1, 0, K, h->tile, h->stream);
-                } else {
-                    ifl::SweepEq e{P.pc, P.alt[3], P.bp, P.relax[3], P.tol[3], 3, 3, 1, 0};
-                    ifl::launch_rb_sweep3d(P, P.pcoef, e, 0, h->stream);
-                }
+            case IFL_KERNEL_SWEEP: {
+                // one launch of the p' smoother as the outer iteration runs it (first launch of a call)
+                const int K = sweepBlockOf(h->cfg, pl.dim);
+                const ifl::SweepSel e{3, 3, K, 0};
+                if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(L, e, 0);
+                else if (pl.dim == 2) ifl::launch_rb_sweep2d(L, ifl::COEF_PCORR, e, e, 1, 0, K);
+                else ifl::launch_rb_sweep3d(L, e, 0);
                 break;
-            case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(P, h->stream); break;
-            case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(P, h->stream); break;
-            case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(P, h->stream); break;
-            case IFL_KERNEL_UPDATE: ifl::launch_update(P, h->stream); break;
+            }
+            case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(L); break;
+            case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(L); break;
+            case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(L); break;
+            case IFL_KERNEL_UPDATE: ifl::launch_update(L); break;
             default: break;
         }
     };
     if (kernel < 0 || kernel >= IFL_KERNEL_COUNT) return fail(IFL_ERR_ARG, "bad kernel id");
-    if (kernel == IFL_KERNEL_SWEEP && h->cfg.smoother == IFL_SMOOTHER_JACOBI) per = 1;
     once();   // warm-up
     IFL_HIP(hipGetLastError());
     // one event pair per launch: the per-dispatch device duration, as a kernel trace reports it

@@ -161,8 +161,86 @@ __device__ __forceinline__ void sumPartials(const double *part, int n, double (&
 
 __device__ __forceinline__ int caseDone(const CaseParams &P) { return P.st->done; }
 
-__device__ __forceinline__ int flatBlock() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-__device__ __forceinline__ bool leader() { return flatBlock() == 0 && threadIdx.x == 0; }
+// ---------------- one case or a batch of cases per launch ----------------
+// A kernel runs either on one case (its CaseParams is a kernel argument and the grid is the
+// case's own) or on a batch: the grid is the concatenation of every case's workgroups, and a
+// workgroup finds its case in a prefix table.  Either way the kernel body sees the case's
+// parameters and the workgroup's position inside that case's own grid.
+enum { MAP_CELL = 0, MAP_TILE = 1, MAP_ONE = 2, MAP_COPY = 3, MAP_ZMARCH = 4, MAP_KINDS = 5 };
+
+struct BatchMap {
+    int ncase;
+    const CaseParams *P;            // device array, one per case
+    const int *start[MAP_KINDS];    // prefix sums of the workgroups per case (ncase + 1)
+    const int *dimx[MAP_KINDS];     // x extent of each case's grid
+};
+
+struct Blk {
+    int x, y, z;      // workgroup position in its case's grid
+    int gx, gy, gz;   // that grid's extents
+    int flat;         // x + gx (y + gy z)
+    int n;            // workgroups in the case's grid
+};
+
+__device__ __forceinline__ int findCase(const int *start, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool BATCH>
+__device__ __forceinline__ const CaseParams &caseOf(const CaseParams &P0, const BatchMap &M, int kind, Blk &b) {
+    if constexpr (BATCH) {
+        const int c = __builtin_amdgcn_readfirstlane(findCase(M.start[kind], M.ncase, blockIdx.x));
+        const int s0 = M.start[kind][c];
+        const int n = M.start[kind][c + 1] - s0;
+        const int gx = M.dimx[kind][c];
+        const int loc = blockIdx.x - s0;
+        b.gx = gx;
+        b.gy = n / gx;
+        b.gz = 1;
+        b.x = loc % gx;
+        b.y = loc / gx;
+        b.z = 0;
+        b.flat = loc;
+        b.n = n;
+        return M.P[c];
+    } else {
+        b.x = blockIdx.x; b.y = blockIdx.y; b.z = blockIdx.z;
+        b.gx = gridDim.x; b.gy = gridDim.y; b.gz = gridDim.z;
+        b.flat = b.x + b.gx * (b.y + b.gy * b.z);
+        b.n = b.gx * b.gy * b.gz;
+        return P0;
+    }
+}
+
+// Workgroups of `kind` of the case (batch) or nSingle (one case).
+template <bool BATCH>
+__device__ __forceinline__ int kindCount(const BatchMap &M, int kind, const CaseParams &P, int nSingle) {
+    if constexpr (BATCH) {
+        const int c = (int)(&P - M.P);
+        return M.start[kind][c + 1] - M.start[kind][c];
+    } else {
+        return nSingle;
+    }
+}
+
+__device__ __forceinline__ bool leader(const Blk &b) { return b.flat == 0 && threadIdx.x == 0; }
+
+// The field a smoother slot works on (u, v, w, p', refinement correction) and its source.
+__device__ __forceinline__ float *slotField(const CaseParams &P, int slot) {
+    return slot == 0 ? P.u : slot == 1 ? P.v : slot == 2 ? P.w : slot == 3 ? P.pc : P.tmp2;
+}
+__device__ __forceinline__ const float *slotSource(const CaseParams &P, int slot) {
+    return slot == 0 ? P.bu : slot == 1 ? P.bv : slot == 2 ? P.bw : slot == 3 ? P.bp : P.tmp;
+}
+__device__ __forceinline__ const float *slotCoef(const CaseParams &P, int slot) {
+    return slot < 3 ? P.mcoef : P.pcoef;
+}
 
 __device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int launch, int kk) {
     return P.spart + (size_t)((slot * 2 + (launch & 1)) * 2 + kk) * P.partStride;
@@ -174,7 +252,8 @@ __device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int 
 // tol.  A met test stops the call after the launch that contains the sweep (with one sweep per
 // launch: right after that sweep, as in the reference).  Workgroup 0 publishes the state after
 // launch L-1.  With L == number of launches this closes the call (sweeps count, last sum).
-__device__ __forceinline__ bool launchRuns(const CaseParams &P, int slot, int L, int kPrev, int nprev, double tol) {
+__device__ __forceinline__ bool launchRuns(const CaseParams &P, const Blk &b, int slot, int L, int kPrev, int nprev,
+                                           double tol) {
     if (L == 0) return true;
     SmoothState &c = P.st->sm[slot];
     const int pp = L & 1;                          // parity of the state after launch L-2
@@ -195,7 +274,7 @@ __device__ __forceinline__ bool launchRuns(const CaseParams &P, int slot, int L,
         }
         ran = before + kPrev;
     }
-    if (leader()) {
+    if (leader(b)) {
         const int q = (L - 1) & 1;
         c.stopped[q] = stopped ? 1 : 0;
         c.maxNorm[q] = mx;

@@ -1,19 +1,16 @@
-// ifl_kernels.hpp - host-side launchers of the gfx950 kernels (kernels.hip, sweep2d.hip).
+// ifl_kernels.hpp - host-side launchers of the gfx950 kernels (kernels.hip, sweep2d.hip,
+// sweep3d.hip).  Every launcher runs its kernel on one case or on a batch of cases (Lc).
 #pragma once
 
 #include "ifl_device.hpp"
 
 namespace ifl {
 
-// One equation of a fused sweep: read `in`, write `out` (ping-pong), source `src`.
-struct SweepEq {
-    const float *in;
-    float *out;
-    const float *src;
-    float rlx;
-    double tol;
+// One equation of a smoother launch.  The field, its ping-pong partner, source and matrix follow
+// from the slot (u, v, w, p', refinement correction); launch parity selects the direction.
+struct SweepSel {
     int slot;
-    int eq;
+    int q;       // equation index for relax / tol (u, v, w, p')
     int k;       // sweeps of this equation in this launch (0..K)
     int kPrev;   // sweeps of this equation in the previous launch of the call
 };
@@ -22,30 +19,38 @@ enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2 };
 enum { TILE_AUTO = 0, TILE_64x16 = 1, TILE_64x8 = 2, TILE_32x8 = 3, TILE_32x4 = 4 };
 enum { RED_MOM = 1, RED_MASS = 2, RED_UPD = 4, RED_FINISH = 8 };
 
-int cell_blocks(const CaseParams &P);
-int rb_blocks(const CaseParams &P);
-int tile_count(int nx, int ny, int tile);
-int default_tile(int nx, int ny);
+// Launch context: one case (P, grids from P) or a batch (device map, total workgroups per kind).
+struct Lc {
+    const CaseParams *P = nullptr;   // the case (batch: any case, only its dim/tile are read)
+    BatchMap M{};
+    bool batch = false;
+    int total[MAP_KINDS] = {0, 0, 0, 0, 0};
+    int tile = TILE_64x16;           // 2D sweep tile
+    hipStream_t s = nullptr;
+};
 
-void launch_init_faces(const CaseParams &P, hipStream_t s);
-void launch_face_bc(const CaseParams &P, hipStream_t s);
-void launch_mom_assemble(const CaseParams &P, hipStream_t s);
-void launch_rhie_chow(const CaseParams &P, hipStream_t s);
-void launch_pcorr_assemble(const CaseParams &P, hipStream_t s);
-void launch_update(const CaseParams &P, hipStream_t s);
-void launch_reduce_cells(const CaseParams &P, int mask, hipStream_t s);
-void launch_pcorr_defect(const CaseParams &P, hipStream_t s);
-void launch_pcorr_accumulate(const CaseParams &P, hipStream_t s);
-void launch_sweep_rb_half(const CaseParams &P, float *fld, const float *coef, const float *src, float rlx, double tol,
-                          int slot, int sweep, hipStream_t s);
-void launch_sweep_jacobi(const CaseParams &P, const float *in, float *out, const float *coef, const float *src,
-                         float rlx, double tol, int slot, int sweep, hipStream_t s);
-void launch_smooth_finish(const CaseParams &P, int slot, int eq, int nL, int kLast, int nprev, double tol,
-                          hipStream_t s);
-void launch_rb_sweep2d(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1, int neq,
-                       int launch, int block, int tile, hipStream_t s);
-void launch_copy(const CaseParams &P, float *dst, const float *src, hipStream_t s);
+int cell_blocks(const CaseParams &P);
+int copy_blocks(const CaseParams &P);
+int tile_count(int nx, int ny, int tile);
+int tile_dimx(int nx, int tile);
+int default_tile(int nx, int ny);
 int zmarch_blocks(const CaseParams &P);
-void launch_rb_sweep3d(const CaseParams &P, const float *coef, const SweepEq &e, int launch, hipStream_t s);
+
+void launch_init_faces(const Lc &L);
+void launch_face_bc(const Lc &L);
+void launch_mom_assemble(const Lc &L);
+void launch_rhie_chow(const Lc &L);
+void launch_pcorr_assemble(const Lc &L);
+void launch_update(const Lc &L);
+void launch_pcorr_defect(const Lc &L);
+void launch_pcorr_accumulate(const Lc &L);
+void launch_reduce_cells(const Lc &L, int mask);
+void launch_sweep_jacobi(const Lc &L, const SweepSel &e, int launch);
+void launch_smooth_finish(const Lc &L, int slot, int q, int nL, int kLast, int sweepKind, int nprev);
+void launch_copy_back(const Lc &L, int slot);
+// 2D fused red-black sweeps (K = block sweeps per launch), one or two equations sharing a matrix
+void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block);
+// 3D z-marching red-black sweep (one case)
+void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch);
 
 }  // namespace ifl

@@ -17,10 +17,10 @@ namespace ifl {
 
 // Visit the cells of this workgroup: f(i, j, k, c).
 template <class F>
-__device__ __forceinline__ void forCells(const CaseParams &P, F &&f) {
+__device__ __forceinline__ void forCells(const CaseParams &P, const Blk &b, F &&f) {
     const int rows = P.ny * P.nz;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
+    const int i = b.x * 64 + (threadIdx.x & 63);
+    const int r0 = b.y * (4 * kRows) + (threadIdx.x >> 6);
 #pragma unroll
     for (int q = 0; q < kRows; ++q) {
         const int r = r0 + 4 * q;
@@ -33,9 +33,11 @@ __device__ __forceinline__ void forCells(const CaseParams &P, F &&f) {
 }
 
 // ---------------- initFaceVel, kernels.cuh:10-45 ----------------
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_init_faces(CaseParams P) {
-    forCells(P, [&](int i, int j, int k, int c) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_init_faces(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         if (i >= 1) P.uf[ufi(P, i, j, k)] = (P.u[c - 1] + P.u[c]) / 2;
         if (j >= 1) P.vf[vfi(P, i, j, k)] = (P.v[c - P.nx] + P.v[c]) / 2;
         if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = (P.w[c - P.nx * P.ny] + P.w[c]) / 2;
@@ -56,10 +58,12 @@ __device__ __forceinline__ void applyFaceBCs(const CaseParams &P, int i, int j, 
     }
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_face_bc(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_face_bc(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
-    forCells(P, [&](int i, int j, int k, int c) { applyFaceBCs<DIM>(P, i, j, k, c); });
+    forCells(P, blk, [&](int i, int j, int k, int c) { applyFaceBCs<DIM>(P, i, j, k, c); });
 }
 
 // ---------------- momentum assembly ----------------
@@ -112,13 +116,15 @@ __device__ __forceinline__ double momResidual(const CaseParams &P, const float *
     return r;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
-    if (leader())
+    if (leader(blk))
         for (int q = 0; q < 4; ++q) P.st->sweepsIter[q] = 0;   // a new outer iteration starts
     double rs[3] = {0.0, 0.0, 0.0};
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         const int nx = P.nx, ny = P.ny, nz = P.nz;
         applyFaceBCs<DIM>(P, i, j, k, c);
         // face velocities (boundary faces hold the values just applied)
@@ -157,13 +163,12 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P) {
             rs[2] += r2 * r2;
         }
     });
-    storeBlockSums<3>(rs, P.partMom + 3 * flatBlock());
+    storeBlockSums<3>(rs, P.partMom + 3 * blk.flat);
 }
 
-// ---------------- red-black Gauss-Seidel half sweep (3D) ----------------
-// GaussSeidelIterateKernel (kernels.cuh:926-1005) restricted to cells of one colour
-// ((i+j+k)&1 == colour), so each half sweep is order independent and the full sweep
-// (red then black) is deterministic.  Neighbours outside the grid are 0 (:939-974).
+// ---------------- one relaxed Gauss-Seidel/Jacobi cell update ----------------
+// GaussSeidelIterateKernel / pointJacobiIterateKernel (kernels.cuh:926-1005, 845-924) for one
+// cell reading `fld`; neighbours outside the grid are 0 (:939-974).
 template <int DIM>
 __device__ __forceinline__ float gsUpdate(const CaseParams &P, const float *fld, const float *coef, const float *src,
                                           int i, int j, int k, int c, float rlx, float &dPhi) {
@@ -185,48 +190,23 @@ __device__ __forceinline__ float gsUpdate(const CaseParams &P, const float *fld,
     return phi_C + dPhi;
 }
 
-// Grid: x over (nx+1)/2 colour cells of a row (64 per workgroup), y over rows (16 per workgroup).
-// Partials of the red launch occupy records [0, nb), the black launch [nb, 2nb).
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_sweep_rb(CaseParams P, float *fld, const float *coef, const float *src,
-                                                     float rlx, double tol, int slot, int sweep, int color) {
-    if (caseDone(P)) return;
-    const int nb = gridDim.x * gridDim.y;
-    if (!launchRuns(P, slot, sweep, 1, 2 * nb, tol)) return;
-    const int rows = P.ny * P.nz;
-    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
-    double s[1] = {0.0};
-#pragma unroll
-    for (int q = 0; q < kRows; ++q) {
-        const int r = r0 + 4 * q;
-        if (r < rows) {
-            const int k = r / P.ny;
-            const int j = r - k * P.ny;
-            const int i = 2 * m + ((j + k + color) & 1);
-            if (i < P.nx) {
-                const int c = i + P.nx * r;
-                float d;
-                fld[c] = gsUpdate<DIM>(P, fld, coef, src, i, j, k, c, rlx, d);
-                s[0] += (double)d * (double)d;
-            }
-        }
-    }
-    storeBlockSums<1>(s, sweepPart(P, slot, sweep, 0) + color * nb + flatBlock());
-}
-
 // ---------------- point Jacobi sweep, kernels.cuh:845-924 ----------------
 // Reads `in`, writes `out` (ping-pong; a sweep that does not run copies), so after an even number
 // of launches the result is in the caller's field and an odd count ends with a copy back:
 // the reference's pointer swap (Solver.cu:894-896) with the result always in the caller's field.
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const float *in, float *out, const float *coef,
-                                                         const float *src, float rlx, double tol, int slot, int sweep) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P0, BatchMap M, SweepSel e, int launch) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
-    const int nb = gridDim.x * gridDim.y;
-    const bool run = launchRuns(P, slot, sweep, 1, nb, tol);
+    const bool run = e.k > 0 && launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q]);
+    float *fld = slotField(P, e.slot), *alt = P.alt[e.slot];
+    const float *in = (launch & 1) ? alt : fld;
+    float *out = (launch & 1) ? fld : alt;
+    const float *coef = slotCoef(P, e.slot), *src = slotSource(P, e.slot);
+    const float rlx = P.relax[e.q];
     double s[1] = {0.0};
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         if (run) {
             float d;
             out[c] = gsUpdate<DIM>(P, in, coef, src, i, j, k, c, rlx, d);
@@ -235,16 +215,29 @@ __global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P, const flo
             out[c] = in[c];
         }
     });
-    if (run) storeBlockSums<1>(s, sweepPart(P, slot, sweep, 0) + flatBlock());
+    if (run) storeBlockSums<1>(s, sweepPart(P, e.slot, launch, 0) + blk.flat);
 }
 
 // Closes a smoother call of nL launches (the last one ran kLast sweeps): publishes the state
 // after the last launch and adds the sweeps that ran to the outer iteration's count.
-__global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P, int slot, int eq, int nL, int kLast, int nprev,
-                                                          double tol) {
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P0, BatchMap M, int slot, int q, int nL, int kLast,
+                                                          int sweepKind, int nprev) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ONE, blk);
     if (caseDone(P)) return;
-    launchRuns(P, slot, nL, kLast, nprev, tol);
-    if (threadIdx.x == 0) P.st->sweepsIter[eq] += P.st->sm[slot].sweeps;
+    launchRuns(P, blk, slot, nL, kLast, kindCount<BATCH>(M, sweepKind, P, nprev), P.tol[q]);
+    if (threadIdx.x == 0) P.st->sweepsIter[q] += P.st->sm[slot].sweeps;
+}
+
+// A sweep call that ended after an odd number of launches left its result in the partner buffer.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P0, BatchMap M, int slot) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_COPY, blk);
+    if (caseDone(P)) return;
+    const int t = blk.flat * kBlock + threadIdx.x;
+    if (t < P.N) slotField(P, slot)[t] = P.alt[slot][t];
 }
 
 // ---------------- Rhie-Chow, kernels.cuh:336-500, then face BCs (Solver.cu:187) ----------------
@@ -273,11 +266,13 @@ __device__ __forceinline__ float rhieChow(const CaseParams &P, int d, int c, int
            + 0.5 * (1 / aC_L + 1 / aC_R) * (p_L - p_R) * area;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
     const float *aC = P.mcoef;   // plane sC
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
         if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, c, j, P.ny, P.nx, P.v, aC);
         if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, c, k, P.nz, P.nx * P.ny, P.w, aC);
@@ -286,11 +281,13 @@ __global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P) {
 }
 
 // ---------------- pressure-correction assembly, kernels.cuh:502-581, and p' = 0 ----------------
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
     double s[1] = {0.0};
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         const int N = P.N, pl = P.nx * P.ny;
         const float *uC = P.mcoef;
         const float rho = P.rho, Ax = P.Ax, Ay = P.Ay, Az = P.Az;
@@ -323,14 +320,16 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P) {
         P.pc[c] = 0.0f;                       // Solver.cu:193
         s[0] += (double)b * (double)b;
     });
-    storeBlockSums<1>(s, P.partMass + flatBlock());
+    storeBlockSums<1>(s, P.partMass + blk.flat);
 }
 
 // ---------------- FP64 defect of the p' system (refinement), delta = 0 ----------------
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         float a[7];
 #pragma unroll
         for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[c + P.N * s] : 0.0f;
@@ -339,9 +338,12 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P) {
     });
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcorr_accumulate(CaseParams P) {
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_pcorr_accumulate(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
-    forCells(P, [&](int, int, int, int c) { P.pc[c] += P.tmp2[c]; });
+    forCells(P, blk, [&](int, int, int, int c) { P.pc[c] += P.tmp2[c]; });
 }
 
 // ---------------- updateField, kernels.cuh:583-843 ----------------
@@ -369,11 +371,13 @@ __device__ __forceinline__ float faceCorr(const CaseParams &P, int d, int c, int
     return P.relax[d] * 0.5 * (1 / aC_L + 1 / aC_R) * (P.pc[c - st] - P.pc[c]) * area;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_update(CaseParams P) {
+template <int DIM, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
-    forCells(P, [&](int i, int j, int k, int c) {
+    forCells(P, blk, [&](int i, int j, int k, int c) {
         const int pl = P.nx * P.ny;
         const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
         P.p[c] += dp;
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_update(CaseParams P) {
             s[6] += (double)d * d;
         }
     });
-    storeBlockSums<7>(s, P.partUpd + 7 * flatBlock());
+    storeBlockSums<7>(s, P.partUpd + 7 * blk.flat);
 }
 
 // ---------------- outer-iteration monitor: Solver.cu:226-268 (FP64) + scaled residual ----------------
@@ -486,8 +490,12 @@ __device__ void finishOuter(const CaseParams &P) {
 }
 
 // One workgroup: totals of the cell-kernel partials (fixed order), then (optionally) the monitor.
-__global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P, int nblk, int mask) {
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P0, BatchMap M, int nSingle, int mask) {
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ONE, blk);
     if (caseDone(P)) return;
+    const int nblk = kindCount<BATCH>(M, MAP_CELL, P, nSingle);
     CaseState &S = *P.st;
     if (mask & RED_MOM) {
         double t[3];
@@ -517,49 +525,50 @@ int cell_blocks(const CaseParams &P) {
     dim3 g = cellGrid(P);
     return (int)(g.x * g.y);
 }
-static dim3 rbGrid(const CaseParams &P) {
-    const int hx = (P.nx + 1) / 2;
-    return dim3((unsigned)((hx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
-}
-int rb_blocks(const CaseParams &P) {
-    dim3 g = rbGrid(P);
-    return (int)(g.x * g.y);
+int copy_blocks(const CaseParams &P) { return (P.N + kBlock - 1) / kBlock; }
+
+static dim3 gridOf(const Lc &L, int kind) {
+    if (L.batch) return dim3((unsigned)L.total[kind]);
+    switch (kind) {
+        case MAP_CELL: return cellGrid(*L.P);
+        case MAP_ONE: return dim3(1);
+        case MAP_COPY: return dim3((unsigned)copy_blocks(*L.P));
+        default: return dim3(1);
+    }
 }
 
-#define IFL_DISPATCH(DIMV, KERNEL, GRID, STREAM, ...)                                   \
-    do {                                                                               \
-        if ((DIMV) == 3) KERNEL<3><<<(GRID), kBlock, 0, (STREAM)>>>(__VA_ARGS__);      \
-        else KERNEL<2><<<(GRID), kBlock, 0, (STREAM)>>>(__VA_ARGS__);                  \
+#define IFL_LAUNCH(L, KIND, KERNEL, ...)                                                               \
+    do {                                                                                              \
+        if ((L).batch) KERNEL<true><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__);  \
+        else KERNEL<false><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__);          \
+    } while (0)
+#define IFL_LAUNCH_DIM(L, KIND, KERNEL, ...)                                                           \
+    do {                                                                                              \
+        const int d_ = (L).P->dim;                                                                    \
+        if ((L).batch) {                                                                              \
+            if (d_ == 3) KERNEL<3, true><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__); \
+            else KERNEL<2, true><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__);         \
+        } else {                                                                                      \
+            if (d_ == 3) KERNEL<3, false><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__); \
+            else KERNEL<2, false><<<gridOf((L), (KIND)), kBlock, 0, (L).s>>>(*(L).P, (L).M, ##__VA_ARGS__);         \
+        }                                                                                             \
     } while (0)
 
-void launch_init_faces(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_init_faces, cellGrid(P), s, P); }
-void launch_face_bc(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_face_bc, cellGrid(P), s, P); }
-void launch_mom_assemble(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_mom_assemble, cellGrid(P), s, P); }
-void launch_rhie_chow(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_rhie_chow, cellGrid(P), s, P); }
-void launch_pcorr_assemble(const CaseParams &P, hipStream_t s) {
-    IFL_DISPATCH(P.dim, k_pcorr_assemble, cellGrid(P), s, P);
+void launch_init_faces(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_init_faces); }
+void launch_face_bc(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_face_bc); }
+void launch_mom_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_mom_assemble); }
+void launch_rhie_chow(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_rhie_chow); }
+void launch_pcorr_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_assemble); }
+void launch_update(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_update); }
+void launch_pcorr_defect(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_defect); }
+void launch_pcorr_accumulate(const Lc &L) { IFL_LAUNCH(L, MAP_CELL, k_pcorr_accumulate); }
+void launch_reduce_cells(const Lc &L, int mask) { IFL_LAUNCH(L, MAP_ONE, k_reduce_cells, cell_blocks(*L.P), mask); }
+void launch_sweep_jacobi(const Lc &L, const SweepSel &e, int launch) {
+    IFL_LAUNCH_DIM(L, MAP_CELL, k_sweep_jacobi, e, launch);
 }
-void launch_update(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_update, cellGrid(P), s, P); }
-void launch_reduce_cells(const CaseParams &P, int mask, hipStream_t s) {
-    k_reduce_cells<<<1, kBlock, 0, s>>>(P, cell_blocks(P), mask);
+void launch_smooth_finish(const Lc &L, int slot, int q, int nL, int kLast, int sweepKind, int nprev) {
+    IFL_LAUNCH(L, MAP_ONE, k_smooth_finish, slot, q, nL, kLast, sweepKind, nprev);
 }
-void launch_pcorr_defect(const CaseParams &P, hipStream_t s) { IFL_DISPATCH(P.dim, k_pcorr_defect, cellGrid(P), s, P); }
-void launch_pcorr_accumulate(const CaseParams &P, hipStream_t s) {
-    k_pcorr_accumulate<<<cellGrid(P), kBlock, 0, s>>>(P);
-}
-
-void launch_sweep_rb_half(const CaseParams &P, float *fld, const float *coef, const float *src, float rlx, double tol,
-                          int slot, int sweep, hipStream_t s) {
-    for (int color = 0; color < 2; ++color)
-        IFL_DISPATCH(P.dim, k_sweep_rb, rbGrid(P), s, P, fld, coef, src, rlx, tol, slot, sweep, color);
-}
-void launch_sweep_jacobi(const CaseParams &P, const float *in, float *out, const float *coef, const float *src,
-                         float rlx, double tol, int slot, int sweep, hipStream_t s) {
-    IFL_DISPATCH(P.dim, k_sweep_jacobi, cellGrid(P), s, P, in, out, coef, src, rlx, tol, slot, sweep);
-}
-void launch_smooth_finish(const CaseParams &P, int slot, int eq, int nL, int kLast, int nprev, double tol,
-                          hipStream_t s) {
-    k_smooth_finish<<<1, kBlock, 0, s>>>(P, slot, eq, nL, kLast, nprev, tol);
-}
+void launch_copy_back(const Lc &L, int slot) { IFL_LAUNCH(L, MAP_COPY, k_copy_back, slot); }
 
 }  // namespace ifl

@@ -24,9 +24,8 @@
 
 namespace ifl {
 
-template <int TX, int TY, int K, int NEQ, int KIND>
-__global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float *coef, SweepEq e0, SweepEq e1,
-                                                      int launch) {
+template <int TX, int TY, int K, int NEQ, int KIND, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1, int launch) {
     constexpr int H = 2 * K;                                   // field halo
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H;
@@ -43,13 +42,27 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
     __shared__ float srinv[PC ? HY : 1][PC ? HX : 1];
     __shared__ float suf[MOM ? UY : 1][MOM ? UX : 1];
     __shared__ float svf[MOM ? VY : 1][MOM ? VX : 1];
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_TILE, blk);
     if (caseDone(P)) return;
     const int nx = P.nx, ny = P.ny;
     const int tilesX = (nx + TX - 1) / TX;
-    const int by = blockIdx.x / tilesX;
-    const int bx = blockIdx.x - by * tilesX;
+    const int by = blk.flat / tilesX;
+    const int bx = blk.flat - by * tilesX;
     const int x0 = bx * TX, y0 = by * TY;
-    const SweepEq eqs[2] = {e0, e1};
+    const SweepSel eqs[2] = {e0, e1};
+    const float *in[2], *src[2];
+    float *out[2];
+    float rlx[2];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) {
+        float *fld = slotField(P, eqs[q].slot), *alt = P.alt[eqs[q].slot];
+        in[q] = (launch & 1) ? alt : fld;
+        out[q] = (launch & 1) ? fld : alt;
+        src[q] = slotSource(P, eqs[q].slot);
+        rlx[q] = P.relax[eqs[q].q];
+    }
+    const float *coef = PC ? P.rinv : slotCoef(P, eqs[0].slot);
     const int tid = threadIdx.x;
 
     // ---- every global load of the tile first ----
@@ -59,20 +72,20 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
         const int e = tid + m * kBlock;
         const int ly = e / HX, lx = e - ly * HX;
         const int gx = x0 - H + lx, gy = y0 - H + ly;
-        const bool in = e < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool ok = e < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         const int g = gy * nx + gx;
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) rphi[q][m] = in ? eqs[q].in[g] : 0.0f;
-        if constexpr (PC) rr[m] = in ? coef[g] : 0.0f;
+        for (int q = 0; q < NEQ; ++q) rphi[q][m] = ok ? in[q][g] : 0.0f;
+        if constexpr (PC) rr[m] = ok ? coef[g] : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
         const int e = tid + m * kBlock;
         const int ly = e / SX, lx = e - ly * SX;
         const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
-        const bool in = e < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool ok = e < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) rsrc[q][m] = in ? eqs[q].src[gy * nx + gx] : 0.0f;
+        for (int q = 0; q < NEQ; ++q) rsrc[q][m] = ok ? src[q][gy * nx + gx] : 0.0f;
     }
     float ruf[NUT], rvf[NVT];
     if constexpr (MOM) {
@@ -99,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
     int kq[2] = {0, 0};
 #pragma unroll
     for (int q = 0; q < NEQ; ++q) {
-        act[q] = eqs[q].k > 0 && launchRuns(P, eqs[q].slot, launch, eqs[q].kPrev, gridDim.x, eqs[q].tol);
+        act[q] = eqs[q].k > 0 && launchRuns(P, blk, eqs[q].slot, launch, eqs[q].kPrev, blk.n, P.tol[eqs[q].q]);
         kq[q] = act[q] ? eqs[q].k : 0;
     }
 
@@ -174,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
             float newPhi = ssrc[q][sy][sx] - a[sE] * sphi[q][ly][lx + 1] - a[sW] * sphi[q][ly][lx - 1]
                            - a[sN] * sphi[q][ly + 1][lx] - a[sS] * sphi[q][ly - 1][lx];
             newPhi /= a[sC];
-            const float d = eqs[q].rlx * (newPhi - phi_C);
+            const float d = rlx[q] * (newPhi - phi_C);
             sphi[q][ly][lx] = phi_C + d;
             if (owned) {
 #pragma unroll
@@ -206,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
         if (gx < nx && gy < ny) {
             const int g = gy * nx + gx;
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) eqs[q].out[g] = sphi[q][r + H][xx + H];
+            for (int q = 0; q < NEQ; ++q) out[q][g] = sphi[q][r + H][xx + H];
         }
     }
 #pragma unroll
@@ -215,49 +228,40 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P, const float 
         for (int j = 0; j < K; ++j)
             if (j < kq[q]) {
                 double a1[1] = {acc[j][q]};
-                storeBlockSums<1>(a1, sweepPart(P, eqs[q].slot, launch, j) + blockIdx.x);
+                storeBlockSums<1>(a1, sweepPart(P, eqs[q].slot, launch, j) + blk.flat);
             }
 }
 
-__global__ __launch_bounds__(kBlock) void k_copy(CaseParams P, float *dst, const float *src) {
-    if (caseDone(P)) return;
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < P.N) dst[t] = src[t];
-}
-
-template <int TX, int TY, int K>
-static void launchTile(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1,
-                       int neq, int launch, hipStream_t s) {
-    const unsigned grid = (unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY));
+template <int TX, int TY, int K, bool BATCH>
+static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch) {
+    const CaseParams &P = *L.P;
+    const dim3 grid = L.batch ? dim3((unsigned)L.total[MAP_TILE])
+                              : dim3((unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY)));
     if (neq == 2) {
-        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM><<<grid, kBlock, 0, s>>>(P, coef, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 2, COEF_STORED><<<grid, kBlock, 0, s>>>(P, coef, e0, e1, launch);
+        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else k_rb_tile2d<TX, TY, K, 2, COEF_STORED, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     } else {
-        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM><<<grid, kBlock, 0, s>>>(P, coef, e0, e1, launch);
-        else if (kind == COEF_PCORR) k_rb_tile2d<TX, TY, K, 1, COEF_PCORR><<<grid, kBlock, 0, s>>>(P, coef, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 1, COEF_STORED><<<grid, kBlock, 0, s>>>(P, coef, e0, e1, launch);
+        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else if (kind == COEF_PCORR) k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else k_rb_tile2d<TX, TY, K, 1, COEF_STORED, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     }
 }
 
-template <int K>
-static void launchK(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1, int neq,
-                    int launch, int tile, hipStream_t s) {
-    switch (tile) {
-        case TILE_64x16: launchTile<64, 16, K>(P, kind, coef, e0, e1, neq, launch, s); break;
-        case TILE_64x8: launchTile<64, 8, K>(P, kind, coef, e0, e1, neq, launch, s); break;
-        case TILE_32x8: launchTile<32, 8, K>(P, kind, coef, e0, e1, neq, launch, s); break;
-        default: launchTile<32, 4, K>(P, kind, coef, e0, e1, neq, launch, s); break;
+template <int K, bool BATCH>
+static void launchK(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch) {
+    switch (L.tile) {
+        case TILE_64x16: launchTile<64, 16, K, BATCH>(L, kind, e0, e1, neq, launch); break;
+        case TILE_64x8: launchTile<64, 8, K, BATCH>(L, kind, e0, e1, neq, launch); break;
+        case TILE_32x8: launchTile<32, 8, K, BATCH>(L, kind, e0, e1, neq, launch); break;
+        default: launchTile<32, 4, K, BATCH>(L, kind, e0, e1, neq, launch); break;
     }
 }
 
-int tile_count(int nx, int ny, int tile) {
-    switch (tile) {
-        case TILE_64x16: return ((nx + 63) / 64) * ((ny + 15) / 16);
-        case TILE_64x8: return ((nx + 63) / 64) * ((ny + 7) / 8);
-        case TILE_32x8: return ((nx + 31) / 32) * ((ny + 7) / 8);
-        default: return ((nx + 31) / 32) * ((ny + 3) / 4);
-    }
-}
+static int tileX(int tile) { return (tile == TILE_64x16 || tile == TILE_64x8) ? 64 : 32; }
+static int tileY(int tile) { return tile == TILE_64x16 ? 16 : tile == TILE_32x4 ? 4 : 8; }
+
+int tile_dimx(int nx, int tile) { return (nx + tileX(tile) - 1) / tileX(tile); }
+int tile_count(int nx, int ny, int tile) { return tile_dimx(nx, tile) * ((ny + tileY(tile) - 1) / tileY(tile)); }
 
 int default_tile(int nx, int ny) {
     if (tile_count(nx, ny, TILE_64x16) >= 768) return TILE_64x16;
@@ -266,14 +270,14 @@ int default_tile(int nx, int ny) {
     return TILE_32x4;
 }
 
-void launch_rb_sweep2d(const CaseParams &P, int kind, const float *coef, const SweepEq &e0, const SweepEq &e1, int neq,
-                       int launch, int block, int tile, hipStream_t s) {
-    if (block == 2) launchK<2>(P, kind, coef, e0, e1, neq, launch, tile, s);
-    else launchK<1>(P, kind, coef, e0, e1, neq, launch, tile, s);
-}
-
-void launch_copy(const CaseParams &P, float *dst, const float *src, hipStream_t s) {
-    k_copy<<<(unsigned)((P.N + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, dst, src);
+void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block) {
+    if (L.batch) {
+        if (block == 2) launchK<2, true>(L, kind, e0, e1, neq, launch);
+        else launchK<1, true>(L, kind, e0, e1, neq, launch);
+    } else {
+        if (block == 2) launchK<2, false>(L, kind, e0, e1, neq, launch);
+        else launchK<1, false>(L, kind, e0, e1, neq, launch);
+    }
 }
 
 }  // namespace ifl

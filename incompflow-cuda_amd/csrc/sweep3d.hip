@@ -20,18 +20,25 @@ namespace ifl {
 constexpr int Z_TX = 64, Z_TY = 8, Z_ZC = 16;
 
 template <int TX, int TY, int ZC>
-__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, const float *coef, SweepEq e, int launch) {
+__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, SweepSel e, int launch) {
     constexpr int HX = TX + 4, HY = TY + 4, PL = HX * HY;
     constexpr int NPL = (PL + kBlock - 1) / kBlock;
     constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;       // red cells of tile + ring per plane
     constexpr int BW = TX / 2, NB = TY * BW;                 // black cells of the tile per plane
     __shared__ float win[4][HY][HX];                         // planes z mod 4
+    Blk blk;
+    caseOf<false>(P, BatchMap{}, MAP_ZMARCH, blk);
     if (caseDone(P)) return;
-    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N, pln = nx * ny;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * ZC;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
+    float *fld = slotField(P, e.slot), *alt = P.alt[e.slot];
+    const float *ein = (launch & 1) ? alt : fld;
+    float *eout = (launch & 1) ? fld : alt;
+    const float *esrc = slotSource(P, e.slot), *coef = slotCoef(P, e.slot);
+    const float erlx = P.relax[e.q];
+    const int x0 = blk.x * TX, y0 = blk.y * TY, z0 = blk.z * ZC;
     const int z1 = min(z0 + ZC, nz);
     const int tid = threadIdx.x;
-    const int nblk = gridDim.x * gridDim.y * gridDim.z;
+
 
     // issue the loads of the first four planes, then decide whether this sweep runs
     float reg[4][NPL];
@@ -42,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, const floa
             const int ly = q / HX, lx = q - ly * HX;
             const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
             const bool in = q < PL && z >= 0 && z < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-            r[m] = in ? e.in[gx + nx * (gy + ny * z)] : 0.0f;
+            r[m] = in ? ein[gx + nx * (gy + ny * z)] : 0.0f;
         }
     };
     auto storePlane = [&](int z, const float (&r)[NPL]) {
@@ -55,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, const floa
     };
 #pragma unroll
     for (int s = 0; s < 4; ++s) loadPlane(z0 - 2 + s, reg[s]);
-    const bool run = launchRuns(P, e.slot, launch, e.kPrev, nblk, e.tol);
+    const bool run = launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q]);
 #pragma unroll
     for (int s = 0; s < 4; ++s) storePlane(z0 - 2 + s, reg[s]);
     __syncthreads();
@@ -70,14 +77,13 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, const floa
         const float phi_N = wc[ly + 1][lx], phi_S = wc[ly - 1][lx];
         const float phi_T = (z != nz - 1) ? win[(z + 1) & 3][ly][lx] : 0.0f;
         const float phi_B = (z != 0) ? win[(z - 1) & 3][ly][lx] : 0.0f;
-        float newPhi = e.src[c] - coef[c + N * sE] * phi_E - coef[c + N * sW] * phi_W - coef[c + N * sN] * phi_N
+        float newPhi = esrc[c] - coef[c + N * sE] * phi_E - coef[c + N * sW] * phi_W - coef[c + N * sN] * phi_N
                        - coef[c + N * sS] * phi_S;
         newPhi -= coef[c + N * sT] * phi_T + coef[c + N * sB] * phi_B;
         newPhi /= coef[c];
-        const float d = e.rlx * (newPhi - phi_C);
+        const float d = erlx * (newPhi - phi_C);
         wc[ly][lx] = phi_C + d;
         if (owned) acc += (double)d * (double)d;
-        (void)pln;
     };
     auto redPlane = [&](int z) {
         if (z < 0 || z >= nz) return;
@@ -123,13 +129,13 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, const floa
         for (int q = tid; q < TX * TY; q += kBlock) {
             const int r = q / TX, xx = q - r * TX;
             const int gx = x0 + xx, gy = y0 + r;
-            if (gx < nx && gy < ny) e.out[gx + nx * (gy + ny * t)] = win[t & 3][r + 2][xx + 2];
+            if (gx < nx && gy < ny) eout[gx + nx * (gy + ny * t)] = win[t & 3][r + 2][xx + 2];
         }
         __syncthreads();
     }
     if (run) {
         double a1[1] = {acc};
-        storeBlockSums<1>(a1, sweepPart(P, e.slot, launch, 0) + flatBlock());
+        storeBlockSums<1>(a1, sweepPart(P, e.slot, launch, 0) + blk.flat);
     }
 }
 
@@ -137,10 +143,11 @@ int zmarch_blocks(const CaseParams &P) {
     return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
 }
 
-void launch_rb_sweep3d(const CaseParams &P, const float *coef, const SweepEq &e, int launch, hipStream_t s) {
+void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch) {
+    const CaseParams &P = *L.P;
     dim3 grid((unsigned)((P.nx + Z_TX - 1) / Z_TX), (unsigned)((P.ny + Z_TY - 1) / Z_TY),
               (unsigned)((P.nz + Z_ZC - 1) / Z_ZC));
-    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC><<<grid, kBlock, 0, s>>>(P, coef, e, launch);
+    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC><<<grid, kBlock, 0, L.s>>>(P, e, launch);
 }
 
 }  // namespace ifl
