@@ -194,7 +194,8 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out);
 
 /* Kernels timed by ifl_time_kernel. */
 enum {
-    IFL_KERNEL_SWEEP = 0,         /* one full red-black sweep of the p' system (both colours) */
+    IFL_KERNEL_SWEEP = 0,         /* one launch of the p' smoother: sweep_block full red-black
+                                     sweeps (2D), one red-black sweep (3D) or one Jacobi sweep */
     IFL_KERNEL_MOM_ASSEMBLE = 1,
     IFL_KERNEL_RHIE_CHOW = 2,
     IFL_KERNEL_PCORR_ASSEMBLE = 3,
@@ -202,11 +203,33 @@ enum {
     IFL_KERNEL_COUNT = 5
 };
 
-/* Measurement hook (no reference counterpart): launch one kernel `reps` times back to back on
- * the handle's stream and return the mean time per launch in microseconds from HIP events
- * (for IFL_KERNEL_SWEEP, per half-sweep launch; *launches_per_rep receives the launches per
- * repetition).  Overwrites the handle's device state of that stage. */
+/* Measurement hook (no reference counterpart): launch one kernel `reps` times on the handle's
+ * stream, each between its own pair of HIP events, and return the mean device time per launch
+ * in microseconds (*launches_per_rep receives the launches per repetition, 1).  Overwrites the
+ * handle's device state of that stage. */
 int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *launches_per_rep);
+
+/* ---- Batched solves (no reference counterpart) ----
+ * A batch runs many independent cases, each with its own grid size, domain, boundary
+ * conditions, fluid properties, relaxation, tolerances and stop rule, through one outer
+ * iteration graph: every kernel launch covers the workgroups of all cases, so small grids
+ * fill the GPU together.  What fixes the launch sequence must agree across the cases: dim,
+ * smoother, n_iter[4], p_refine, sweep_block and tile.  A case stops on its own stop rule;
+ * its workgroups then return at once.  Every case's result (fields, iteration count,
+ * history) equals that of ifl_solve on the same case with the same tile, bit for bit. */
+typedef struct ifl_batch ifl_batch;
+
+/* Create n cases from cfgs[0..n-1]. */
+int ifl_batch_create(const ifl_config *cfgs, int n, ifl_batch **out);
+int ifl_batch_destroy(ifl_batch *b);
+int ifl_batch_size(const ifl_batch *b);
+/* Handle of case i for field access, face profiles and VTK output (owned by the batch: do not
+ * destroy; do not call ifl_solve on it while the batch solves). */
+int ifl_batch_case(ifl_batch *b, int i, ifl_solver **out);
+/* Run outer iterations until every case has stopped or max_iter (<=0: the largest
+ * num_outer_iter) iterations ran.  reports (optional, n entries) receive each case's report;
+ * elapsed_ms is the batch's wall time.  Returns IFL_ERR_DIVERGED if any case diverged. */
+int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports);
 
 #ifdef __cplusplus
 }

@@ -41,7 +41,7 @@ struct SmoothState {
 };
 
 struct CaseState {
-    int done;               // 0 running, 1 converged, 2 diverged: every kernel returns at once
+    int done;               // 0 running, 1 converged, 2 diverged, 3 iteration cap: every kernel returns at once
     int pad0;
     long long iter;         // outer iterations executed
     SmoothState sm[kSlots];
@@ -54,6 +54,7 @@ struct CaseState {
     double lastScaled;
     int lastConverged;
     int pad;
+    long long iterCap;      // > 0: stop (done 3) once iter reaches it (batched solves)
 };
 
 struct CaseParams {

@@ -35,6 +35,8 @@ int tile_count(int nx, int ny, int tile);
 int tile_dimx(int nx, int tile);
 int default_tile(int nx, int ny);
 int zmarch_blocks(const CaseParams &P);
+int zmarch_dimx(const CaseParams &P);
+int cell_dimx(const CaseParams &P);
 
 void launch_init_faces(const Lc &L);
 void launch_face_bc(const Lc &L);
@@ -50,7 +52,7 @@ void launch_smooth_finish(const Lc &L, int slot, int q, int nL, int kLast, int s
 void launch_copy_back(const Lc &L, int slot);
 // 2D fused red-black sweeps (K = block sweeps per launch), one or two equations sharing a matrix
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block);
-// 3D z-marching red-black sweep (one case)
+// 3D z-marching red-black sweep
 void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch);
 
 }  // namespace ifl

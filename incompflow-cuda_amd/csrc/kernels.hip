@@ -487,6 +487,7 @@ __device__ void finishOuter(const CaseParams &P) {
     S.iter += 1;
     if (!finite) S.done = 2;
     else if (conv) S.done = 1;
+    else if (S.iterCap > 0 && S.iter >= S.iterCap) S.done = 3;
 }
 
 // One workgroup: totals of the cell-kernel partials (fixed order), then (optionally) the monitor.
@@ -525,6 +526,7 @@ int cell_blocks(const CaseParams &P) {
     dim3 g = cellGrid(P);
     return (int)(g.x * g.y);
 }
+int cell_dimx(const CaseParams &P) { return (P.nx + 63) / 64; }
 int copy_blocks(const CaseParams &P) { return (P.N + kBlock - 1) / kBlock; }
 
 static dim3 gridOf(const Lc &L, int kind) {

@@ -19,23 +19,26 @@ namespace ifl {
 
 constexpr int Z_TX = 64, Z_TY = 8, Z_ZC = 16;
 
-template <int TX, int TY, int ZC>
-__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P, SweepSel e, int launch) {
+template <int TX, int TY, int ZC, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e, int launch) {
     constexpr int HX = TX + 4, HY = TY + 4, PL = HX * HY;
     constexpr int NPL = (PL + kBlock - 1) / kBlock;
     constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;       // red cells of tile + ring per plane
     constexpr int BW = TX / 2, NB = TY * BW;                 // black cells of the tile per plane
     __shared__ float win[4][HY][HX];                         // planes z mod 4
     Blk blk;
-    caseOf<false>(P, BatchMap{}, MAP_ZMARCH, blk);
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH, blk);
     if (caseDone(P)) return;
     const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
+    const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
+    const int bx = blk.flat % gxc, byz = blk.flat / gxc;
+    const int by = byz % gyc, bz = byz / gyc;
     float *fld = slotField(P, e.slot), *alt = P.alt[e.slot];
     const float *ein = (launch & 1) ? alt : fld;
     float *eout = (launch & 1) ? fld : alt;
     const float *esrc = slotSource(P, e.slot), *coef = slotCoef(P, e.slot);
     const float erlx = P.relax[e.q];
-    const int x0 = blk.x * TX, y0 = blk.y * TY, z0 = blk.z * ZC;
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
     const int z1 = min(z0 + ZC, nz);
     const int tid = threadIdx.x;
 
@@ -143,11 +146,17 @@ int zmarch_blocks(const CaseParams &P) {
     return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
 }
 
+int zmarch_dimx(const CaseParams &P) { return (P.nx + Z_TX - 1) / Z_TX; }
+
 void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch) {
     const CaseParams &P = *L.P;
+    if (L.batch) {
+        k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, true><<<dim3((unsigned)L.total[MAP_ZMARCH]), kBlock, 0, L.s>>>(P, L.M, e, launch);
+        return;
+    }
     dim3 grid((unsigned)((P.nx + Z_TX - 1) / Z_TX), (unsigned)((P.ny + Z_TY - 1) / Z_TY),
               (unsigned)((P.nz + Z_ZC - 1) / Z_ZC));
-    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC><<<grid, kBlock, 0, L.s>>>(P, e, launch);
+    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, false><<<grid, kBlock, 0, L.s>>>(P, L.M, e, launch);
 }
 
 }  // namespace ifl

@@ -206,3 +206,44 @@ def adjoint_channel_config(nx=1024, ny=256):
     c.num_outer_iter = 400000
     c.init[4] = 0.0
     return c
+
+
+def splitmix64(seed):
+    """SplitMix64 stream of uniforms in [0, 1) (53-bit mantissa)."""
+    m = (1 << 64) - 1
+    x = seed & m
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & m
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        yield (z >> 11) * (1.0 / (1 << 53))
+
+
+def uneven_batch_configs(n=64, seed=20261203, tol_residual=1e-4, max_iter=100000):
+    """Seeded batch of uneven 2D cases.  Per case, in draw order: type (u < 0.5 cavity, else
+    channel), nx = 16*(4 + floor(29u)), ny = 16*(4 + floor(29u)), Re = 10 + 990u; rho = 1,
+    mu = 1/Re, U = 1.  Cavity: unit square, lid u = 1 on the north wall.  Channel: height 1,
+    length nx/ny (square cells), uniform inlet u = 1 on the west face, east outlet, north/south
+    walls.  Scaled-residual stop rule."""
+    g = splitmix64(seed)
+    out = []
+    for _ in range(n):
+        is_cavity = next(g) < 0.5
+        nx = 16 * (4 + int(29 * next(g)))
+        ny = 16 * (4 + int(29 * next(g)))
+        re = 10.0 + 990.0 * next(g)
+        c = cavity_config(n=nx, re=re, p_refine=0, tol_residual=tol_residual, max_iter=max_iter)
+        c.nx, c.ny = nx, ny
+        if not is_cavity:
+            c.xmin, c.xmax, c.ymin, c.ymax = 0.0, nx / ny, 0.0, 1.0
+            for f in range(6):
+                c.bc_type[f] = WALL
+                for q in range(3):
+                    c.bc_val[f][q] = 0.0
+            c.bc_type[EAST] = OUTLET
+            c.bc_type[WEST] = INLET
+            c.bc_val[WEST][0] = 1.0
+        out.append(c)
+    return out

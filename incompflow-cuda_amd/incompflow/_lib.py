@@ -29,6 +29,11 @@ EXPORTS = [
     ("ifl_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
     ("ifl_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("ifl_batch_create", C.c_int, [C.POINTER(_abi.Config), C.c_int, C.POINTER(C.c_void_p)]),
+    ("ifl_batch_destroy", C.c_int, [C.c_void_p]),
+    ("ifl_batch_size", C.c_int, [C.c_void_p]),
+    ("ifl_batch_case", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    ("ifl_batch_solve", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.Report)]),
 ]
 
 _lib = None
@@ -66,15 +71,19 @@ def _check(st):
 class Solver:
     """One case on the GPU: the reference's Solver (include/Solver.h:46-52) over the C-ABI."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, _handle=None, _owner=None):
         self.cfg = cfg.copy()
-        self.h = C.c_void_p()
-        _check(lib().ifl_create(C.byref(self.cfg), C.byref(self.h)))
+        self._owner = _owner          # a Batch that owns the handle, or None
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = C.c_void_p()
+            _check(lib().ifl_create(C.byref(self.cfg), C.byref(self.h)))
 
     def close(self):
-        if self.h:
+        if self.h and self._owner is None:
             lib().ifl_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -133,3 +142,55 @@ class Solver:
 
     def write_vtk(self, path):
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))
+
+
+class Batch:
+    """Many independent cases solved together (ifl_batch_* in include/incompflow.h).
+
+    Cases may differ in grid size, domain, boundary conditions, fluid properties, relaxation,
+    tolerances and stop rule; dim, smoother, n_iter, p_refine, sweep_block and tile must agree.
+    ``cases[i]`` is a Solver view of case i (fields, profiles, VTK) owned by the batch."""
+
+    def __init__(self, cfgs):
+        cfgs = [c.copy() for c in cfgs]
+        arr = (_abi.Config * len(cfgs))(*cfgs)
+        self.h = C.c_void_p()
+        _check(lib().ifl_batch_create(arr, len(cfgs), C.byref(self.h)))
+        self.cases = []
+        for i, c in enumerate(cfgs):
+            hc = C.c_void_p()
+            _check(lib().ifl_batch_case(self.h, i, C.byref(hc)))
+            self.cases.append(Solver(c, _handle=hc, _owner=self))
+
+    def __len__(self):
+        return len(self.cases)
+
+    def close(self):
+        if self.h:
+            for s in self.cases:
+                s.h = C.c_void_p()
+            lib().ifl_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, max_iter=0):
+        """Run until every case stopped; returns (status, [dict per case])."""
+        reps = (_abi.Report * len(self.cases))()
+        st = lib().ifl_batch_solve(self.h, max_iter, reps)
+        if st not in (0, _abi.IFL_ERR_DIVERGED):
+            _check(st)
+        out = [{"iterations": r.iterations, "converged": bool(r.converged), "status": r.status,
+                "elapsed_ms": r.elapsed_ms, "final_scaled": r.final_scaled, "sweeps_total": r.sweeps_total}
+               for r in reps]
+        return st, out

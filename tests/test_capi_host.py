@@ -65,3 +65,42 @@ def test_null_arguments():
     assert L.ifl_create(None, None) == _abi.IFL_ERR_ARG
     assert L.ifl_destroy(None) == _abi.IFL_OK
     assert L.ifl_solve(None, 1, None, None, 0) == _abi.IFL_ERR_ARG
+    assert L.ifl_batch_destroy(None) == _abi.IFL_OK
+    assert L.ifl_batch_solve(None, 1, None) == _abi.IFL_ERR_ARG
+    assert L.ifl_batch_size(None) == 0
+    h = C.c_void_p()
+    assert L.ifl_batch_case(None, 0, C.byref(h)) == _abi.IFL_ERR_ARG
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda c: setattr(c, "smoother", _abi.SMOOTHER_JACOBI), "must agree"),
+    (lambda c: c.n_iter.__setitem__(3, 7), "must agree"),
+    (lambda c: setattr(c, "p_refine", 2), "must agree"),
+    (lambda c: setattr(c, "sweep_block", 2), "must agree"),
+    (lambda c: setattr(c, "tile", 3), "must agree"),
+    (lambda c: setattr(c, "nx", 2), "case 1: nx and ny"),
+])
+def test_batch_rejects_mismatched_cases_before_any_device_work(mutate, msg):
+    cs = (_abi.Config * 2)(_abi.cavity_config(n=32), _abi.cavity_config(n=48))
+    mutate(cs[1])
+    h = C.c_void_p()
+    st = _lib.lib().ifl_batch_create(cs, 2, C.byref(h))
+    assert st == _abi.IFL_ERR_ARG
+    assert not h.value
+    assert msg in _lib.lib().ifl_last_error().decode()
+
+
+def test_batch_workload_generator():
+    """The seeded uneven batch: 64 cases, sizes in 16*{4..32}, Re in [10, 1000], both types."""
+    cs = _abi.uneven_batch_configs()
+    assert len(cs) == 64
+    sizes = [(c.nx, c.ny) for c in cs]
+    assert all(n % 16 == 0 and 64 <= n <= 512 for s in sizes for n in s)
+    res = [1.0 / c.viscosity for c in cs]
+    assert all(10.0 <= r <= 1000.0 for r in res)
+    kinds = [c.bc_type[_abi.WEST] for c in cs]
+    assert 16 < kinds.count(_abi.INLET) < 48
+    assert sum(c.nx * c.ny for c in cs) == 5340160      # fixed by the seed
+    # SplitMix64 known answer (seed 0): first output 0xe220a8397b1dcdaf
+    g = _abi.splitmix64(0)
+    assert next(g) == (0xE220A8397B1DCDAF >> 11) / float(1 << 53)

@@ -1,0 +1,142 @@
+"""Batched solves: every case of a batch must equal its own single-case solve bit for bit
+(fields, iteration count, convergence flag), whatever the other cases in the batch are."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import STATE_FIELDS, assert_bitwise, cavity, cavity3d, channel, duct3d
+from incompflow import Batch, Solver, _abi
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _fields(s, cfg):
+    return {f: s.get(f, shaped=False) for f in STATE_FIELDS if cfg.field_size(f) > 0}
+
+
+def _check_batch(cfgs, max_iter, label):
+    _log(f"[batch] {label}: {len(cfgs)} cases, max_iter={max_iter}")
+    with Batch(cfgs) as b:
+        st, reps = b.solve(max_iter)
+        got = [_fields(b.cases[i], cfgs[i]) for i in range(len(cfgs))]
+    for i, c in enumerate(cfgs):
+        with Solver(c) as s:
+            r = s.solve(max_iter, history=False)
+            want = _fields(s, c)
+        assert reps[i]["iterations"] == r["iterations"], (label, i)
+        assert reps[i]["converged"] == r["converged"], (label, i)
+        assert reps[i]["final_scaled"] == r["final_scaled"], (label, i)
+        for f in want:
+            assert_bitwise(got[i][f], want[f], f"{label} case {i} field {f}")
+    return st, reps
+
+
+def _mixed_2d(tile=3):
+    cs = [channel(24, 12, 30), cavity(20, 16), cavity(70, 37, re=400.0), cavity(33, 18, re=20.0),
+          channel(64, 16, 30), cavity(64, 64, re=1000.0)]
+    cs[4].relax[0] = cs[4].relax[1] = 0.6
+    cs[5].relax[3] = 0.2
+    for c in cs:
+        c.tile = tile
+        c.num_outer_iter = 30
+    return cs
+
+
+@pytest.mark.parametrize("tile", [1, 3, 4])
+def test_batch_fixed_iterations_equal_single_solves(tile):
+    _check_batch(_mixed_2d(tile), 30, f"fixed tile={tile}")
+
+
+def test_batch_stop_rules_caps_and_repacking():
+    """Cases stop at different iterations (scaled residual, own iteration caps); the batch
+    repacks its graph as they stop (check every 8 iterations)."""
+    cs = []
+    for k, (n, m, re, tol, cap) in enumerate([(24, 24, 10.0, 1e-2, 400), (40, 20, 100.0, 1e-2, 400),
+                                               (32, 32, 400.0, 3e-3, 400), (20, 20, 50.0, 1e-3, 9),
+                                               (48, 24, 50.0, 1e-9, 17), (36, 36, 100.0, 1e-9, 60)]):
+        c = cavity(n, m, re=re)
+        c.stop_rule = _abi.STOP_SCALED
+        c.tol_residual = tol
+        c.num_outer_iter = cap
+        c.check_every = 8
+        c.tile = 2
+        cs.append(c)
+    faithful = channel(30, 10, 400)
+    faithful.stop_rule = _abi.STOP_FAITHFUL
+    faithful.check_every = 8
+    faithful.tile = 2
+    cs.append(faithful)
+    st, reps = _check_batch(cs, 0, "stop rules")
+    assert st == 0
+    its = [r["iterations"] for r in reps]
+    assert len(set(its)) >= 4, its          # the cases really stop at different iterations
+    assert its[3] == 9 and its[4] == 17 and its[5] == 60
+
+
+def test_batch_case_matches_oracle():
+    """Tie the batched path to the oracle directly on one case of a mixed batch."""
+    cs = _mixed_2d(1)[:3]
+    for c in cs:
+        c.num_outer_iter = 12
+    with Batch(cs) as b:
+        b.solve(12)
+        got = _fields(b.cases[2], cs[2])
+    o = Oracle(cs[2])
+    o.solve(12)
+    for f in got:
+        assert_bitwise(got[f], o.get(f).ravel(), f"oracle field {f}")
+
+
+def test_batch_3d():
+    cs = [cavity3d(10, 8, 6), duct3d(12, 6, 5), cavity3d(20, 17, 33, re=300.0)]
+    for c in cs:
+        c.num_outer_iter = 8
+    _check_batch(cs, 8, "3d")
+
+
+def test_batch_jacobi():
+    cs = _mixed_2d(1)[:4]
+    for c in cs:
+        c.smoother = _abi.SMOOTHER_JACOBI
+        c.num_outer_iter = 10
+    _check_batch(cs, 10, "jacobi")
+
+
+@pytest.mark.parametrize("sweep_block,p_refine", [(2, 0), (1, 1), (2, 1)])
+def test_batch_sweep_block_and_refine(sweep_block, p_refine):
+    cs = _mixed_2d(2)
+    for c in cs:
+        c.sweep_block = sweep_block
+        c.p_refine = p_refine
+        c.n_iter[1] = 7           # u and v with different sweep counts
+        c.num_outer_iter = 15
+    _check_batch(cs, 15, f"K={sweep_block} refine={p_refine}")
+
+
+def test_batch_inlet_profile_and_reuse():
+    """Face profiles set on a case handle reach the batch; a second solve continues."""
+    cs = [channel(24, 12, 10), channel(40, 16, 10)]
+    for c in cs:
+        c.tile = 4
+    prof = [0.001 * (1.0 + 0.1 * np.sin(np.arange(c.ny))) for c in cs]
+    with Batch(cs) as b:
+        for i in range(2):
+            b.cases[i].set_face_profile(_abi.WEST, prof[i])
+        b.solve(6)
+        b.solve(4)
+        got = [_fields(b.cases[i], cs[i]) for i in range(2)]
+    for i, c in enumerate(cs):
+        with Solver(c) as s:
+            s.set_face_profile(_abi.WEST, prof[i])
+            s.solve(6, history=False)
+            s.solve(4, history=False)
+            want = _fields(s, c)
+        for f in want:
+            assert_bitwise(got[i][f], want[f], f"profile case {i} field {f}")
