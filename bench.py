@@ -8,6 +8,12 @@ value      : smoother cell-updates per second over the timed steps (cells x swee
              the whole outer iteration's time in the denominator)
 roofline   : the red-black sweep kernel, timed with HIP events on the library's stream
 cpu_baseline: the FP32 CPU oracle (OpenMP, host cores) on a bounded number of outer iterations
+batch      : the seeded 64-case uneven batch (2D cavities and channels, 64..512 cells a side,
+             Re 10..1000, SplitMix64 seed 20261203) solved to the same scaled residual in one batch,
+             against the same cases solved one after another (unless --no-batch).  Each case of
+             the batch equals its single solve bit for bit, so both runs execute the same iterations;
+             the sequential time is measured in full with --batch-sequential full, else it is each
+             case's measured single-solve time per iteration times its iteration count.
 time_to_residual_s: a separate solve from rest to the scaled residual --target (unless --no-converge).
              The default target is 1e-4: in FP32 the 1024x1024 cavity's scaled residual stalls at
              ~4.5e-5 (v-momentum; continuity at ~1.3e-5), so 1e-5 cannot be reached at this size.
@@ -47,6 +53,57 @@ def sweep_bytes_per_cell(dim):
     return 4 * ((1 + 2 * dim) + 1 + 2)
 
 
+def batch_leg(args, incompflow, _abi):
+    cfgs = _abi.uneven_batch_configs(tol_residual=args.batch_target, max_iter=args.batch_max_iter)
+    for c in cfgs:
+        c.sweep_block = args.sweep_block
+    cells = sum(c.nx * c.ny for c in cfgs)
+    log(f"[bench] batch: {len(cfgs)} cases, {cells} cells, to scaled residual {args.batch_target:g}")
+    b = incompflow.Batch(cfgs)
+    device_sync()
+    t0 = time.perf_counter()
+    st, reps = b.solve()
+    device_sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    its = [r["iterations"] for r in reps]
+    conv = sum(1 for r in reps if r["converged"])
+    log(f"[bench] batch {tb:.2f} s, {conv}/{len(cfgs)} converged, max {max(its)} iterations")
+    seq = 0.0
+    if args.batch_sequential == "full":
+        for i, c in enumerate(cfgs):
+            s = incompflow.Solver(c)
+            t0 = time.perf_counter()
+            r = s.solve(0, history=False)
+            device_sync()
+            seq += time.perf_counter() - t0
+            s.close()
+            if r["iterations"] != its[i] or r["converged"] != reps[i]["converged"]:
+                raise RuntimeError(f"case {i}: single solve ran {r['iterations']} iterations, batch {its[i]}")
+            if i % 8 == 7:
+                log(f"[bench]   sequential: {i + 1} cases, {seq:.1f} s")
+        kind = "measured: every case solved alone to the same residual"
+    else:
+        for i, c in enumerate(cfgs):
+            c2 = c.copy()
+            c2.stop_rule = _abi.STOP_NONE
+            s = incompflow.Solver(c2)
+            s.solve(20, history=False)
+            device_sync()
+            t0 = time.perf_counter()
+            s.solve(args.batch_probe_iters, history=False)
+            device_sync()
+            seq += (time.perf_counter() - t0) / args.batch_probe_iters * its[i]
+            s.close()
+        kind = (f"estimated: each case's single-solve wall time per iteration "
+                f"({args.batch_probe_iters} iterations) x its iteration count")
+    log(f"[bench] sequential {seq:.2f} s ({kind.split(':')[0]}), speedup {seq / tb:.2f}x")
+    return {"cases": len(cfgs), "cells": cells, "residual_target": args.batch_target,
+            "max_iter": args.batch_max_iter, "converged": conv, "iterations_max": max(its),
+            "iterations_sum": sum(its), "batch_s": tb, "sequential_s": seq, "sequential_kind": kind,
+            "speedup": seq / tb, "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,10 +116,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=600)
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--sweep-block", type=int, default=1)
+    ap.add_argument("--sweep-block", type=int, default=2)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-target", type=float, default=1e-4)
+    ap.add_argument("--batch-max-iter", type=int, default=20000)
+    ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="estimate")
+    ap.add_argument("--batch-probe-iters", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -169,6 +231,10 @@ def main():
         o.close()
         log(f"[bench] cpu oracle {1000 * tc / it_cpu:.1f} ms/iteration on {threads} threads")
 
+    batch = None
+    if not args.no_batch and cfg.dim == 2:
+        batch = batch_leg(args, incompflow, _abi)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -202,6 +268,7 @@ def main():
         "converged": converged,
         "gpu_vs_cpu_per_step": None,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     if cpu:
         out["gpu_vs_cpu_per_step"] = cpu["ms_per_step"] / ms_per_step

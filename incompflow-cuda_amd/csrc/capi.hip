@@ -50,6 +50,7 @@ struct ifl_solver {
     ifl::CaseParams P{};
     hipStream_t stream = nullptr;
     char *arena = nullptr;
+    size_t arenaBytes = 0;
     float *temp = nullptr;
     float *prof[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
@@ -436,6 +437,7 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     for (auto &p : parts) total += align256(p.bytes);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&h->arena, total);
+    h->arenaBytes = total;
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
     if (e != hipSuccess) {
@@ -748,6 +750,150 @@ int ifl_write_vtk(ifl_solver *h, const char *path) {
 }
 
 }  // extern "C"
+
+// ---------------- autotuner ----------------
+namespace {
+
+// A copy of the whole device arena (fields, matrices, monitor, history) and the iteration count.
+struct Snapshot {
+    char *buf = nullptr;
+    size_t bytes = 0;
+    long long iter = 0;
+    ~Snapshot() {
+        if (buf) (void)hipFree(buf);
+    }
+};
+
+int snapTake(ifl_solver *h, Snapshot &s) {
+    s.bytes = h->arenaBytes;
+    s.iter = h->iter;
+    IFL_HIP(hipMalloc((void **)&s.buf, s.bytes));
+    IFL_HIP(hipMemcpyAsync(s.buf, h->arena, s.bytes, hipMemcpyDeviceToDevice, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    return IFL_OK;
+}
+
+int snapRestore(ifl_solver *h, const Snapshot &s) {
+    IFL_HIP(hipMemcpyAsync(h->arena, s.buf, s.bytes, hipMemcpyDeviceToDevice, h->stream));
+    IFL_HIP(hipStreamSynchronize(h->stream));
+    h->iter = s.iter;
+    return IFL_OK;
+}
+
+struct Settings {
+    int tile, K, nIter[4], stopRule;
+};
+
+void applySettings(ifl_solver *h, const Settings &x) {
+    h->cfg.tile = x.tile;
+    if (h->P.dim == 2) h->tile = x.tile;
+    h->cfg.sweep_block = x.K;
+    for (int q = 0; q < 4; ++q) h->cfg.n_iter[q] = x.nIter[q];
+    h->cfg.stop_rule = x.stopRule;
+    h->P.stopRule = x.stopRule;
+    h->graphDirty = true;
+}
+
+// n outer iterations from the current state: device time, final scaled residual, divergence.
+int trialRun(ifl_solver *h, int n, double &ms, double &scaled, bool &diverged) {
+    int st = buildGraph(h);
+    if (st != IFL_OK) return st;
+    if ((st = armCase(h, h->stream, 0)) != IFL_OK) return st;
+    if (h->iter == 0) ifl::launch_init_faces(planOf(h).L);
+    IFL_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int q = 0; q < n; ++q) IFL_HIP(hipGraphLaunch(h->graph, h->stream));
+    IFL_HIP(hipEventRecord(h->ev1, h->stream));
+    IFL_HIP(hipEventSynchronize(h->ev1));
+    float f = 0;
+    IFL_HIP(hipEventElapsedTime(&f, h->ev0, h->ev1));
+    ms = f;
+    ifl::CaseState s{};
+    if ((st = readState(h, s)) != IFL_OK) return st;
+    h->iter = s.iter;
+    scaled = s.lastScaled;
+    diverged = s.done == 2 || !std::isfinite(scaled);
+    return IFL_OK;
+}
+
+}  // namespace
+
+extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_result *out) {
+    if (!h || !out || (what & ~(IFL_TUNE_LAUNCH | IFL_TUNE_SWEEPS))) return fail(IFL_ERR_ARG, "bad argument");
+    std::memset(out, 0, sizeof(*out));
+    const int trial = trial_iters > 0 ? trial_iters : 300;
+    const ifl_config c0 = h->cfg;
+    Settings best{h->P.dim == 2 ? h->tile : 0, std::max(1, c0.sweep_block), {}, IFL_STOP_NONE};
+    for (int q = 0; q < 4; ++q) best.nIter[q] = c0.n_iter[q];
+    int st;
+    double ms = 0, sc = 0;
+    bool div = false;
+    applySettings(h, best);
+    if (h->iter < 20 && (st = trialRun(h, (int)(20 - h->iter), ms, sc, div)) != IFL_OK) return st;
+    Snapshot snap;
+    if ((st = snapTake(h, snap)) != IFL_OK) return st;
+    ifl::CaseState s0{};
+    if ((st = readState(h, s0)) != IFL_OK) return st;
+    double bestUs = 0.0, bestRate = 0.0;
+    auto timeIter = [&](const Settings &x, double &us) -> int {
+        applySettings(h, x);
+        int r = trialRun(h, 3, ms, sc, div);                      // graph build and warm-up
+        if (r == IFL_OK) r = trialRun(h, 30, ms, sc, div);
+        if (r == IFL_OK) r = snapRestore(h, snap);
+        us = 1000.0 * ms / 30.0;
+        out->trials++;
+        return r;
+    };
+    if ((what & IFL_TUNE_LAUNCH) && h->P.dim == 2 && c0.smoother == IFL_SMOOTHER_RBGS) {
+        bestUs = 1e300;
+        Settings x = best;
+        for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_32x4; ++tile)
+            for (int K = 1; K <= 2; ++K) {
+                x.tile = tile;
+                x.K = K;
+                double us;
+                if ((st = timeIter(x, us)) != IFL_OK) return st;
+                if (us < bestUs) {
+                    bestUs = us;
+                    best.tile = tile;
+                    best.K = K;
+                }
+            }
+    }
+    if (what & IFL_TUNE_SWEEPS) {
+        const int uvOpts[3] = {5, 10, 20}, pOpts[3] = {10, 20, 40};
+        bestRate = -1e300;
+        Settings chosen = best;
+        for (int a : uvOpts)
+            for (int b : pOpts) {
+                Settings x = best;
+                x.nIter[0] = x.nIter[1] = x.nIter[2] = a;
+                if (h->P.dim == 2) x.nIter[2] = c0.n_iter[2];
+                x.nIter[3] = b;
+                applySettings(h, x);
+                if ((st = trialRun(h, trial, ms, sc, div)) != IFL_OK) return st;
+                if ((st = snapRestore(h, snap)) != IFL_OK) return st;
+                out->trials++;
+                if (div || !(sc > 0.0) || !(s0.lastScaled > 0.0)) continue;
+                const double rate = std::log10(s0.lastScaled / sc) / (ms / 1000.0);
+                if (rate > bestRate) {
+                    bestRate = rate;
+                    chosen = x;
+                }
+            }
+        best = chosen;
+        if (bestRate == -1e300) bestRate = 0.0;
+    }
+    if (!(what & IFL_TUNE_LAUNCH) || bestUs == 0.0 || (what & IFL_TUNE_SWEEPS))
+        if ((st = timeIter(best, bestUs)) != IFL_OK) return st;
+    best.stopRule = c0.stop_rule;
+    applySettings(h, best);
+    out->tile = best.tile;
+    out->sweep_block = best.K;
+    for (int q = 0; q < 4; ++q) out->n_iter[q] = best.nIter[q];
+    out->us_per_iter = bestUs;
+    out->decades_per_s = bestRate;
+    return IFL_OK;
+}
 
 // ---------------- batched solves ----------------
 // The batch owns one solver handle per case (arena, monitor, history ring) and runs them all

@@ -112,6 +112,15 @@ class Report(C.Structure):
                 ("sweeps_total", C.c_double)]
 
 
+TUNE_LAUNCH, TUNE_SWEEPS = 1, 2
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("sweep_block", C.c_int32), ("n_iter", C.c_int32 * 4),
+                ("us_per_iter", C.c_double), ("decades_per_s", C.c_double), ("trials", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 def default_config():
     """The reference's shipped include/config.h (1024x256 channel on [-2,2]x[-0.5,0.5])."""
     c = Config()

@@ -29,6 +29,7 @@ EXPORTS = [
     ("ifl_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
     ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
     ("ifl_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    ("ifl_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_abi.TuneResult)]),
     ("ifl_batch_create", C.c_int, [C.POINTER(_abi.Config), C.c_int, C.POINTER(C.c_void_p)]),
     ("ifl_batch_destroy", C.c_int, [C.c_void_p]),
     ("ifl_batch_size", C.c_int, [C.c_void_p]),
@@ -142,6 +143,19 @@ class Solver:
 
     def write_vtk(self, path):
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))
+
+    def autotune(self, what=_abi.TUNE_LAUNCH | _abi.TUNE_SWEEPS, trial_iters=0):
+        """Pick tile, sweep_block and sweeps per outer iteration by measurement (ifl_autotune);
+        the choice is applied to this solver and mirrored into self.cfg."""
+        r = _abi.TuneResult()
+        _check(lib().ifl_autotune(self.h, what, trial_iters, C.byref(r)))
+        if self.cfg.dim == 2:
+            self.cfg.tile = r.tile
+        self.cfg.sweep_block = r.sweep_block
+        for q in range(4):
+            self.cfg.n_iter[q] = r.n_iter[q]
+        return {"tile": r.tile, "sweep_block": r.sweep_block, "n_iter": list(r.n_iter),
+                "us_per_iter": r.us_per_iter, "decades_per_s": r.decades_per_s, "trials": r.trials}
 
 
 class Batch:

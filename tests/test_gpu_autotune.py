@@ -1,0 +1,60 @@
+"""Autotuner: choices come from the candidate sets, the solver state after tuning is the state
+before the trials (bit for bit), and the chosen settings are the ones the solve then uses."""
+import numpy as np
+import pytest
+
+from helpers import STATE_FIELDS, assert_bitwise, cavity
+from incompflow import Solver, _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(s, cfg):
+    return {f: s.get(f, shaped=False) for f in STATE_FIELDS if cfg.field_size(f) > 0}
+
+
+def _case():
+    c = cavity(48, 40, re=200.0)
+    c.stop_rule = _abi.STOP_SCALED
+    c.tol_residual = 1e-12
+    return c
+
+
+def test_autotune_choices_and_restored_state():
+    c = _case()
+    with Solver(c) as s:
+        r = s.autotune(trial_iters=40)
+        after = _fields(s, c)
+        assert r["tile"] in (1, 2, 3, 4)
+        assert r["sweep_block"] in (1, 2)
+        assert r["n_iter"][0] in (5, 10, 20) and r["n_iter"][1] == r["n_iter"][0]
+        assert r["n_iter"][3] in (10, 20, 40)
+        assert r["trials"] == 8 + 9 + 1
+        assert r["us_per_iter"] > 0 and r["decades_per_s"] > 0
+        rep = s.solve(30)
+        assert rep["iterations"] == 30
+        sweeps = np.array([[h.sweeps[q] for q in range(4)] for h in rep["history"]])
+        assert sweeps[:, 3].max() <= r["n_iter"][3]
+        assert sweeps[:, 0].max() <= r["n_iter"][0]
+    # a fresh case had 20 warm-up iterations with its own settings and the stop test off
+    w = c.copy()
+    w.stop_rule = _abi.STOP_NONE
+    with Solver(w) as s2:
+        s2.solve(20, history=False)
+        want = _fields(s2, w)
+    for f in want:
+        assert_bitwise(after[f], want[f], f"field {f}")
+
+
+def test_autotune_launch_only_keeps_state_of_running_case():
+    c = _case()
+    with Solver(c) as s:
+        s.solve(25, history=False)
+        before = _fields(s, c)
+        r = s.autotune(_abi.TUNE_LAUNCH)
+        assert r["trials"] == 8
+        assert r["n_iter"] == [10, 10, 10, 20]
+        for f, v in _fields(s, c).items():
+            assert_bitwise(v, before[f], f"field {f}")
+        rep = s.solve(5)
+        assert rep["iterations"] == 5
