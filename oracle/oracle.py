@@ -45,6 +45,18 @@ def lib():
         L.orc_config_size.restype = C.c_size_t
         L.orc_record_size.restype = C.c_size_t
         L.orc_set_threads.argtypes = [C.c_int]
+        dp = C.POINTER(C.c_double)
+        L.orc_adj_setup.argtypes = [vp, C.c_int]
+        L.orc_adj_sizes.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.orc_adj_set_weights.argtypes = [vp, dp]
+        L.orc_adj_objective.argtypes = [vp]
+        L.orc_adj_objective.restype = C.c_double
+        L.orc_adj_tangent.argtypes = [vp, dp, dp, dp]
+        L.orc_adj_transpose.argtypes = [vp, dp, dp, dp]
+        L.orc_adj_iterate.argtypes = [vp, C.c_int, dp]
+        L.orc_adj_get_lambda.argtypes = [vp, dp]
+        L.orc_adj_set_lambda.argtypes = [vp, dp]
+        L.orc_adj_gradient.argtypes = [vp, dp]
         assert L.orc_config_size() == C.sizeof(_abi.Config), "ifl_config layout mismatch"
         assert L.orc_record_size() == C.sizeof(_abi.IterRecord), "ifl_iter_record layout mismatch"
         _lib = L
@@ -123,3 +135,57 @@ class Oracle:
 
     def set_iter(self, it):
         lib().orc_set_iter(self.h, int(it))
+
+    # ---- discrete adjoint (oracle/adjoint_oracle.hpp) ----
+    # Flat layouts: state = [u, v, w, p (N each), uf, vf, wf]; parameters = [face profiles
+    # E, W, N, S, T, B (cells on each face), mu].
+    def adj_setup(self, objective=0):
+        self._check(lib().orc_adj_setup(self.h, objective))
+        nv, npar = C.c_size_t(), C.c_size_t()
+        self._check(lib().orc_adj_sizes(self.h, C.byref(nv), C.byref(npar)))
+        self.adj_nvec, self.adj_npar = nv.value, npar.value
+
+    @staticmethod
+    def _dp(a):
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def adj_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.adj_nvec
+        self._check(lib().orc_adj_set_weights(self.h, self._dp(w)))
+
+    def adj_objective(self):
+        return lib().orc_adj_objective(self.h)
+
+    def adj_tangent(self, dx, dth):
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        dth = np.ascontiguousarray(dth, dtype=np.float64)
+        out = np.zeros(self.adj_nvec)
+        self._check(lib().orc_adj_tangent(self.h, self._dp(dx), self._dp(dth), self._dp(out)))
+        return out
+
+    def adj_transpose(self, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros(self.adj_nvec)
+        tb = np.zeros(self.adj_npar)
+        self._check(lib().orc_adj_transpose(self.h, self._dp(lam), self._dp(out), self._dp(tb)))
+        return out, tb
+
+    def adj_iterate(self, n):
+        sc = C.c_double()
+        self._check(lib().orc_adj_iterate(self.h, int(n), C.byref(sc)))
+        return sc.value
+
+    def adj_lambda(self):
+        out = np.zeros(self.adj_nvec)
+        self._check(lib().orc_adj_get_lambda(self.h, self._dp(out)))
+        return out
+
+    def adj_set_lambda(self, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._check(lib().orc_adj_set_lambda(self.h, self._dp(lam)))
+
+    def adj_gradient(self):
+        out = np.zeros(self.adj_npar)
+        self._check(lib().orc_adj_gradient(self.h, self._dp(out)))
+        return out

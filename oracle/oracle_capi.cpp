@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 
+#include "adjoint_oracle.hpp"
 #include "simple_oracle.hpp"
 
 namespace {
@@ -14,7 +15,55 @@ struct Handle {
     int fp64;
     oracle::Simple<float> *f;
     oracle::Simple<double> *d;
+    oracle::Adjoint<float> *af = nullptr;
+    oracle::Adjoint<double> *ad = nullptr;
 };
+
+// Flat layouts of adjoint vectors for ctypes: state = [u, v, w, p (N each), uf, vf, wf],
+// parameters = [profiles of faces E, W, N, S, T, B (cells on each face), mu].
+template <class Real>
+size_t vecSize(const oracle::Adjoint<Real> &a) {
+    return 4 * (size_t)a.Nc + a.s.uf.size() + a.s.vf.size() + a.s.wf.size();
+}
+template <class Real>
+size_t parSize(const oracle::Adjoint<Real> &a) {
+    size_t n = 1;
+    for (int f = 0; f < 6; ++f) n += a.faceCells(f);
+    return n;
+}
+template <class Real>
+void unpackVec(const oracle::Adjoint<Real> &a, const double *src, oracle::AdjVec<Real> &x) {
+    a.alloc(x);
+    size_t o = 0;
+    for (int d = 0; d < 3; ++d)
+        for (auto &e : x.u[d]) e = (Real)src[o++];
+    for (auto &e : x.p) e = (Real)src[o++];
+    for (int d = 0; d < 3; ++d)
+        for (auto &e : x.f[d]) e = (Real)src[o++];
+}
+template <class Real>
+void packVec(const oracle::AdjVec<Real> &x, double *dst) {
+    size_t o = 0;
+    for (int d = 0; d < 3; ++d)
+        for (auto e : x.u[d]) dst[o++] = (double)e;
+    for (auto e : x.p) dst[o++] = (double)e;
+    for (int d = 0; d < 3; ++d)
+        for (auto e : x.f[d]) dst[o++] = (double)e;
+}
+template <class Real>
+void unpackPar(const oracle::Adjoint<Real> &a, const double *src, oracle::AdjParams &t) {
+    a.allocParams(t);
+    size_t o = 0;
+    for (int f = 0; f < 6; ++f)
+        for (auto &e : t.prof[f]) e = src[o++];
+    t.mu = src[o];
+}
+inline void packPar(const oracle::AdjParams &t, double *dst) {
+    size_t o = 0;
+    for (int f = 0; f < 6; ++f)
+        for (auto e : t.prof[f]) dst[o++] = e;
+    dst[o] = t.mu;
+}
 
 thread_local std::string g_err;
 
@@ -110,6 +159,8 @@ int orc_create(const ifl_config *cfg, int fp64, void **out) {
 int orc_destroy(void *hv) {
     Handle *h = (Handle *)hv;
     if (!h) return IFL_OK;
+    delete h->af;
+    delete h->ad;
     delete h->f;
     delete h->d;
     delete h;
@@ -165,6 +216,110 @@ int orc_solve(void *hv, int max_iter, ifl_iter_record *hist, size_t cap, int *it
 int orc_set_iter(void *hv, long long it) {
     Handle *h = (Handle *)hv;
     if (h->fp64) h->d->iter = it; else h->f->iter = it;
+    return IFL_OK;
+}
+
+// ---- discrete adjoint (adjoint_oracle.hpp) ----
+// Set up the adjoint at the current state (which should be a converged fixed point).
+// objective 0: pressure drop between the first and last cell layers along x; 1: linear
+// weights given by orc_adj_set_weights.
+int orc_adj_setup(void *hv, int objective) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s, auto *&a) {
+        using A = std::remove_pointer_t<std::remove_reference_t<decltype(a)>>;
+        delete a;
+        a = new A(*s);
+        a->setup();
+        if (objective == 0) a->setPressureDropObjective();
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d, h->ad) : go(h->f, h->af);
+}
+
+int orc_adj_sizes(void *hv, size_t *nvec, size_t *npar) {
+    Handle *h = (Handle *)hv;
+    if (h->fp64 ? !h->ad : !h->af) { g_err = "adjoint not set up"; return IFL_ERR_ARG; }
+    if (h->fp64) { *nvec = vecSize(*h->ad); *npar = parSize(*h->ad); }
+    else { *nvec = vecSize(*h->af); *npar = parSize(*h->af); }
+    return IFL_OK;
+}
+
+// Objective weights as a state vector (u, v, w, p parts are used).
+int orc_adj_set_weights(void *hv, const double *w) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *a) {
+        using R = typename std::remove_reference_t<decltype(a->gp)>::value_type;
+        oracle::AdjVec<R> x;
+        unpackVec(*a, w, x);
+        for (int d = 0; d < 3; ++d) a->gu[d] = x.u[d];
+        a->gp = x.p;
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->ad) : go(h->af);
+}
+
+double orc_adj_objective(void *hv) {
+    Handle *h = (Handle *)hv;
+    return h->fp64 ? h->ad->objective() : h->af->objective();
+}
+
+// out = G_x dx + G_theta dth (tangent of one outer iteration at the fixed point)
+int orc_adj_tangent(void *hv, const double *dx, const double *dth, double *out) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *a) {
+        using R = typename std::remove_reference_t<decltype(a->gp)>::value_type;
+        oracle::AdjVec<R> x, o;
+        oracle::AdjParams t;
+        unpackVec(*a, dx, x);
+        unpackPar(*a, dth, t);
+        a->tangent(x, t, o);
+        packVec(o, out);
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->ad) : go(h->af);
+}
+
+// (out, tbar) = (G_x^T l, G_theta^T l), plus the objective gradient added to out
+int orc_adj_transpose(void *hv, const double *l, double *out, double *tbar) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *a) {
+        using R = typename std::remove_reference_t<decltype(a->gp)>::value_type;
+        oracle::AdjVec<R> x, o;
+        oracle::AdjParams t;
+        unpackVec(*a, l, x);
+        a->transpose(x, o, t);
+        packVec(o, out);
+        packPar(t, tbar);
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->ad) : go(h->af);
+}
+
+// n adjoint iterations lambda <- G_x^T lambda + J_x^T; *scaled = the last scaled increment
+int orc_adj_iterate(void *hv, int n, double *scaled) {
+    Handle *h = (Handle *)hv;
+    double sc = 0;
+    for (int q = 0; q < n; ++q) sc = h->fp64 ? h->ad->iterate() : h->af->iterate();
+    if (scaled) *scaled = sc;
+    return IFL_OK;
+}
+
+int orc_adj_get_lambda(void *hv, double *dst) {
+    Handle *h = (Handle *)hv;
+    if (h->fp64) packVec(h->ad->lam, dst); else packVec(h->af->lam, dst);
+    return IFL_OK;
+}
+
+int orc_adj_set_lambda(void *hv, const double *src) {
+    Handle *h = (Handle *)hv;
+    if (h->fp64) unpackVec(*h->ad, src, h->ad->lam); else unpackVec(*h->af, src, h->af->lam);
+    return IFL_OK;
+}
+
+// G_theta^T lambda of the last iteration: the gradient once lambda has converged
+int orc_adj_gradient(void *hv, double *tbar) {
+    Handle *h = (Handle *)hv;
+    packPar(h->fp64 ? h->ad->grad : h->af->grad, tbar);
     return IFL_OK;
 }
 
