@@ -233,6 +233,46 @@ typedef struct ifl_tune_result {
 
 int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_result *out);
 
+/* ---- Discrete adjoint (no reference counterpart) ----
+ * Gradient of an objective J of the converged state with respect to the normal-velocity
+ * profile of an inlet face and to the viscosity mu, by the fixed-point adjoint of the SIMPLE
+ * outer iteration: lambda <- G_x^T lambda + J_x^T (transposed stencils of the momentum, Rhie-
+ * Chow, p' and correction stages, the upwind switch and the 8/3 boundary rows, solved with
+ * transposed red-black sweeps), then dJ/dtheta = lambda^T G_theta.  Call it after a solve has
+ * converged; only the converged state is used.  The adjoint iterate converges at the forward
+ * iteration's own rate. */
+enum { IFL_OBJ_PRESSURE_DROP = 0,  /* mean p of the first cell layer along x minus the last */
+       IFL_OBJ_LINEAR = 1 };       /* sum of weights_u*u + weights_v*v + weights_w*w + weights_p*p */
+
+typedef struct ifl_adjoint_opts {
+    int32_t objective;             /* IFL_OBJ_* */
+    int32_t max_iter;              /* adjoint iterations at most (0: 200000) */
+    double tol;                    /* stop when the scaled adjoint increment is below tol (0: 1e-7) */
+    double grad_tol;               /* or when dJ/dmu changed by less than grad_tol (relative) over
+                                      the last check_every iterations (0: off).  In FP32 the
+                                      increment stalls near 1e-4 while the gradient has long
+                                      settled; this is the practical stop rule there. */
+    const float *weights[4];       /* IFL_OBJ_LINEAR: weights of u, v, w, p (cells each; NULL = 0) */
+    int32_t warm_start;            /* 1: continue from the previous call's lambda */
+    int32_t check_every;           /* iterations between host reads of the monitor (0: 64) */
+} ifl_adjoint_opts;
+
+typedef struct ifl_adjoint_report {
+    int32_t iterations;            /* adjoint iterations run by this call */
+    int32_t converged;
+    double J;                      /* objective at the current state */
+    double final_scaled;           /* last scaled increment */
+    double elapsed_ms;
+} ifl_adjoint_report;
+
+void ifl_adjoint_opts_default(ifl_adjoint_opts *o);
+/* dJ_dprofile (optional, n_profile = cells of `face`, which must be an inlet) and dJ_dmu
+ * (optional) receive the gradient. */
+int ifl_adjoint(ifl_solver *h, const ifl_adjoint_opts *o, int face, double *dJ_dprofile, size_t n_profile,
+                double *dJ_dmu, ifl_adjoint_report *report);
+/* Host copy of an adjoint component (field = IFL_FIELD_U, V, W, P, UF, VF, WF). */
+int ifl_adjoint_lambda(ifl_solver *h, int field, float *dst, size_t n);
+
 /* ---- Batched solves (no reference counterpart) ----
  * A batch runs many independent cases, each with its own grid size, domain, boundary
  * conditions, fluid properties, relaxation, tolerances and stop rule, through one outer

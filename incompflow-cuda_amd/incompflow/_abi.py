@@ -113,6 +113,17 @@ class Report(C.Structure):
 
 
 TUNE_LAUNCH, TUNE_SWEEPS = 1, 2
+OBJ_PRESSURE_DROP, OBJ_LINEAR = 0, 1
+
+
+class AdjointOpts(C.Structure):
+    _fields_ = [("objective", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double), ("grad_tol", C.c_double),
+                ("weights", C.POINTER(C.c_float) * 4), ("warm_start", C.c_int32), ("check_every", C.c_int32)]
+
+
+class AdjointReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("J", C.c_double),
+                ("final_scaled", C.c_double), ("elapsed_ms", C.c_double)]
 
 
 class TuneResult(C.Structure):

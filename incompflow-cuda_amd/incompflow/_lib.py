@@ -30,6 +30,10 @@ EXPORTS = [
     ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
     ("ifl_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("ifl_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_abi.TuneResult)]),
+    ("ifl_adjoint_opts_default", None, [C.POINTER(_abi.AdjointOpts)]),
+    ("ifl_adjoint", C.c_int, [C.c_void_p, C.POINTER(_abi.AdjointOpts), C.c_int, C.POINTER(C.c_double), C.c_size_t,
+                              C.POINTER(C.c_double), C.POINTER(_abi.AdjointReport)]),
+    ("ifl_adjoint_lambda", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
     ("ifl_batch_create", C.c_int, [C.POINTER(_abi.Config), C.c_int, C.POINTER(C.c_void_p)]),
     ("ifl_batch_destroy", C.c_int, [C.c_void_p]),
     ("ifl_batch_size", C.c_int, [C.c_void_p]),
@@ -143,6 +147,50 @@ class Solver:
 
     def write_vtk(self, path):
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))
+
+    def adjoint(self, objective=_abi.OBJ_PRESSURE_DROP, face=_abi.WEST, weights=None, max_iter=0, tol=0.0,
+                warm_start=False, check_every=0, profile_gradient=True, grad_tol=0.0):
+        """Gradient of the objective at the current (converged) state w.r.t. the normal-velocity
+        profile of `face` (an inlet) and mu (ifl_adjoint).  weights: dict field id -> array
+        (OBJ_LINEAR).  tol < 0 runs exactly max_iter adjoint iterations."""
+        o = _abi.AdjointOpts()
+        lib().ifl_adjoint_opts_default(C.byref(o))
+        o.objective = objective
+        if max_iter:
+            o.max_iter = max_iter
+        if tol:
+            o.tol = tol
+        o.grad_tol = grad_tol
+        o.warm_start = 1 if warm_start else 0
+        if check_every:
+            o.check_every = check_every
+        keep = []
+        if weights:
+            for q, fid in enumerate((_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W, _abi.FIELD_P)):
+                if fid in weights:
+                    a = np.ascontiguousarray(np.asarray(weights[fid], dtype=np.float32).ravel())
+                    keep.append(a)
+                    o.weights[q] = a.ctypes.data_as(C.POINTER(C.c_float))
+        c = self.cfg
+        nprof = 0
+        gp = None
+        if profile_gradient:
+            nprof = c.ny * c.nz if face < 2 else c.nx * c.nz if face < 4 else c.nx * c.ny
+            gp = np.zeros(nprof, dtype=np.float64)
+        gmu = C.c_double()
+        rep = _abi.AdjointReport()
+        _check(lib().ifl_adjoint(self.h, C.byref(o), face,
+                                 gp.ctypes.data_as(C.POINTER(C.c_double)) if gp is not None else None,
+                                 nprof, C.byref(gmu), C.byref(rep)))
+        return {"dJ_dprofile": gp, "dJ_dmu": gmu.value, "iterations": rep.iterations,
+                "converged": bool(rep.converged), "J": rep.J, "final_scaled": rep.final_scaled,
+                "elapsed_ms": rep.elapsed_ms}
+
+    def adjoint_lambda(self, field, shaped=True):
+        n = self.cfg.field_size(field)
+        out = np.empty(n, dtype=np.float32)
+        _check(lib().ifl_adjoint_lambda(self.h, field, out.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return out.reshape(self.cfg.shape(field)) if shaped else out
 
     def autotune(self, what=_abi.TUNE_LAUNCH | _abi.TUNE_SWEEPS, trial_iters=0):
         """Pick tile, sweep_block and sweeps per outer iteration by measurement (ifl_autotune);
