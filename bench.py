@@ -14,6 +14,10 @@ batch      : the seeded 64-case uneven batch (2D cavities and channels, 64..512 
              the batch equals its single solve bit for bit, so both runs execute the same iterations;
              the sequential time is measured in full with --batch-sequential full, else it is each
              case's measured single-solve time per iteration times its iteration count.
+adjoint    : the 1024x256 channel with a noisy parabolic inlet solved to the scaled residual
+             --adjoint-forward-target (1e-5: FP32 stalls near 9.4e-6 on this grid), then the discrete
+             adjoint of the pressure drop until dJ/dmu settles; wall times of both, and with
+             --adjoint-fd central differences of device solves along mu and along the profile.
 time_to_residual_s: a separate solve from rest to the scaled residual --target (unless --no-converge).
              The default target is 1e-4: in FP32 the 1024x1024 cavity's scaled residual stalls at
              ~4.5e-5 (v-momentum; continuity at ~1.3e-5), so 1e-5 cannot be reached at this size.
@@ -104,6 +108,63 @@ def batch_leg(args, incompflow, _abi):
             "speedup": seq / tb, "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
 
 
+def adjoint_leg(args, incompflow, _abi):
+    import numpy as np
+    c = _abi.adjoint_channel_config(1024, 256)
+    c.stop_rule = _abi.STOP_SCALED
+    c.tol_residual = args.adjoint_forward_target
+    c.num_outer_iter = 200000
+    c.sweep_block = args.sweep_block
+    prof = np.array(_abi.adjoint_inlet_profile(c))
+    log(f"[bench] adjoint channel 1024x256: forward to {c.tol_residual:g}")
+    s = incompflow.Solver(c)
+    s.set_face_profile(_abi.WEST, prof)
+    device_sync()
+    t0 = time.perf_counter()
+    rf = s.solve(0, history=False)
+    device_sync()
+    tf = time.perf_counter() - t0
+    log(f"[bench]   forward {rf['iterations']} iterations, {tf:.2f} s, converged {rf['converged']}")
+    t0 = time.perf_counter()
+    ra = s.adjoint(_abi.OBJ_PRESSURE_DROP, _abi.WEST, tol=1e-7, grad_tol=1e-6, check_every=500, max_iter=200000)
+    device_sync()
+    ta = time.perf_counter() - t0
+    s.close()
+    J, gmu, gp = ra["J"], ra["dJ_dmu"], ra["dJ_dprofile"]
+    log(f"[bench]   adjoint {ra['iterations']} iterations, {ta:.2f} s, J {J:.6e}, dJ/dmu {gmu:.6e}")
+    out = {"workload": "2D channel 1024x256 on [0,0.5]x[0,0.05], rho 1, mu 1e-3, parabolic inlet "
+                       "(u_max 0.02) + 1e-3 SplitMix64/Box-Muller noise, J = mean p(i=0) - mean p(i=nx-1)",
+           "forward_target": c.tol_residual, "forward_iterations": rf["iterations"],
+           "forward_converged": rf["converged"], "forward_s": tf,
+           "adjoint_iterations": ra["iterations"], "adjoint_converged": ra["converged"], "adjoint_s": ta,
+           "adjoint_over_forward": ta / tf, "adjoint_ms_per_iteration": ra["elapsed_ms"] / max(1, ra["iterations"]),
+           "J": J, "dJ_dmu": gmu, "dJ_dprofile_norm": float(np.linalg.norm(gp)),
+           # creeping-flow identities (J linear in mu and in the inflow at Re ~ 1)
+           "mu_dJdmu_over_J": c.viscosity * gmu / J, "theta_dJdtheta_over_J": float(np.dot(prof, gp)) / J,
+           "fd": None}
+    if args.adjoint_fd:
+        def J_of(mu, p):
+            c2 = c.copy()
+            c2.viscosity = mu
+            s2 = incompflow.Solver(c2)
+            s2.set_face_profile(_abi.WEST, p)
+            s2.solve(0, history=False)
+            pp = s2.get(_abi.FIELD_P).astype(np.float64)
+            s2.close()
+            return float(pp[:, :, 0].mean() - pp[:, :, c.nx - 1].mean())
+        h = 1e-2
+        fd_mu = (J_of(c.viscosity * (1 + h), prof) - J_of(c.viscosity * (1 - h), prof)) / (2 * h * c.viscosity)
+        fd_dir = (J_of(c.viscosity, prof * (1 + h)) - J_of(c.viscosity, prof * (1 - h))) / (2 * h)
+        adj_dir = float(np.dot(prof, gp))
+        out["fd"] = {"step": h, "dJ_dmu_fd": fd_mu, "dJ_dmu_rel_err": abs(gmu - fd_mu) / abs(fd_mu),
+                     "dir_profile_fd": fd_dir, "dir_profile_adjoint": adj_dir,
+                     "dir_profile_rel_err": abs(adj_dir - fd_dir) / abs(fd_dir),
+                     "kind": "central differences of FP32 device solves to the same target"}
+        log(f"[bench]   FD check: dJ/dmu rel err {out['fd']['dJ_dmu_rel_err']:.2e}, "
+            f"profile direction rel err {out['fd']['dir_profile_rel_err']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +186,9 @@ def main():
     ap.add_argument("--batch-max-iter", type=int, default=20000)
     ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="estimate")
     ap.add_argument("--batch-probe-iters", type=int, default=300)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
+    ap.add_argument("--adjoint-fd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -234,6 +298,9 @@ def main():
     batch = None
     if not args.no_batch and cfg.dim == 2:
         batch = batch_leg(args, incompflow, _abi)
+    adjoint = None
+    if not args.no_adjoint and cfg.dim == 2:
+        adjoint = adjoint_leg(args, incompflow, _abi)
 
     out = {
         "metric": METRIC,
@@ -269,6 +336,7 @@ def main():
         "gpu_vs_cpu_per_step": None,
         "cpu_baseline": cpu,
         "batch": batch,
+        "adjoint": adjoint,
     }
     if cpu:
         out["gpu_vs_cpu_per_step"] = cpu["ms_per_step"] / ms_per_step

@@ -267,3 +267,20 @@ def uneven_batch_configs(n=64, seed=20261203, tol_residual=1e-4, max_iter=100000
             c.bc_val[WEST][0] = 1.0
         out.append(c)
     return out
+
+
+def adjoint_inlet_profile(cfg, u_max=0.02, sigma=1e-3, seed=20261203):
+    """Inlet profile of the adjoint channel: u_j = 4 u_max y_j (H - y_j) / H^2 + sigma xi_j with
+    y_j = (j + 1/2) dy and xi_j standard normal from SplitMix64 uniforms through Box-Muller."""
+    import math
+    H = cfg.ymax - cfg.ymin
+    dy = H / cfg.ny
+    g = splitmix64(seed)
+    out = []
+    for j in range(cfg.ny):
+        y = (j + 0.5) * dy
+        u1 = next(g)
+        u2 = next(g)
+        xi = math.sqrt(-2.0 * math.log(max(u1, 1e-300))) * math.cos(2.0 * math.pi * u2)
+        out.append(4.0 * u_max * y * (H - y) / (H * H) + sigma * xi)
+    return out
