@@ -90,7 +90,8 @@ int validate(const ifl_config &c) {
     if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
     if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
     if (c.tile < 0 || c.tile > 4) return fail(IFL_ERR_ARG, "tile must be 0..4");
-    if (c.sweep_block < 0 || c.sweep_block > 2) return fail(IFL_ERR_ARG, "sweep_block must be 0, 1 or 2");
+    if (c.sweep_block < 0 || c.sweep_block > 4 || c.sweep_block == 3)
+        return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2 or 4");
     return IFL_OK;
 }
 
@@ -142,8 +143,11 @@ struct Plan {
     int dim;
 };
 
-int sweepBlockOf(const ifl_config &c, int dim) {
-    return (dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block == 2) ? 2 : 1;
+// Sweeps per launch of a smoother call: sweep_block for p' (2D red-black), at most 2 for the
+// momentum equations (their fused launch stages two fields and the face velocities), 1 otherwise.
+int sweepBlockOf(const ifl_config &c, int dim, int slot) {
+    if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS || c.sweep_block <= 1) return 1;
+    return slot < 3 ? std::min(c.sweep_block, 2) : c.sweep_block;
 }
 
 // Workgroup kind of a smoother launch and, for one case, its workgroup count.
@@ -160,82 +164,125 @@ int sweepRecordsSingle(const Plan &pl) {
     }
 }
 
+// How a smoother call is closed.  The outer-iteration path (fast) lets a call with an odd launch
+// count start in the partner buffer (which the kernels keep equal to the field: the update writes
+// the velocities to both, p' and its correction start from zero in both), so the result lands in
+// the caller's field without a copy, and hands the call's closing step to workgroup 0 of the next
+// kernel (pending).  The stage-by-stage path copies back and launches k_smooth_finish, and leaves
+// the momentum fields' partners equal to them.
+struct SmoothMode {
+    bool fast = false;
+    ifl::Finish *pending = nullptr;
+    bool accumLast = false;      // the last launch adds the correction into p' (2D red-black)
+};
+
 // Smoother calls of Solver.cu:178-196.  Every launch reads one buffer of a ping-pong pair and
-// writes the other; an odd launch count ends with a copy back.  Equations listed together share
-// the matrix and are swept in the same launches (u and v in 2D); each keeps its own norms and
-// stop state.  A call ends with k_smooth_finish.
-void enqueueSmoothers(const Plan &pl, const int *slots, int neq) {
+// writes the other.  Equations listed together share the matrix and are swept in the same
+// launches (u and v in 2D); each keeps its own norms and stop state.
+void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMode &md = SmoothMode{}) {
     const ifl_config &c = *pl.cfg;
-    const int K = sweepBlockOf(c, pl.dim);
+    const int K = sweepBlockOf(c, pl.dim, slots[0]);
     const bool fused = pl.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS;
-    int n[2], nL[2], nLmax = 0;
+    if (!fused && neq > 1) {
+        for (int k = 0; k < neq; ++k) enqueueSmoothers(pl, &slots[k], 1, md);
+        return;
+    }
+    int n[2], nL[2], flip[2], nLmax = 0;
     for (int k = 0; k < neq; ++k) {
         const int q = slots[k] < 4 ? slots[k] : 3;
         n[k] = c.n_iter[q];
         nL[k] = (n[k] + K - 1) / K;
+        flip[k] = (md.fast && nL[k] % 2 == 1) ? 1 : 0;
         nLmax = std::max(nLmax, nL[k]);
     }
-    if (fused || neq == 1) {
-        for (int L = 0; L < nLmax; ++L) {
-            ifl::SweepSel se[2];
-            int m = 0;
-            for (int k = 0; k < neq; ++k) {
-                const int kk = std::min(K, n[k] - L * K);
-                if (kk <= 0) continue;
-                const int kp = L > 0 ? std::min(K, n[k] - (L - 1) * K) : 0;
-                se[m++] = ifl::SweepSel{slots[k], slots[k] < 4 ? slots[k] : 3, kk, kp};
-            }
-            if (m == 1) se[1] = se[0];
-            if (fused) {
-                const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR;
-                ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
-            } else {
-                for (int k = 0; k < m; ++k) {
-                    if (c.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(pl.L, se[k], L);
-                    else ifl::launch_rb_sweep3d(pl.L, se[k], L);
-                }
+    for (int L = 0; L < nLmax; ++L) {
+        ifl::SweepSel se[2];
+        int m = 0;
+        for (int k = 0; k < neq; ++k) {
+            const int kk = std::min(K, n[k] - L * K);
+            if (kk <= 0) continue;
+            const int kp = L > 0 ? std::min(K, n[k] - (L - 1) * K) : 0;
+            se[m] = ifl::SweepSel{slots[k], slots[k] < 4 ? slots[k] : 3, kk, kp};
+            se[m].flip = flip[k];
+            se[m].accum = (md.accumLast && fused && L == nL[k] - 1) ? 1 : 0;
+            ++m;
+        }
+        if (m == 1) se[1] = se[0];
+        if (fused) {
+            const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR;
+            ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
+        } else {
+            for (int k = 0; k < m; ++k) {
+                if (c.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(pl.L, se[k], L);
+                else ifl::launch_rb_sweep3d(pl.L, se[k], L);
             }
         }
-    } else {
-        for (int k = 0; k < neq; ++k) enqueueSmoothers(pl, &slots[k], 1);
-        return;
     }
     for (int k = 0; k < neq; ++k) {
-        if (nL[k] % 2 == 1) ifl::launch_copy_back(pl.L, slots[k]);
+        if (!md.fast) {
+            if (nL[k] % 2 == 1) ifl::launch_copy_back(pl.L, slots[k], 0);
+            else if (slots[k] < 3 && nL[k] > 0) ifl::launch_copy_back(pl.L, slots[k], 1);
+        }
         if (n[k] > 0) {
             const int q = slots[k] < 4 ? slots[k] : 3;
             const int kLast = n[k] - (nL[k] - 1) * K;
-            ifl::launch_smooth_finish(pl.L, slots[k], q, nL[k], kLast, sweepKind(pl), sweepRecordsSingle(pl));
+            if (md.fast && md.pending) {
+                ifl::Finish &f = *md.pending;
+                f.kind = sweepKind(pl);
+                f.nprev = sweepRecordsSingle(pl);
+                f.slot[f.n] = slots[k];
+                f.q[f.n] = q;
+                f.nL[f.n] = nL[k];
+                f.kLast[f.n] = kLast;
+                ++f.n;
+            } else {
+                ifl::launch_smooth_finish(pl.L, slots[k], q, nL[k], kLast, sweepKind(pl), sweepRecordsSingle(pl));
+            }
         }
     }
 }
 
-void enqueueMomentumSmoothers(const Plan &pl) {
+void enqueueMomentumSmoothers(const Plan &pl, const SmoothMode &md = SmoothMode{}) {
     if (pl.dim == 2 && pl.cfg->smoother == IFL_SMOOTHER_RBGS) {      // u and v share the matrix
         const int slots[2] = {0, 1};
-        enqueueSmoothers(pl, slots, 2);
+        enqueueSmoothers(pl, slots, 2, md);
         return;
     }
-    for (int q = 0; q < pl.dim; ++q) enqueueSmoothers(pl, &q, 1);   // :178, :180, :183
+    for (int q = 0; q < pl.dim; ++q) enqueueSmoothers(pl, &q, 1, md);   // :178, :180, :183
 }
 
-void enqueuePressureCorrection(const Plan &pl) {
+// Solver.cu:196 plus the FP64-defect refinement cycles.  Returns the calls still to be closed
+// (fast path) by the next kernel.
+ifl::Finish enqueuePressureCorrection(const Plan &pl, bool fast = false) {
     const int sp = 3, sd = 4;
-    enqueueSmoothers(pl, &sp, 1);                                    // Solver.cu:196
-    for (int r = 0; r < pl.cfg->p_refine; ++r) {                     // FP64-defect refinement
-        ifl::launch_pcorr_defect(pl.L);
-        enqueueSmoothers(pl, &sd, 1);
-        ifl::launch_pcorr_accumulate(pl.L);
+    const bool fused = pl.dim == 2 && pl.cfg->smoother == IFL_SMOOTHER_RBGS;
+    ifl::Finish pend;
+    SmoothMode md;
+    md.fast = fast;
+    md.pending = &pend;
+    enqueueSmoothers(pl, &sp, 1, md);
+    for (int r = 0; r < pl.cfg->p_refine; ++r) {
+        ifl::launch_pcorr_defect(pl.L, pend);
+        pend = ifl::Finish{};
+        SmoothMode mr = md;
+        mr.accumLast = fast && fused && r == pl.cfg->p_refine - 1;
+        enqueueSmoothers(pl, &sd, 1, mr);
+        if (!mr.accumLast) ifl::launch_pcorr_accumulate(pl.L);
     }
+    return pend;
 }
 
 void enqueueOuter(const Plan &pl) {
     ifl::launch_mom_assemble(pl.L);                                  // :165-175
-    enqueueMomentumSmoothers(pl);                                    // :178-184
-    ifl::launch_rhie_chow(pl.L);                                     // :186-187
+    ifl::Finish fm;
+    SmoothMode md;
+    md.fast = true;
+    md.pending = &fm;
+    enqueueMomentumSmoothers(pl, md);                                // :178-184
+    ifl::launch_rhie_chow(pl.L, fm);                                 // :186-187
     ifl::launch_pcorr_assemble(pl.L);                                // :189-193
-    enqueuePressureCorrection(pl);                                   // :196
-    ifl::launch_update(pl.L);                                        // :198
+    const ifl::Finish fp = enqueuePressureCorrection(pl, true);      // :196
+    ifl::launch_update(pl.L, fp);                                    // :198
     ifl::launch_reduce_cells(pl.L, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH);  // :200-268
 }
 
@@ -464,9 +511,11 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     // Solver.cu:13-104: uniform initial u, v, w, p, temp; everything else zero.
     e = hipMemsetAsync(h->arena, 0, total, h->stream);
     std::vector<float> host(N);
-    float *initDst[5] = {P.u, P.v, P.w, P.p, temp};
-    for (int q = 0; q < 5 && e == hipSuccess; ++q) {
-        std::fill(host.begin(), host.end(), (float)cfg->init[q]);
+    // u, v, w also into their smoothers' partner buffers, which stay equal to them
+    float *initDst[8] = {P.u, P.v, P.w, P.p, temp, P.alt[0], P.alt[1], P.alt[2]};
+    const int initSrc[8] = {0, 1, 2, 3, 4, 0, 1, 2};
+    for (int q = 0; q < 8 && e == hipSuccess; ++q) {
+        std::fill(host.begin(), host.end(), (float)cfg->init[initSrc[q]]);
         e = hipMemcpyAsync(initDst[q], host.data(), N * 4, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
@@ -536,6 +585,8 @@ int ifl_set_field(ifl_solver *h, int field, const float *src, size_t n) {
     if (!dst) return fail(IFL_ERR_ARG, "bad field id");
     if (n != fieldSize(h->cfg, field)) return fail(IFL_ERR_ARG, "size mismatch");
     IFL_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, h->stream));
+    if (field == IFL_FIELD_U || field == IFL_FIELD_V || field == IFL_FIELD_W)   // and the partner buffer
+        IFL_HIP(hipMemcpyAsync(h->P.alt[field - IFL_FIELD_U], src, n * 4, hipMemcpyHostToDevice, h->stream));
     IFL_HIP(hipStreamSynchronize(h->stream));
     return IFL_OK;
 }
@@ -682,7 +733,7 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
         switch (kernel) {
             case IFL_KERNEL_SWEEP: {
                 // one launch of the p' smoother as the outer iteration runs it (first launch of a call)
-                const int K = sweepBlockOf(h->cfg, pl.dim);
+                const int K = sweepBlockOf(h->cfg, pl.dim, 3);
                 const ifl::SweepSel e{3, 3, K, 0};
                 if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(L, e, 0);
                 else if (pl.dim == 2) ifl::launch_rb_sweep2d(L, ifl::COEF_PCORR, e, e, 1, 0, K);
@@ -862,7 +913,7 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         bestUs = 1e300;
         Settings x = best;
         for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_32x4; ++tile)
-            for (int K = 1; K <= 2; ++K) {
+            for (int K = 1; K <= 4; K *= 2) {
                 x.tile = tile;
                 x.K = K;
                 double us;
@@ -1192,7 +1243,24 @@ int batchBuild(ifl_batch *b, const std::vector<int> &ids) {
     const int n = (int)ids.size();
     const size_t pBytes = align256(sizeof(ifl::CaseParams) * n);
     const size_t kBytes = sizeof(int) * (2 * n + 1);
-    const size_t need = pBytes + ifl::MAP_KINDS * align256(kBytes);
+    // per-workgroup tables: count the workgroups of every kind first
+    size_t blkBytes[ifl::MAP_KINDS];
+    for (int k = 0; k < ifl::MAP_KINDS; ++k) {
+        long long tot = 0;
+        for (int c = 0; c < n; ++c) {
+            const ifl::CaseParams &P = b->cases[ids[c]]->P;
+            switch (k) {
+                case ifl::MAP_CELL: tot += ifl::cell_blocks(P); break;
+                case ifl::MAP_TILE: tot += P.dim == 2 ? ifl::tile_count(P.nx, P.ny, b->tile ? b->tile : ifl::TILE_64x16) : 0; break;
+                case ifl::MAP_COPY: tot += ifl::copy_blocks(P); break;
+                case ifl::MAP_ZMARCH: tot += P.dim == 3 ? ifl::zmarch_blocks(P) : 0; break;
+                default: tot += 1; break;
+            }
+        }
+        blkBytes[k] = align256(sizeof(int4) * (size_t)std::max(1LL, tot));
+    }
+    size_t need = pBytes + ifl::MAP_KINDS * align256(kBytes);
+    for (int k = 0; k < ifl::MAP_KINDS; ++k) need += blkBytes[k];
     if (need > b->tableBytes) {
         if (b->table) IFL_HIP(hipFree(b->table));
         b->table = nullptr;
@@ -1237,6 +1305,16 @@ int batchBuild(ifl_batch *b, const std::vector<int> &ids) {
         L.total[k] = (int)acc;
         L.M.start[k] = (const int *)(b->table + off);
         L.M.dimx[k] = (const int *)(b->table + off) + n + 1;
+    }
+    size_t boff = pBytes + ifl::MAP_KINDS * align256(kBytes);
+    for (int k = 0; k < ifl::MAP_KINDS; ++k) {
+        const int *start = (const int *)(ht.data() + pBytes + k * align256(kBytes));
+        const int *dimx = start + n + 1;
+        int4 *e = (int4 *)(ht.data() + boff);
+        for (int c = 0; c < n; ++c)
+            for (int w = start[c]; w < start[c + 1]; ++w) e[w] = make_int4(c, w - start[c], start[c + 1] - start[c], dimx[c]);
+        L.M.blk[k] = (const int4 *)(b->table + boff);
+        boff += blkBytes[k];
     }
     for (int c = 0; c < n; ++c) hp[c] = b->cases[ids[c]]->P;
     IFL_HIP(hipMemcpyAsync(b->table, ht.data(), need, hipMemcpyHostToDevice, b->stream));

@@ -165,7 +165,7 @@ __device__ __forceinline__ int caseDone(const CaseParams &P) { return P.st->done
 // ---------------- one case or a batch of cases per launch ----------------
 // A kernel runs either on one case (its CaseParams is a kernel argument and the grid is the
 // case's own) or on a batch: the grid is the concatenation of every case's workgroups, and a
-// workgroup finds its case in a prefix table.  Either way the kernel body sees the case's
+// workgroup finds its case in a per-workgroup table.  Either way the kernel body sees the case's
 // parameters and the workgroup's position inside that case's own grid.
 enum { MAP_CELL = 0, MAP_TILE = 1, MAP_ONE = 2, MAP_COPY = 3, MAP_ZMARCH = 4, MAP_KINDS = 5 };
 
@@ -174,6 +174,7 @@ struct BatchMap {
     const CaseParams *P;            // device array, one per case
     const int *start[MAP_KINDS];    // prefix sums of the workgroups per case (ncase + 1)
     const int *dimx[MAP_KINDS];     // x extent of each case's grid
+    const int4 *blk[MAP_KINDS];     // per workgroup: {case, index in the case's grid, its size, x extent}
 };
 
 struct Blk {
@@ -183,24 +184,15 @@ struct Blk {
     int n;            // workgroups in the case's grid
 };
 
-__device__ __forceinline__ int findCase(const int *start, int n, int b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <bool BATCH>
 __device__ __forceinline__ const CaseParams &caseOf(const CaseParams &P0, const BatchMap &M, int kind, Blk &b) {
     if constexpr (BATCH) {
-        const int c = __builtin_amdgcn_readfirstlane(findCase(M.start[kind], M.ncase, blockIdx.x));
-        const int s0 = M.start[kind][c];
-        const int n = M.start[kind][c + 1] - s0;
-        const int gx = M.dimx[kind][c];
-        const int loc = blockIdx.x - s0;
+        // one (scalar) load: the workgroup's entry of the batch's block table
+        const int4 e = M.blk[kind][blockIdx.x];
+        const int c = __builtin_amdgcn_readfirstlane(e.x);
+        const int loc = __builtin_amdgcn_readfirstlane(e.y);
+        const int n = __builtin_amdgcn_readfirstlane(e.z);
+        const int gx = __builtin_amdgcn_readfirstlane(e.w);
         b.gx = gx;
         b.gy = n / gx;
         b.gz = 1;
@@ -236,6 +228,11 @@ __device__ __forceinline__ bool leader(const Blk &b) { return b.flat == 0 && thr
 __device__ __forceinline__ float *slotField(const CaseParams &P, int slot) {
     return slot == 0 ? P.u : slot == 1 ? P.v : slot == 2 ? P.w : slot == 3 ? P.pc : P.tmp2;
 }
+// The slot's ping-pong partner (a select chain: indexing the kernel-argument array with a
+// runtime value would copy the whole argument block to scratch memory).
+__device__ __forceinline__ float *slotAlt(const CaseParams &P, int slot) {
+    return slot == 0 ? P.alt[0] : slot == 1 ? P.alt[1] : slot == 2 ? P.alt[2] : slot == 3 ? P.alt[3] : P.alt[4];
+}
 __device__ __forceinline__ const float *slotSource(const CaseParams &P, int slot) {
     return slot == 0 ? P.bu : slot == 1 ? P.bv : slot == 2 ? P.bw : slot == 3 ? P.bp : P.tmp;
 }
@@ -253,6 +250,70 @@ __device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int 
 // tol.  A met test stops the call after the launch that contains the sweep (with one sweep per
 // launch: right after that sweep, as in the reference).  Workgroup 0 publishes the state after
 // launch L-1.  With L == number of launches this closes the call (sweeps count, last sum).
+// The stop state after launch L-2 of a smoother call (what launchRuns reads), loaded early by
+// kernels that overlap these loads with their own work.
+struct PrevState {
+    bool stoppedBefore;
+    double mx;
+    int before;
+};
+__device__ __forceinline__ PrevState loadPrev(const CaseParams &P, int slot, int L) {
+    PrevState v{false, -1e20, 0};
+    if (L >= 2) {
+        const SmoothState &c = P.st->sm[slot];
+        const int pp = L & 1;
+        v.stoppedBefore = c.stopped[pp] != 0;
+        v.mx = c.maxNorm[pp];
+        v.before = c.swAt[pp];
+    }
+    return v;
+}
+
+// launchRuns with the totals T[kk] of the previous launch's sweeps already reduced by the caller
+// and the state already loaded.
+template <int K>
+__device__ __forceinline__ bool launchDecide(const CaseParams &P, const Blk &b, int slot, int L, int kPrev,
+                                             const PrevState &pv, const double (&T)[K], double tol) {
+    if (L == 0) return true;
+    double mx = pv.mx;
+    bool stopped = pv.stoppedBefore;
+    double last = 0.0;
+    int ran = pv.before;
+    if (!pv.stoppedBefore) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            if (kk < kPrev) {
+                last = T[kk];
+                const double norm = sqrt(last / (double)P.N);
+                mx = fmax(norm, mx);
+                if (norm / (mx + 1e-20) < tol) stopped = true;
+            }
+        }
+        ran = pv.before + kPrev;
+    }
+    if (leader(b)) {
+        SmoothState &c = P.st->sm[slot];
+        const int q = (L - 1) & 1;
+        c.stopped[q] = stopped ? 1 : 0;
+        c.maxNorm[q] = mx;
+        c.swAt[q] = ran;
+        if (!pv.stoppedBefore) c.lastSum = last;
+        c.sweeps = ran;
+    }
+    return !stopped;
+}
+
+// Smoother calls whose closing step (k_smooth_finish) workgroup 0 of another kernel performs.
+struct Finish {
+    int n = 0;
+    int kind = 0;      // workgroup kind of the calls' launches (their partial counts)
+    int nprev = 0;     // their workgroup count (one case)
+    int slot[3] = {0, 0, 0};
+    int q[3] = {0, 0, 0};
+    int nL[3] = {0, 0, 0};
+    int kLast[3] = {0, 0, 0};
+};
+
 __device__ __forceinline__ bool launchRuns(const CaseParams &P, const Blk &b, int slot, int L, int kPrev, int nprev,
                                            double tol) {
     if (L == 0) return true;
@@ -284,6 +345,19 @@ __device__ __forceinline__ bool launchRuns(const CaseParams &P, const Blk &b, in
         c.sweeps = ran;
     }
     return !stopped;
+}
+
+// Close the smoother calls listed in fin (workgroup 0 of each case only): publish the state after
+// each call's last launch and add its sweeps to the outer iteration's count.
+template <bool BATCH>
+__device__ __forceinline__ void runFinish(const CaseParams &P, const BatchMap &M, const Blk &b, const Finish &fin) {
+    if (fin.n == 0 || b.flat != 0) return;
+    for (int i = 0; i < fin.n; ++i) {
+        int np = fin.nprev;
+        if constexpr (BATCH) np = M.start[fin.kind][(int)(&P - M.P) + 1] - M.start[fin.kind][(int)(&P - M.P)];
+        launchRuns(P, b, fin.slot[i], fin.nL[i], fin.kLast[i], np, P.tol[fin.q[i]]);
+        if (threadIdx.x == 0) P.st->sweepsIter[fin.q[i]] += P.st->sm[fin.slot[i]].sweeps;
+    }
 }
 
 }  // namespace ifl

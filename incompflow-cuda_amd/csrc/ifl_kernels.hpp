@@ -13,6 +13,9 @@ struct SweepSel {
     int q;       // equation index for relax / tol (u, v, w, p')
     int k;       // sweeps of this equation in this launch (0..K)
     int kPrev;   // sweeps of this equation in the previous launch of the call
+    int flip = 0;    // 1: launch L reads the partner buffer when L is even (odd launch counts end
+                     //    in the caller's field without a copy; the partner holds the field)
+    int accum = 0;   // 1: also add the result into p' (last launch of a refinement correction)
 };
 
 enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2 };
@@ -41,15 +44,16 @@ int cell_dimx(const CaseParams &P);
 void launch_init_faces(const Lc &L);
 void launch_face_bc(const Lc &L);
 void launch_mom_assemble(const Lc &L);
-void launch_rhie_chow(const Lc &L);
+// fin: smoother calls closed by workgroup 0 of this launch (instead of k_smooth_finish)
+void launch_rhie_chow(const Lc &L, const Finish &fin = Finish{});
 void launch_pcorr_assemble(const Lc &L);
-void launch_update(const Lc &L);
-void launch_pcorr_defect(const Lc &L);
+void launch_update(const Lc &L, const Finish &fin = Finish{});
+void launch_pcorr_defect(const Lc &L, const Finish &fin = Finish{});
 void launch_pcorr_accumulate(const Lc &L);
 void launch_reduce_cells(const Lc &L, int mask);
 void launch_sweep_jacobi(const Lc &L, const SweepSel &e, int launch);
 void launch_smooth_finish(const Lc &L, int slot, int q, int nL, int kLast, int sweepKind, int nprev);
-void launch_copy_back(const Lc &L, int slot);
+void launch_copy_back(const Lc &L, int slot, int dir);   // dir 0: partner -> field, 1: field -> partner
 // 2D fused red-black sweeps (K = block sweeps per launch), one or two equations sharing a matrix
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block);
 // 3D z-marching red-black sweep

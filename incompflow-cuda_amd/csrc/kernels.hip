@@ -200,9 +200,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_jacobi(CaseParams P0, BatchMap
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
     const bool run = e.k > 0 && launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q]);
-    float *fld = slotField(P, e.slot), *alt = P.alt[e.slot];
-    const float *in = (launch & 1) ? alt : fld;
-    float *out = (launch & 1) ? fld : alt;
+    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
+    const bool odd = ((launch + e.flip) & 1) != 0;
+    const float *in = odd ? alt : fld;
+    float *out = odd ? fld : alt;
     const float *coef = slotCoef(P, e.slot), *src = slotSource(P, e.slot);
     const float rlx = P.relax[e.q];
     double s[1] = {0.0};
@@ -230,14 +231,18 @@ __global__ __launch_bounds__(kBlock) void k_smooth_finish(CaseParams P0, BatchMa
     if (threadIdx.x == 0) P.st->sweepsIter[q] += P.st->sm[slot].sweeps;
 }
 
-// A sweep call that ended after an odd number of launches left its result in the partner buffer.
+// dir 0: a sweep call that ended after an odd number of launches left its result in the partner
+// buffer; copy it back.  dir 1: copy the field into its partner (keeps them equal).
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P0, BatchMap M, int slot) {
+__global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P0, BatchMap M, int slot, int dir) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_COPY, blk);
     if (caseDone(P)) return;
     const int t = blk.flat * kBlock + threadIdx.x;
-    if (t < P.N) slotField(P, slot)[t] = P.alt[slot][t];
+    if (t < P.N) {
+        if (dir == 0) slotField(P, slot)[t] = slotAlt(P, slot)[t];
+        else slotAlt(P, slot)[t] = slotField(P, slot)[t];
+    }
 }
 
 // ---------------- Rhie-Chow, kernels.cuh:336-500, then face BCs (Solver.cu:187) ----------------
@@ -267,10 +272,11 @@ __device__ __forceinline__ float rhieChow(const CaseParams &P, int d, int c, int
 }
 
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P0, BatchMap M) {
+__global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P0, BatchMap M, Finish fin) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
+    runFinish<BATCH>(P, M, blk, fin);
     const float *aC = P.mcoef;   // plane sC
     forCells(P, blk, [&](int i, int j, int k, int c) {
         if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
@@ -318,6 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchM
         if (DIM == 3) b += rho * Az * (P.wf[wfi(P, i, j, k)] - P.wf[wfi(P, i, j, k + 1)]);
         P.bp[c] = b;
         P.pc[c] = 0.0f;                       // Solver.cu:193
+        P.alt[3][c] = 0.0f;                   // the partner buffer too (odd launch counts start there)
         s[0] += (double)b * (double)b;
     });
     storeBlockSums<1>(s, P.partMass + blk.flat);
@@ -325,16 +332,18 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchM
 
 // ---------------- FP64 defect of the p' system (refinement), delta = 0 ----------------
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P0, BatchMap M) {
+__global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P0, BatchMap M, Finish fin) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
+    runFinish<BATCH>(P, M, blk, fin);
     forCells(P, blk, [&](int i, int j, int k, int c) {
         float a[7];
 #pragma unroll
         for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[c + P.N * s] : 0.0f;
         P.tmp[c] = (float)momResidual<DIM>(P, P.pc, a, P.bp[c], i, j, k, c);
         P.tmp2[c] = 0.0f;
+        P.alt[4][c] = 0.0f;
     });
 }
 
@@ -372,25 +381,34 @@ __device__ __forceinline__ float faceCorr(const CaseParams &P, int d, int c, int
 }
 
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M) {
+__global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M, Finish fin) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
+    runFinish<BATCH>(P, M, blk, fin);
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     forCells(P, blk, [&](int i, int j, int k, int c) {
         const int pl = P.nx * P.ny;
         const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
         P.p[c] += dp;
         s[0] += (double)dp * dp;
+        // corrected velocities also go to the smoothers' partner buffers, which a momentum call
+        // with an odd launch count reads first
         const float du = velCorr(P, 0, i, P.nx, c, 1);
-        P.u[c] += du;
+        const float un = P.u[c] + du;
+        P.u[c] = un;
+        P.alt[0][c] = un;
         s[1] += (double)du * du;
         const float dv = velCorr(P, 1, j, P.ny, c, P.nx);
-        P.v[c] += dv;
+        const float vn = P.v[c] + dv;
+        P.v[c] = vn;
+        P.alt[1][c] = vn;
         s[2] += (double)dv * dv;
         if (DIM == 3) {
             const float dw = velCorr(P, 2, k, P.nz, c, pl);
-            P.w[c] += dw;
+            const float wn = P.w[c] + dw;
+            P.w[c] = wn;
+            P.alt[2][c] = wn;
             s[3] += (double)dw * dw;
         }
         if (i >= 1) {
@@ -559,10 +577,10 @@ static dim3 gridOf(const Lc &L, int kind) {
 void launch_init_faces(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_init_faces); }
 void launch_face_bc(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_face_bc); }
 void launch_mom_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_mom_assemble); }
-void launch_rhie_chow(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_rhie_chow); }
+void launch_rhie_chow(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_rhie_chow, fin); }
 void launch_pcorr_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_assemble); }
-void launch_update(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_update); }
-void launch_pcorr_defect(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_defect); }
+void launch_update(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_update, fin); }
+void launch_pcorr_defect(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_defect, fin); }
 void launch_pcorr_accumulate(const Lc &L) { IFL_LAUNCH(L, MAP_CELL, k_pcorr_accumulate); }
 void launch_reduce_cells(const Lc &L, int mask) { IFL_LAUNCH(L, MAP_ONE, k_reduce_cells, cell_blocks(*L.P), mask); }
 void launch_sweep_jacobi(const Lc &L, const SweepSel &e, int launch) {
@@ -571,6 +589,6 @@ void launch_sweep_jacobi(const Lc &L, const SweepSel &e, int launch) {
 void launch_smooth_finish(const Lc &L, int slot, int q, int nL, int kLast, int sweepKind, int nprev) {
     IFL_LAUNCH(L, MAP_ONE, k_smooth_finish, slot, q, nL, kLast, sweepKind, nprev);
 }
-void launch_copy_back(const Lc &L, int slot) { IFL_LAUNCH(L, MAP_COPY, k_copy_back, slot); }
+void launch_copy_back(const Lc &L, int slot, int dir) { IFL_LAUNCH(L, MAP_COPY, k_copy_back, slot, dir); }
 
 }  // namespace ifl

@@ -24,12 +24,15 @@
 
 namespace ifl {
 
+// Colour-split LDS layout: cell (lx, ly) of a staged region has colour (lx + ly) & 1 (tile origins
+// and halos are even, so this is the global red-black colour) and sits at [colour][ly][lx >> 1].
+// The same-colour cells of a half sweep are then contiguous, and so are their neighbours.
 template <int TX, int TY, int K, int NEQ, int KIND, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1, int launch) {
     constexpr int H = 2 * K;                                   // field halo
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
-    constexpr int HX = TX + 2 * H, HY = TY + 2 * H;
-    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;          // cells ever updated (sources)
+    constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
+    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0, SX2 = SX / 2;   // cells ever updated (sources)
     constexpr bool PC = KIND == COEF_PCORR, MOM = KIND == COEF_MOMENTUM;
     constexpr int UX = SX + 1, UY = SY;                        // uf faces of the updated cells
     constexpr int VX = SX, VY = SY + 1;                        // vf faces of the updated cells
@@ -37,14 +40,21 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     constexpr int NSR = (SX * SY + kBlock - 1) / kBlock;
     constexpr int NUT = MOM ? (UX * UY + kBlock - 1) / kBlock : 1;
     constexpr int NVT = MOM ? (VX * VY + kBlock - 1) / kBlock : 1;
-    __shared__ float sphi[NEQ][HY][HX];
-    __shared__ float ssrc[NEQ][SY][SX];
-    __shared__ float srinv[PC ? HY : 1][PC ? HX : 1];
+    constexpr int NPF = 4;                                     // prefetched partials per thread and sweep
+    static_assert(HX % 2 == 0 && SX % 2 == 0, "colour split needs even widths");
+    __shared__ float sphi[NEQ][2][HY][HX2];
+    __shared__ float ssrc[NEQ][2][SY][SX2];
+    // p': 1/aC while the coefficients are formed, then the coefficient towards +x (same storage),
+    // and the coefficient towards +y
+    __shared__ float sco[PC ? 2 : 1][PC ? 2 : 1][PC ? HY : 1][PC ? HX2 : 1];
+    auto &srv = sco[0];
+    auto &sex = sco[0];
+    auto &sey = sco[PC ? 1 : 0];
     __shared__ float suf[MOM ? UY : 1][MOM ? UX : 1];
     __shared__ float svf[MOM ? VY : 1][MOM ? VX : 1];
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_TILE, blk);
-    if (caseDone(P)) return;
+    const int done = caseDone(P);          // checked after the loads are issued (they are in bounds)
     const int nx = P.nx, ny = P.ny;
     const int tilesX = (nx + TX - 1) / TX;
     const int by = blk.flat / tilesX;
@@ -56,13 +66,13 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     float rlx[2];
 #pragma unroll
     for (int q = 0; q < NEQ; ++q) {
-        float *fld = slotField(P, eqs[q].slot), *alt = P.alt[eqs[q].slot];
-        in[q] = (launch & 1) ? alt : fld;
-        out[q] = (launch & 1) ? fld : alt;
+        float *fld = slotField(P, eqs[q].slot), *alt = slotAlt(P, eqs[q].slot);
+        const bool odd = ((launch + eqs[q].flip) & 1) != 0;
+        in[q] = odd ? alt : fld;
+        out[q] = odd ? fld : alt;
         src[q] = slotSource(P, eqs[q].slot);
         rlx[q] = P.relax[eqs[q].q];
     }
-    const float *coef = PC ? P.rinv : slotCoef(P, eqs[0].slot);
     const int tid = threadIdx.x;
 
     // ---- every global load of the tile first ----
@@ -76,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
         const int g = gy * nx + gx;
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) rphi[q][m] = ok ? in[q][g] : 0.0f;
-        if constexpr (PC) rr[m] = ok ? coef[g] : 0.0f;
+        if constexpr (PC) rr[m] = ok ? P.rinv[g] : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
@@ -106,25 +116,16 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
             rvf[m] = in ? P.vf[vfi(P, gx, fy, 0)] : 0.0f;
         }
     }
-
-    // ---- stop decision of this launch from the previous launch's partials ----
-    bool act[2] = {false, false};
-    int kq[2] = {0, 0};
-#pragma unroll
-    for (int q = 0; q < NEQ; ++q) {
-        act[q] = eqs[q].k > 0 && launchRuns(P, blk, eqs[q].slot, launch, eqs[q].kPrev, blk.n, P.tol[eqs[q].q]);
-        kq[q] = act[q] ? eqs[q].k : 0;
-    }
-
-    // ---- stage to LDS ----
+    // ---- stage to LDS (colour split) ----
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int e = tid + m * kBlock;
         if (e < HX * HY) {
             const int ly = e / HX, lx = e - ly * HX;
+            const int c = (lx + ly) & 1;
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) sphi[q][ly][lx] = rphi[q][m];
-            if constexpr (PC) srinv[ly][lx] = rr[m];
+            for (int q = 0; q < NEQ; ++q) sphi[q][c][ly][lx >> 1] = rphi[q][m];
+            if constexpr (PC) srv[c][ly][lx >> 1] = rr[m];
         }
     }
 #pragma unroll
@@ -132,8 +133,9 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
         const int e = tid + m * kBlock;
         if (e < SX * SY) {
             const int ly = e / SX, lx = e - ly * SX;
+            const int c = (lx + ly) & 1;
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) ssrc[q][ly][lx] = rsrc[q][m];
+            for (int q = 0; q < NEQ; ++q) ssrc[q][c][ly][lx >> 1] = rsrc[q][m];
         }
     }
     if constexpr (MOM) {
@@ -148,47 +150,99 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
             if (e < VX * VY) svf[e / VX][e % VX] = rvf[m];
         }
     }
+    // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*256
+    // records are loaded now (after the tile's registers are free) and added after the relax
+    // phases, so their latency overlaps them ----
+    const int nprev = blk.n;
+    PrevState prev[NEQ];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) prev[q] = loadPrev(P, eqs[q].slot, launch);
+    double pf[NEQ][K][NPF];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q)
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const bool need = launch > 0 && kk < eqs[q].kPrev;
+            const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
+#pragma unroll
+            for (int f = 0; f < NPF; ++f) {
+                const int bidx = tid + f * kBlock;
+                pf[q][kk][f] = (need && bidx < nprev) ? part[bidx] : 0.0;
+            }
+        }
+    if (done) return;                      // a finished case: nothing is written
     __syncthreads();
 
-    const double Kx = -0.5 * P.rho * P.Ax * P.Ax;   // kernels.cuh:518 prefactor, same evaluation order
+    // ---- p' edge coefficients, once per launch (calcPresCorrLinkCoefKernel, kernels.cuh:517-529):
+    // a_E(c) = -0.5 rho Ax^2 (1/aC_c + 1/aC_E) in double, rounded; a_W(c) = a_E(c - 1) exactly
+    // (the float sum is commutative), likewise a_S(c) = a_N(c - nx) ----
+    const double Kx = -0.5 * P.rho * P.Ax * P.Ax;
     const double Ky = -0.5 * P.rho * P.Ay * P.Ay;
+    if constexpr (PC) {
+        float rex[NST], rey[NST];
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int e = tid + m * kBlock;
+            rex[m] = rey[m] = 0.0f;
+            if (e < HX * HY) {
+                const int ly = e / HX, lx = e - ly * HX;
+                const int c = (lx + ly) & 1;
+                const int gx = x0 - H + lx, gy = y0 - H + ly;
+                const bool inGrid = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                const float rC = srv[c][ly][lx >> 1];
+                if (inGrid && gx < nx - 1 && lx + 1 < HX) rex[m] = (float)(Kx * (rC + srv[c ^ 1][ly][(lx + 1) >> 1]));
+                if (inGrid && gy < ny - 1 && ly + 1 < HY) rey[m] = (float)(Ky * (rC + srv[c ^ 1][ly + 1][lx >> 1]));
+            }
+        }
+        __syncthreads();   // every 1/aC read before its storage takes the +x coefficients
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int e = tid + m * kBlock;
+            if (e < HX * HY) {
+                const int ly = e / HX, lx = e - ly * HX;
+                const int c = (lx + ly) & 1;
+                sex[c][ly][lx >> 1] = rex[m];
+                sey[c][ly][lx >> 1] = rey[m];
+            }
+        }
+        __syncthreads();
+    }
+
     double acc[K][2];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = 0.0;
 
-    // relax the cell (gx, gy) in half sweep p (sweep p/2 of this launch)
-    auto relax = [&](int gx, int gy, int p) {
-        const int lx = gx - x0 + H, ly = gy - y0 + H;     // field / 1/aC position
-        const int sx = gx - x0 + R0, sy = gy - y0 + R0;   // source / face position
+    // relax the cell (gx, gy) of colour `col` in half sweep p (sweep p/2 of this launch)
+    auto relax = [&](int gx, int gy, int col, int p) {
+        const int lx = gx - x0 + H, ly = gy - y0 + H;     // field position
+        const int ix = lx >> 1, ixE = (lx + 1) >> 1, ixW = (lx - 1) >> 1;
+        const int oc = col ^ 1;
         float a[7];
-        if constexpr (KIND == COEF_STORED) {
-            const int g = gy * nx + gx;
-#pragma unroll
-            for (int s = 0; s < 5; ++s) a[s] = coef[g + P.N * s];
-        } else if constexpr (MOM) {
+        if constexpr (MOM) {
+            const int sx = gx - x0 + R0, sy = gy - y0 + R0;
             const float uw = suf[sy][sx], ue = suf[sy][sx + 1];
             const float vs = svf[sy][sx], vn = svf[sy + 1][sx];
             momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
             momBcRowsCoef<2>(P, gx, gy, 0, a);
         } else {
-            const float rC = srinv[ly][lx];
-            a[sE] = (gx < nx - 1) ? (float)(Kx * (rC + srinv[ly][lx + 1])) : 0.0f;
-            a[sW] = (gx > 0) ? (float)(Kx * (rC + srinv[ly][lx - 1])) : 0.0f;
-            a[sN] = (gy < ny - 1) ? (float)(Ky * (rC + srinv[ly + 1][lx])) : 0.0f;
-            a[sS] = (gy > 0) ? (float)(Ky * (rC + srinv[ly - 1][lx])) : 0.0f;
+            a[sE] = sex[col][ly][ix];
+            a[sW] = sex[oc][ly][ixW];
+            a[sN] = sey[col][ly][ix];
+            a[sS] = sey[oc][ly - 1][ix];
             a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
         }
         const bool owned = gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY;
         const int j = p >> 1;
+        const int sxi = (lx - 1) >> 1, syi = ly - 1;       // source position (same colour)
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) {
-            if (2 * kq[q] <= p) continue;
-            const float phi_C = sphi[q][ly][lx];
-            float newPhi = ssrc[q][sy][sx] - a[sE] * sphi[q][ly][lx + 1] - a[sW] * sphi[q][ly][lx - 1]
-                           - a[sN] * sphi[q][ly + 1][lx] - a[sS] * sphi[q][ly - 1][lx];
+            if (2 * eqs[q].k <= p) continue;
+            const float phi_C = sphi[q][col][ly][ix];
+            float newPhi = ssrc[q][col][syi][sxi] - a[sE] * sphi[q][oc][ly][ixE] - a[sW] * sphi[q][oc][ly][ixW]
+                           - a[sN] * sphi[q][oc][ly + 1][ix] - a[sS] * sphi[q][oc][ly - 1][ix];
             newPhi /= a[sC];
             const float d = rlx[q] * (newPhi - phi_C);
-            sphi[q][ly][lx] = phi_C + d;
+            sphi[q][col][ly][ix] = phi_C + d;
             if (owned) {
 #pragma unroll
                 for (int jj = 0; jj < K; ++jj)
@@ -197,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
         }
     };
 
-    const int nph = 2 * (kq[0] > kq[NEQ - 1] ? kq[0] : kq[NEQ - 1]);
+    const int kmax = eqs[0].k > eqs[NEQ - 1].k ? eqs[0].k : eqs[NEQ - 1].k;
+    const int nph = 2 * kmax;
 #pragma unroll
     for (int p = 0; p < 2 * K; ++p) {
         if (p >= nph) break;
@@ -208,28 +263,74 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
             const int r = e / half, kk = e - r * half;
             const int gy = ys + r;
             const int gx = xs + 2 * kk + ((xs + gy + color) & 1);
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) relax(gx, gy, p);
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) relax(gx, gy, color, p);
         }
         __syncthreads();
     }
-    // write the tile (a stopped equation is copied through unchanged, keeping the ping-pong parity)
+
+    // ---- stop decision (launchRuns) from the prefetched and remaining partials ----
+    bool act[2] = {true, true};
+    {
+        double t[NEQ * K];
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                double s = 0.0;
+                if (launch > 0 && kk < eqs[q].kPrev) {
+                    const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
+#pragma unroll
+                    for (int f = 0; f < NPF; ++f) s += pf[q][kk][f];
+                    for (int bidx = tid + NPF * kBlock; bidx < nprev; bidx += kBlock) s += part[bidx];
+                }
+                t[q * K + kk] = s;
+            }
+        if (launch > 0) {
+            double tot[NEQ * K];
+            blockSum<NEQ * K>(t, tot);
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) {
+                double tq[K];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) tq[kk] = tot[q * K + kk];
+                act[q] = eqs[q].k > 0 && launchDecide<K>(P, blk, eqs[q].slot, launch, eqs[q].kPrev, prev[q], tq, P.tol[eqs[q].q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) act[q] = eqs[q].k > 0;
+        }
+    }
+
+    // ---- write the tile: computed values, or the input unchanged for a stopped equation (keeps
+    // the ping-pong parity) ----
     for (int e = tid; e < TX * TY; e += kBlock) {
         const int r = e / TX, xx = e - r * TX;
         const int gx = x0 + xx, gy = y0 + r;
         if (gx < nx && gy < ny) {
             const int g = gy * nx + gx;
+            const int ly = r + H, lx = xx + H;
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) out[q][g] = sphi[q][r + H][xx + H];
+            for (int q = 0; q < NEQ; ++q) {
+                const float val = act[q] ? sphi[q][(lx + ly) & 1][ly][lx >> 1] : in[q][g];
+                out[q][g] = val;
+                if (eqs[q].accum) P.pc[g] += val;       // the accumulation of k_pcorr_accumulate
+            }
         }
     }
+    // ---- this launch's sweep norms: one workgroup reduction for all equations and sweeps ----
+    {
+        double v[NEQ * K], tot[NEQ * K];
 #pragma unroll
-    for (int q = 0; q < NEQ; ++q)
+        for (int q = 0; q < NEQ; ++q)
 #pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (j < kq[q]) {
-                double a1[1] = {acc[j][q]};
-                storeBlockSums<1>(a1, sweepPart(P, eqs[q].slot, launch, j) + blk.flat);
-            }
+            for (int j = 0; j < K; ++j) v[q * K + j] = acc[j][q];
+        blockSum<NEQ * K>(v, tot);
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q)
+#pragma unroll
+            for (int j = 0; j < K; ++j)     // constant indices: no private array in scratch
+                if (tid == q * K + j && act[q] && j < eqs[q].k) sweepPart(P, eqs[q].slot, launch, j)[blk.flat] = tot[q * K + j];
+    }
 }
 
 template <int TX, int TY, int K, bool BATCH>
@@ -237,13 +338,14 @@ static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel
     const CaseParams &P = *L.P;
     const dim3 grid = L.batch ? dim3((unsigned)L.total[MAP_TILE])
                               : dim3((unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY)));
-    if (neq == 2) {
-        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 2, COEF_STORED, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+    if constexpr (K <= 2) {
+        if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     } else {
-        if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else if (kind == COEF_PCORR) k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 1, COEF_STORED, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        (void)neq;
+        (void)kind;
+        k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     }
 }
 
@@ -272,10 +374,12 @@ int default_tile(int nx, int ny) {
 
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block) {
     if (L.batch) {
-        if (block == 2) launchK<2, true>(L, kind, e0, e1, neq, launch);
+        if (block == 4 && kind == COEF_PCORR) launchK<4, true>(L, kind, e0, e1, neq, launch);
+        else if (block >= 2) launchK<2, true>(L, kind, e0, e1, neq, launch);
         else launchK<1, true>(L, kind, e0, e1, neq, launch);
     } else {
-        if (block == 2) launchK<2, false>(L, kind, e0, e1, neq, launch);
+        if (block == 4 && kind == COEF_PCORR) launchK<4, false>(L, kind, e0, e1, neq, launch);
+        else if (block >= 2) launchK<2, false>(L, kind, e0, e1, neq, launch);
         else launchK<1, false>(L, kind, e0, e1, neq, launch);
     }
 }

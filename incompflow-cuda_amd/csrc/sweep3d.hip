@@ -33,9 +33,10 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
     const int bx = blk.flat % gxc, byz = blk.flat / gxc;
     const int by = byz % gyc, bz = byz / gyc;
-    float *fld = slotField(P, e.slot), *alt = P.alt[e.slot];
-    const float *ein = (launch & 1) ? alt : fld;
-    float *eout = (launch & 1) ? fld : alt;
+    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
+    const bool odd = ((launch + e.flip) & 1) != 0;
+    const float *ein = odd ? alt : fld;
+    float *eout = odd ? fld : alt;
     const float *esrc = slotSource(P, e.slot), *coef = slotCoef(P, e.slot);
     const float erlx = P.relax[e.q];
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;

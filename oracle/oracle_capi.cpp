@@ -105,7 +105,7 @@ int run_stage(oracle::Simple<Real> *s, int stage, double *out, size_t n_out) {
             Real *fld = q == 0 ? s->u.data() : q == 1 ? s->v.data() : s->w.data();
             const Real *cf = q == 0 ? s->ucoef.data() : q == 1 ? s->vcoef.data() : s->wcoef.data();
             const Real *src = q == 0 ? s->bu.data() : q == 1 ? s->bv.data() : s->bw.data();
-            tmp[0] = s->iterate(fld, cf, src, s->nIter[q], s->relax[q], s->tol[q]);
+            tmp[0] = s->iterateMomentum(fld, cf, src, s->nIter[q], s->relax[q], s->tol[q]);
             tmp[1] = s->lastSweepSum;
             break;
         }

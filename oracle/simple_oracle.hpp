@@ -117,7 +117,8 @@ public:
         stopRule = c.stop_rule;
         tolRes = c.tol_residual;
         pRefine = c.p_refine;
-        sweepBlock = c.sweep_block > 1 ? c.sweep_block : 1;
+        // sweeps per launch on the device: sweep_block for p' (2D red-black), at most 2 for momentum
+        sweepBlock = (c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block > 1) ? c.sweep_block : 1;
         numOuterIter = c.num_outer_iter;
         // Solver.cu:13-104: uniform initial u,v,w,p,temp; everything else zero.
         u.assign(Nc, (Real)c.init[0]);
@@ -379,6 +380,12 @@ public:
     // With sweepBlock = 1 this is the reference loop; with sweepBlock = k the test still runs
     // after every sweep but a met test ends the call only after the current block of k sweeps.
     int iterate(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl) {
+        return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlock);
+    }
+    int iterateMomentum(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl) {
+        return iterateBlocked(fld, coef, src, n, rlx, tl, std::min(sweepBlock, 2));
+    }
+    int iterateBlocked(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl, int block) {
         double maxNorm = -1e20;
         int done = 0;
         bool stopPending = false;
@@ -391,7 +398,7 @@ public:
             maxNorm = std::max(norm, maxNorm);
             double relNorm = norm / (maxNorm + 1e-20);
             if (relNorm < (double)tl) stopPending = true;
-            if (stopPending && (it + 1) % sweepBlock == 0) break;
+            if (stopPending && (it + 1) % block == 0) break;
         }
         lastSweeps = done;
         return done;
@@ -630,9 +637,9 @@ public:
         assembleMomentum();                              // :167-175
         double res[3];
         momentumResidual(res);
-        rec.sweeps[0] = iterate(u.data(), ucoef.data(), bu.data(), nIter[0], relax[0], tol[0]);   // :178
-        rec.sweeps[1] = iterate(v.data(), vcoef.data(), bv.data(), nIter[1], relax[1], tol[1]);   // :180
-        if (dim == 3) rec.sweeps[2] = iterate(w.data(), wcoef.data(), bw.data(), nIter[2], relax[2], tol[2]);
+        rec.sweeps[0] = iterateMomentum(u.data(), ucoef.data(), bu.data(), nIter[0], relax[0], tol[0]);   // :178
+        rec.sweeps[1] = iterateMomentum(v.data(), vcoef.data(), bv.data(), nIter[1], relax[1], tol[1]);   // :180
+        if (dim == 3) rec.sweeps[2] = iterateMomentum(w.data(), wcoef.data(), bw.data(), nIter[2], relax[2], tol[2]);
         RhieChowInterpolate();                           // :186-187
         calcPresCorrLinkCoef();                          // :189
         double mass = calcPresCorrSrcTerm();             // :191

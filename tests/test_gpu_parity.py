@@ -235,17 +235,19 @@ def test_odd_and_unequal_sweep_counts():
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
+@pytest.mark.parametrize("block", [2, 4])
 @pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_32x4])
 @pytest.mark.parametrize("name", ["channel2d", "cavity2d"])
-def test_two_sweeps_per_launch(tile, name):
-    """sweep_block=2 (two red-black sweeps per launch over a 4-cell halo) vs the oracle's blocked
-    stop rule, with odd and unequal sweep counts and the p' refinement."""
+def test_two_sweeps_per_launch(tile, name, block):
+    """sweep_block=2 and 4 (two or four red-black sweeps per launch over a 4- or 8-cell halo; the
+    momentum equations keep 2) vs the oracle's blocked stop rule, with odd and unequal sweep counts
+    and the p' refinement."""
     cfg = CASES[name]()
-    cfg.sweep_block = 2
+    cfg.sweep_block = block
     cfg.tile = tile
     cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[3] = 5, 4, 7
     cfg.p_refine = 1
-    _log(f"[block2] case={name} tile={tile}")
+    _log(f"[block{block}] case={name} tile={tile}")
     g = Solver(cfg)
     o = Oracle(cfg)
     rg = g.solve(6)

@@ -450,17 +450,23 @@ def test_sweep_block_only_delays_the_stop():
     for f in range(4):
         base.tol[f] = 0.0
     runs = []
-    for blk in (1, 2):
+    for blk in (1, 2, 4):
         c = base.copy()
         c.sweep_block = blk
         o = Oracle(c)
         o.solve(4)
         runs.append([o.get(f) for f in (A.FIELD_U, A.FIELD_V, A.FIELD_P)])
-    for a, b in zip(*runs):
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     # from rest v has no source in the first iteration: zero increments stop the v smoother
     c1, c2 = cavity(12, 10), cavity(12, 10)
     c2.sweep_block = 2
     h1 = Oracle(c1).solve(1)["history"][0]
     h2 = Oracle(c2).solve(1)["history"][0]
     assert h1.sweeps[1] == 1 and h2.sweeps[1] == 2
+    # sweep_block 4 applies to p' only: the momentum equations keep blocks of 2
+    c4 = cavity(12, 10)
+    c4.sweep_block = 4
+    h4 = Oracle(c4).solve(1)["history"][0]
+    assert h4.sweeps[1] == 2
