@@ -1,4 +1,4 @@
-// sweep2d.hip - K (1 or 2) full red-black Gauss-Seidel sweeps per launch for 2D grids.
+// sweep2d.hip - K (1, 2 or 4) full red-black Gauss-Seidel sweeps per launch for 2D grids.
 //
 // A workgroup owns a TX x TY tile.  It stages the old field on the tile plus a 2K-cell halo in
 // LDS and runs the 2K half sweeps R1 B1 (R2 B2) on shrinking regions: half sweep p updates the
@@ -21,6 +21,8 @@
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
+
+#include <utility>
 
 namespace ifl {
 
@@ -333,19 +335,256 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     }
 }
 
+// ---------------- the p' system: a kernel of its own ----------------
+// Same launch contract as k_rb_tile2d with one equation of kind COEF_PCORR, organised so that
+// the relax loop is mostly arithmetic:
+//   * one flat LDS block holds four planes with a common colour-split layout [colour][ly][ix]
+//     over the field region: the field, the coefficients towards +x and +y, and the source
+//     (rows 1..HY-2 only), so every operand of a cell is a compile-time offset from one of two
+//     addresses (its own colour, the other colour);
+//   * in half sweep p (compile-time) a thread owns one column ix of its colour and marches down
+//     pairs of rows; the lx parity of a colour alternates row by row, so within a pair the E/W
+//     offsets are constants;
+//   * tiles whose ring lies inside the grid skip the per-cell grid test.
+template <int TX, int TY, int K, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, SweepSel e0, int launch) {
+    constexpr int H = 2 * K;
+    constexpr int R0 = 2 * K - 1;
+    constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
+    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;
+    constexpr int NST = (HX * HY + kBlock - 1) / kBlock;
+    constexpr int NSR = (SX * SY + kBlock - 1) / kBlock;
+    constexpr int NPF = 4;
+    constexpr int CO = HY * HX2;            // colour stride of the field and coefficient planes
+    constexpr int PL = 2 * CO;              // their plane stride
+    constexpr int CS = (HY - 2) * HX2;      // colour stride of the source plane (rows 1..HY-2)
+    constexpr int oPHI = 0, oEX = PL, oEY = 2 * PL, oSRC = 3 * PL;
+    static_assert(HX % 2 == 0, "colour split needs an even width");
+    __shared__ float lds[3 * PL + 2 * CS];
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_TILE, blk);
+    const int done = caseDone(P);
+    const int nx = P.nx, ny = P.ny;
+    const int tilesX = (nx + TX - 1) / TX;
+    const int by = blk.flat / tilesX;
+    const int bx = blk.flat - by * tilesX;
+    const int x0 = bx * TX, y0 = by * TY;
+    const SweepSel e = e0;
+    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
+    const bool oddL = ((launch + e.flip) & 1) != 0;
+    const float *in = oddL ? alt : fld;
+    float *out = oddL ? fld : alt;
+    const float *src = slotSource(P, e.slot);
+    const float rlx = P.relax[e.q];
+    const int tid = threadIdx.x;
+
+    // ---- global loads ----
+    float rphi[NST], rr[NST], rsrc[NSR];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int i = tid + m * kBlock;
+        const int ly = i / HX, lx = i - ly * HX;
+        const int gx = x0 - H + lx, gy = y0 - H + ly;
+        const bool ok = i < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const int g = gy * nx + gx;
+        rphi[m] = ok ? in[g] : 0.0f;
+        rr[m] = ok ? P.rinv[g] : 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < NSR; ++m) {
+        const int i = tid + m * kBlock;
+        const int ly = i / SX, lx = i - ly * SX;
+        const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
+        const bool ok = i < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        rsrc[m] = ok ? src[gy * nx + gx] : 0.0f;
+    }
+    if (done) return;
+
+    // ---- stage: field and 1/aC (in the +x plane for now), source at field coordinates ----
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int i = tid + m * kBlock;
+        if (i < HX * HY) {
+            const int ly = i / HX, lx = i - ly * HX;
+            const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
+            lds[oPHI + a] = rphi[m];
+            lds[oEX + a] = rr[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NSR; ++m) {
+        const int i = tid + m * kBlock;
+        if (i < SX * SY) {
+            const int sy = i / SX, sx = i - sy * SX;
+            const int lx = sx + 1, ly = sy + 1;
+            lds[oSRC + ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1)] = rsrc[m];
+        }
+    }
+    // partials of the previous launch (stop decision), in flight during the relax phases
+    const int nprev = blk.n;
+    const PrevState prev = loadPrev(P, e.slot, launch);
+    double pf[K][NPF];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+        const bool need = launch > 0 && kk < e.kPrev;
+        const double *part = sweepPart(P, e.slot, launch - 1, kk);
+#pragma unroll
+        for (int f = 0; f < NPF; ++f) {
+            const int bidx = tid + f * kBlock;
+            pf[kk][f] = (need && bidx < nprev) ? part[bidx] : 0.0;
+        }
+    }
+    __syncthreads();
+    // ---- p' coefficients towards +x and +y (calcPresCorrLinkCoefKernel, kernels.cuh:517-529) ----
+    const double Kx = -0.5 * P.rho * P.Ax * P.Ax;
+    const double Ky = -0.5 * P.rho * P.Ay * P.Ay;
+    {
+        float rex[NST], rey[NST];
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int i = tid + m * kBlock;
+            rex[m] = rey[m] = 0.0f;
+            if (i < HX * HY) {
+                const int ly = i / HX, lx = i - ly * HX;
+                const int c = (lx + ly) & 1;
+                const int gx = x0 - H + lx, gy = y0 - H + ly;
+                const bool inGrid = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                const float rC = lds[oEX + c * CO + ly * HX2 + (lx >> 1)];
+                if (inGrid && gx < nx - 1 && lx + 1 < HX)
+                    rex[m] = (float)(Kx * (rC + lds[oEX + (c ^ 1) * CO + ly * HX2 + ((lx + 1) >> 1)]));
+                if (inGrid && gy < ny - 1 && ly + 1 < HY)
+                    rey[m] = (float)(Ky * (rC + lds[oEX + (c ^ 1) * CO + (ly + 1) * HX2 + (lx >> 1)]));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int i = tid + m * kBlock;
+            if (i < HX * HY) {
+                const int ly = i / HX, lx = i - ly * HX;
+                const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
+                lds[oEX + a] = rex[m];
+                lds[oEY + a] = rey[m];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- the 2K half sweeps ----
+    double acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = 0.0;
+    const int nph = 2 * e.k;
+    auto phase = [&](auto pcst) {
+        constexpr int p = decltype(pcst)::value;
+        constexpr int col = p & 1, ring = 2 * K - 1 - p;
+        constexpr int lyS = H - ring, rows = TY + 2 * ring;
+        constexpr int lxS = H - ring, lxE = H + TX + ring;        // [lxS, lxE)
+        constexpr int ixLo = lxS >> 1, ixHi = (lxE - 1) >> 1;
+        constexpr int ncol = ixHi - ixLo + 1;
+        constexpr int G = kBlock / ncol;                          // row-pair groups
+        constexpr int npair = (rows + 1) / 2;
+        constexpr int par0 = (lyS + col) & 1;                     // lx parity of the colour in row lyS
+        static_assert(G >= 1, "tile too wide for one workgroup");
+        const bool interior = x0 - ring >= 0 && x0 + TX + ring <= nx && y0 - ring >= 0 && y0 + TY + ring <= ny;
+        const int ci = tid % ncol, g = tid / ncol;
+        if (g < G) {
+            const int ix = ixLo + ci;
+            for (int m = g; m < npair; m += G) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int ly = lyS + 2 * m + h;
+                    const int par = h == 0 ? par0 : 1 - par0;
+                    const int lx = 2 * ix + par;
+                    const int gx = x0 - H + lx, gy = y0 - H + ly;
+                    bool ok = (2 * m + h < rows) && lx >= lxS && lx < lxE;
+                    if (!interior) ok = ok && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                    if (ok) {
+                        const int own = col * CO + ly * HX2 + ix;
+                        const int oth = (1 - col) * CO + ly * HX2 + ix;
+                        const float aE = lds[oEX + own];
+                        const float aW = lds[oEX + oth + par - 1];
+                        const float aN = lds[oEY + own];
+                        const float aS = lds[oEY + oth - HX2];
+                        const float aC = -(aE + aW + aN + aS);
+                        const float phi_C = lds[oPHI + own];
+                        // source plane: own - col * (CO - CS) - HX2 (a constant offset)
+                        float newPhi = lds[oSRC + own - col * (CO - CS) - HX2] - aE * lds[oPHI + oth + par]
+                                       - aW * lds[oPHI + oth + par - 1]
+                                       - aN * lds[oPHI + oth + HX2] - aS * lds[oPHI + oth - HX2];
+                        newPhi /= aC;
+                        const float d = rlx * (newPhi - phi_C);
+                        lds[oPHI + own] = phi_C + d;
+                        if (lx >= H && lx < H + TX && ly >= H && ly < H + TY) acc[p >> 1] += (double)d * (double)d;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+    [&]<int... Ps>(std::integer_sequence<int, Ps...>) {
+        ((Ps < nph ? (phase(std::integral_constant<int, Ps>{}), 0) : 0), ...);
+    }(std::make_integer_sequence<int, 2 * K>{});
+
+    // ---- stop decision ----
+    bool act = true;
+    {
+        double t[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            double sacc = 0.0;
+            if (launch > 0 && kk < e.kPrev) {
+                const double *part = sweepPart(P, e.slot, launch - 1, kk);
+#pragma unroll
+                for (int f = 0; f < NPF; ++f) sacc += pf[kk][f];
+                for (int bidx = tid + NPF * kBlock; bidx < nprev; bidx += kBlock) sacc += part[bidx];
+            }
+            t[kk] = sacc;
+        }
+        if (launch > 0) {
+            double tot[K];
+            blockSum<K>(t, tot);
+            act = e.k > 0 && launchDecide<K>(P, blk, e.slot, launch, e.kPrev, prev, tot, P.tol[e.q]);
+        } else {
+            act = e.k > 0;
+        }
+    }
+    // ---- write the tile (or copy the input through for a stopped call) ----
+    for (int i = tid; i < TX * TY; i += kBlock) {
+        const int r = i / TX, xx = i - r * TX;
+        const int gx = x0 + xx, gy = y0 + r;
+        if (gx < nx && gy < ny) {
+            const int g = gy * nx + gx;
+            const int ly = r + H, lx = xx + H;
+            const float val = act ? lds[oPHI + ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1)] : in[g];
+            out[g] = val;
+            if (e.accum) P.pc[g] += val;
+        }
+    }
+    // ---- this launch's sweep norms ----
+    {
+        double tot[K];
+        blockSum<K>(acc, tot);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (tid == j && act && j < e.k) sweepPart(P, e.slot, launch, j)[blk.flat] = tot[j];
+    }
+}
+
 template <int TX, int TY, int K, bool BATCH>
 static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch) {
     const CaseParams &P = *L.P;
     const dim3 grid = L.batch ? dim3((unsigned)L.total[MAP_TILE])
                               : dim3((unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY)));
+    if (kind == COEF_PCORR) {
+        k_rb_pc2d<TX, TY, K, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, launch);
+        return;
+    }
     if constexpr (K <= 2) {
         if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else if (kind == COEF_MOMENTUM) k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     } else {
         (void)neq;
-        (void)kind;
-        k_rb_tile2d<TX, TY, K, 1, COEF_PCORR, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e0, e1, launch);
     }
 }
 
