@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=600)
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--sweep-block", type=int, default=2)
+    ap.add_argument("--sweep-block", type=int, default=4)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")

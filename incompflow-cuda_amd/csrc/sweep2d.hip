@@ -1,23 +1,21 @@
 // sweep2d.hip - K (1, 2 or 4) full red-black Gauss-Seidel sweeps per launch for 2D grids.
 //
 // A workgroup owns a TX x TY tile.  It stages the old field on the tile plus a 2K-cell halo in
-// LDS and runs the 2K half sweeps R1 B1 (R2 B2) on shrinking regions: half sweep p updates the
-// cells of its colour within 2K-1-p cells of the tile, which is exactly the set the later half
-// sweeps read.  Cells outside the tile are recomputed from the same old values their owning tile
-// uses, so they are bit-identical, and each launch equals the reference's red pass followed by
-// its black pass, K times (GaussSeidelIterateKernel, kernels.cuh:926-1005, in red-black order).
-// Reads come from one buffer and writes go to the other (ping-pong), so no workgroup reads a
-// value another workgroup has already updated.
+// LDS and runs the 2K half sweeps R1 B1 (R2 B2 ...) on shrinking regions: half sweep p updates
+// the cells of its colour within 2K-1-p cells of the tile, which is exactly the set the later
+// half sweeps read.  Cells outside the tile are recomputed from the same old values their owning
+// tile uses, so they are bit-identical, and each launch equals the reference's red pass followed
+// by its black pass, K times (GaussSeidelIterateKernel, kernels.cuh:926-1005, in red-black
+// order).  Reads come from one buffer and writes go to the other (ping-pong), so no workgroup
+// reads a value another workgroup has already updated.
 //
-// Coefficients are not read from stored planes in the hot loop:
-//   * momentum: recomputed from the four face velocities and the boundary rows (the same
-//     functions as the assembly kernel, ifl_stencil.hpp), one set for u and v, which are swept
-//     together in one launch since they share the matrix;
-//   * pressure correction: recomputed from 1/aC of the momentum matrix (calcPresCorrLinkCoef-
-//     Kernel, kernels.cuh:517-529).
-// A p' sweep moves 4 words per cell (field in/out, 1/aC, source) instead of the 8 of a stored
-// 5-point stencil; with K = 2 the field, 1/aC and source are read once for two sweeps.
-// Every global load of the tile is issued before the stop decision so their latency overlaps.
+//   * k_rb_tile2d: the momentum equations.  Coefficients are recomputed from the four face
+//     velocities and the boundary rows (the functions of the assembly kernel, ifl_stencil.hpp),
+//     one set for u and v, which are swept together in one launch since they share the matrix.
+//   * k_rb_pc2d: the p' system, with the coefficients towards +x and +y of the stored p' matrix
+//     (the coefficient towards -x of a cell is the +x one of its west neighbour, bit for bit).
+// Every global load of the tile is issued first; the stop decision, which sums the previous
+// launch's partials, is taken after the relax phases, with those loads in flight meanwhile.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
@@ -35,7 +33,8 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
     constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0, SX2 = SX / 2;   // cells ever updated (sources)
-    constexpr bool PC = KIND == COEF_PCORR, MOM = KIND == COEF_MOMENTUM;
+    constexpr bool MOM = KIND == COEF_MOMENTUM;
+    static_assert(MOM, "the p' system has its own kernel (k_rb_pc2d)");
     constexpr int UX = SX + 1, UY = SY;                        // uf faces of the updated cells
     constexpr int VX = SX, VY = SY + 1;                        // vf faces of the updated cells
     constexpr int NST = (HX * HY + kBlock - 1) / kBlock;
@@ -46,12 +45,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     static_assert(HX % 2 == 0 && SX % 2 == 0, "colour split needs even widths");
     __shared__ float sphi[NEQ][2][HY][HX2];
     __shared__ float ssrc[NEQ][2][SY][SX2];
-    // p': 1/aC while the coefficients are formed, then the coefficient towards +x (same storage),
-    // and the coefficient towards +y
-    __shared__ float sco[PC ? 2 : 1][PC ? 2 : 1][PC ? HY : 1][PC ? HX2 : 1];
-    auto &srv = sco[0];
-    auto &sex = sco[0];
-    auto &sey = sco[PC ? 1 : 0];
     __shared__ float suf[MOM ? UY : 1][MOM ? UX : 1];
     __shared__ float svf[MOM ? VY : 1][MOM ? VX : 1];
     Blk blk;
@@ -78,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     const int tid = threadIdx.x;
 
     // ---- every global load of the tile first ----
-    float rphi[NEQ][NST], rr[PC ? NST : 1], rsrc[NEQ][NSR];
+    float rphi[NEQ][NST], rsrc[NEQ][NSR];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int e = tid + m * kBlock;
@@ -88,7 +81,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
         const int g = gy * nx + gx;
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) rphi[q][m] = ok ? in[q][g] : 0.0f;
-        if constexpr (PC) rr[m] = ok ? P.rinv[g] : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
@@ -127,7 +119,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
             const int c = (lx + ly) & 1;
 #pragma unroll
             for (int q = 0; q < NEQ; ++q) sphi[q][c][ly][lx >> 1] = rphi[q][m];
-            if constexpr (PC) srv[c][ly][lx >> 1] = rr[m];
         }
     }
 #pragma unroll
@@ -175,41 +166,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
     if (done) return;                      // a finished case: nothing is written
     __syncthreads();
 
-    // ---- p' edge coefficients, once per launch (calcPresCorrLinkCoefKernel, kernels.cuh:517-529):
-    // a_E(c) = -0.5 rho Ax^2 (1/aC_c + 1/aC_E) in double, rounded; a_W(c) = a_E(c - 1) exactly
-    // (the float sum is commutative), likewise a_S(c) = a_N(c - nx) ----
-    const double Kx = -0.5 * P.rho * P.Ax * P.Ax;
-    const double Ky = -0.5 * P.rho * P.Ay * P.Ay;
-    if constexpr (PC) {
-        float rex[NST], rey[NST];
-#pragma unroll
-        for (int m = 0; m < NST; ++m) {
-            const int e = tid + m * kBlock;
-            rex[m] = rey[m] = 0.0f;
-            if (e < HX * HY) {
-                const int ly = e / HX, lx = e - ly * HX;
-                const int c = (lx + ly) & 1;
-                const int gx = x0 - H + lx, gy = y0 - H + ly;
-                const bool inGrid = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-                const float rC = srv[c][ly][lx >> 1];
-                if (inGrid && gx < nx - 1 && lx + 1 < HX) rex[m] = (float)(Kx * (rC + srv[c ^ 1][ly][(lx + 1) >> 1]));
-                if (inGrid && gy < ny - 1 && ly + 1 < HY) rey[m] = (float)(Ky * (rC + srv[c ^ 1][ly + 1][lx >> 1]));
-            }
-        }
-        __syncthreads();   // every 1/aC read before its storage takes the +x coefficients
-#pragma unroll
-        for (int m = 0; m < NST; ++m) {
-            const int e = tid + m * kBlock;
-            if (e < HX * HY) {
-                const int ly = e / HX, lx = e - ly * HX;
-                const int c = (lx + ly) & 1;
-                sex[c][ly][lx >> 1] = rex[m];
-                sey[c][ly][lx >> 1] = rey[m];
-            }
-        }
-        __syncthreads();
-    }
-
     double acc[K][2];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = 0.0;
@@ -220,18 +176,12 @@ __global__ __launch_bounds__(kBlock) void k_rb_tile2d(CaseParams P0, BatchMap M,
         const int ix = lx >> 1, ixE = (lx + 1) >> 1, ixW = (lx - 1) >> 1;
         const int oc = col ^ 1;
         float a[7];
-        if constexpr (MOM) {
+        {
             const int sx = gx - x0 + R0, sy = gy - y0 + R0;
             const float uw = suf[sy][sx], ue = suf[sy][sx + 1];
             const float vs = svf[sy][sx], vn = svf[sy + 1][sx];
             momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
             momBcRowsCoef<2>(P, gx, gy, 0, a);
-        } else {
-            a[sE] = sex[col][ly][ix];
-            a[sW] = sex[oc][ly][ixW];
-            a[sN] = sey[col][ly][ix];
-            a[sS] = sey[oc][ly - 1][ix];
-            a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
         }
         const bool owned = gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY;
         const int j = p >> 1;
@@ -379,7 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
     const int tid = threadIdx.x;
 
     // ---- global loads ----
-    float rphi[NST], rr[NST], rsrc[NSR];
+    // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
+    // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
+    const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
+    float rphi[NST], rex[NST], rey[NST], rsrc[NSR];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int i = tid + m * kBlock;
@@ -388,7 +341,8 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
         const bool ok = i < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         const int g = gy * nx + gx;
         rphi[m] = ok ? in[g] : 0.0f;
-        rr[m] = ok ? P.rinv[g] : 0.0f;
+        rex[m] = ok ? cfE[g] : 0.0f;
+        rey[m] = ok ? cfN[g] : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
@@ -400,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
     }
     if (done) return;
 
-    // ---- stage: field and 1/aC (in the +x plane for now), source at field coordinates ----
+    // ---- stage: field, coefficients, source at field coordinates ----
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int i = tid + m * kBlock;
@@ -408,7 +362,8 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
             const int ly = i / HX, lx = i - ly * HX;
             const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
             lds[oPHI + a] = rphi[m];
-            lds[oEX + a] = rr[m];
+            lds[oEX + a] = rex[m];
+            lds[oEY + a] = rey[m];
         }
     }
 #pragma unroll
@@ -435,41 +390,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
         }
     }
     __syncthreads();
-    // ---- p' coefficients towards +x and +y (calcPresCorrLinkCoefKernel, kernels.cuh:517-529) ----
-    const double Kx = -0.5 * P.rho * P.Ax * P.Ax;
-    const double Ky = -0.5 * P.rho * P.Ay * P.Ay;
-    {
-        float rex[NST], rey[NST];
-#pragma unroll
-        for (int m = 0; m < NST; ++m) {
-            const int i = tid + m * kBlock;
-            rex[m] = rey[m] = 0.0f;
-            if (i < HX * HY) {
-                const int ly = i / HX, lx = i - ly * HX;
-                const int c = (lx + ly) & 1;
-                const int gx = x0 - H + lx, gy = y0 - H + ly;
-                const bool inGrid = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-                const float rC = lds[oEX + c * CO + ly * HX2 + (lx >> 1)];
-                if (inGrid && gx < nx - 1 && lx + 1 < HX)
-                    rex[m] = (float)(Kx * (rC + lds[oEX + (c ^ 1) * CO + ly * HX2 + ((lx + 1) >> 1)]));
-                if (inGrid && gy < ny - 1 && ly + 1 < HY)
-                    rey[m] = (float)(Ky * (rC + lds[oEX + (c ^ 1) * CO + (ly + 1) * HX2 + (lx >> 1)]));
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < NST; ++m) {
-            const int i = tid + m * kBlock;
-            if (i < HX * HY) {
-                const int ly = i / HX, lx = i - ly * HX;
-                const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
-                lds[oEX + a] = rex[m];
-                lds[oEY + a] = rey[m];
-            }
-        }
-        __syncthreads();
-    }
-
     // ---- the 2K half sweeps ----
     double acc[K];
 #pragma unroll
@@ -486,7 +406,11 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
         constexpr int npair = (rows + 1) / 2;
         constexpr int par0 = (lyS + col) & 1;                     // lx parity of the colour in row lyS
         static_assert(G >= 1, "tile too wide for one workgroup");
-        const bool interior = x0 - ring >= 0 && x0 + TX + ring <= nx && y0 - ring >= 0 && y0 + TY + ring <= ny;
+        // Lanes at the ends of a column range hold a cell one ring beyond this half sweep's region
+        // and update it too: later half sweeps read only cells within their own ring plus one, and
+        // it is not written out, so its value is never used.  Cells outside the grid are read as
+        // neighbours (with coefficient 0) and must keep their staged 0: boundary tiles skip them.
+        const bool interior = x0 - ring - 1 >= 0 && x0 + TX + ring + 1 <= nx && y0 - ring - 1 >= 0 && y0 + TY + ring + 1 <= ny;
         const int ci = tid % ncol, g = tid / ncol;
         if (g < G) {
             const int ix = ixLo + ci;
@@ -497,9 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
                     const int par = h == 0 ? par0 : 1 - par0;
                     const int lx = 2 * ix + par;
                     const int gx = x0 - H + lx, gy = y0 - H + ly;
-                    bool ok = (2 * m + h < rows) && lx >= lxS && lx < lxE;
-                    if (!interior) ok = ok && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-                    if (ok) {
+                    if (interior || (gx >= 0 && gx < nx && gy >= 0 && gy < ny)) {
                         const int own = col * CO + ly * HX2 + ix;
                         const int oth = (1 - col) * CO + ly * HX2 + ix;
                         const float aE = lds[oEX + own];
@@ -515,7 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_rb_pc2d(CaseParams P0, BatchMap M, S
                         newPhi /= aC;
                         const float d = rlx * (newPhi - phi_C);
                         lds[oPHI + own] = phi_C + d;
-                        if (lx >= H && lx < H + TX && ly >= H && ly < H + TY) acc[p >> 1] += (double)d * (double)d;
+                        const bool owned = lx >= H && lx < H + TX && ly >= H && ly < H + TY;
+                        const double dd = owned ? (double)d : 0.0;
+                        acc[p >> 1] += dd * dd;
                     }
                 }
             }
