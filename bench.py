@@ -231,13 +231,23 @@ def main():
 
     # ---- roofline of the dominant kernel (one red-black sweep of the p' system) ----
     us, per = s.time_kernel(_abi.KERNEL_SWEEP, reps=args.kernel_reps)
-    kb = max(1, args.sweep_block)       # sweeps per launch of the fused kernel
+    kb = max(1, args.sweep_block) if cfg.dim == 2 else 1   # sweeps per launch of the p' kernel
     alg_bytes = kb * sweep_bytes_per_cell(cfg.dim) * cells / per   # one launch updates 1/per of the cells
     achieved = alg_bytes / (us * 1e-6) / 1e9
-    # the kernel's own compulsory traffic: 2D p' in/out, 1/aC, b' (coefficients rebuilt from 1/aC);
-    # 3D field in/out, source and the 7 stored coefficient planes
-    own_bytes = (16.0 if cfg.dim == 2 else 40.0) * cells * kb / per
+    # the kernel's own compulsory traffic per launch: 2D p' in/out, b', the +x and +y coefficient
+    # planes (the -x/-y ones are the neighbours'), once for all kb sweeps of the launch;
+    # 3D field in/out, source and the 7 stored coefficient planes, one sweep per launch
+    own_bytes = (20.0 * cells if cfg.dim == 2 else 40.0 * cells) / per
     own_gbs = own_bytes / (us * 1e-6) / 1e9
+    # HBM traffic of the same kernel on the same workload, from the committed PMC summary
+    # (FETCH_SIZE and WRITE_SIZE runs, scaled by a known-size copy): null on other workloads
+    traffic = None
+    traffic_source = None
+    tfile = os.path.join(ROOT, "profiles", "k_rb_pc2d_cavity1024_k4_traffic.json")
+    if cfg.dim == 2 and n == 1024 and kb == 4 and args.tile in (0, 1) and os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh)["traffic_bytes_per_launch"]
+        traffic_source = os.path.relpath(tfile, ROOT)
     other = {}
     for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
                       ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE)):
@@ -323,8 +333,8 @@ def main():
                    "cells": cells, "sweeps_per_step": sweeps / args.steps, "sweep_block": kb,
                    "tile": cfg.tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": ("k_rb_tile2d (fused red-black sweeps of p')" if cfg.dim == 2 else
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
+                     "kernel": ("k_rb_pc2d (fused red-black sweeps of p')" if cfg.dim == 2 else
                                 "k_rb_zmarch3d (one fused red-black sweep of p')"), "avg_launch_us": us,
                      "alg_bytes_per_launch": alg_bytes, "own_bytes_per_launch": own_bytes,
                      "own_gbs": own_gbs},

@@ -485,7 +485,7 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
         {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
         {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
         {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
-        {(void **)&P.spart, (size_t)ifl::kSlots * 4 * pstride * 8},
+        {(void **)&P.spart, (size_t)ifl::kSlots * 2 * ifl::kMaxBlock * pstride * 8},
         {(void **)&P.partMom, 3 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
         {(void **)&P.st, sizeof(ifl::CaseState)},
         {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},

@@ -240,8 +240,10 @@ __device__ __forceinline__ const float *slotCoef(const CaseParams &P, int slot) 
     return slot < 3 ? P.mcoef : P.pcoef;
 }
 
+// Sweep partials: [slot][launch parity][sweep in the launch, < kMaxBlock][partStride].
+constexpr int kMaxBlock = 4;    // most red-black sweeps per launch
 __device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int launch, int kk) {
-    return P.spart + (size_t)((slot * 2 + (launch & 1)) * 2 + kk) * P.partStride;
+    return P.spart + (size_t)((slot * 2 + (launch & 1)) * kMaxBlock + kk) * P.partStride;
 }
 
 // Whether launch L of the smoother call in `slot` runs.  Every workgroup evaluates, from their

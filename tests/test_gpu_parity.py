@@ -257,6 +257,24 @@ def test_two_sweeps_per_launch(tile, name, block):
     np.testing.assert_allclose(_rec_array(rg["history"]), _rec_array(ro["history"]), rtol=1e-11, atol=1e-300)
 
 
+@pytest.mark.parametrize("block", [2, 4])
+def test_sweep_blocks_on_interior_tiles_with_refinement(block):
+    """A grid with interior tiles (their ring lies inside the grid, the unchecked fast path),
+    20 p' sweeps (five launches of four: an odd count, started from the partner buffer) and the
+    refinement, whose correction is added to p' by its last launch; 12 outer iterations."""
+    cfg = CASES["cavity2d"]()
+    cfg.nx, cfg.ny = 256, 96
+    cfg.sweep_block = block
+    cfg.p_refine = 1
+    _log(f"[interior] block={block} grid=256x96")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(12)
+    ro = o.solve(12)
+    _compare_all(g, o, cfg, f"interior block {block}")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+
+
 @pytest.mark.parametrize("shape", [(70, 20, 40), (9, 17, 33)])
 def test_3d_zmarch_multiple_tiles_and_chunks(shape):
     """The 3D fused sweep with several x/y tiles and z chunks (chunk edges recompute red planes)."""
