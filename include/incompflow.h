@@ -256,6 +256,10 @@ typedef struct ifl_adjoint_opts {
     const float *weights[4];       /* IFL_OBJ_LINEAR: weights of u, v, w, p (cells each; NULL = 0) */
     int32_t warm_start;            /* 1: continue from the previous call's lambda */
     int32_t check_every;           /* iterations between host reads of the monitor (0: 64) */
+    int32_t tile;                  /* 2D transposed sweeps, tile and sweeps per launch: 0 = by grid
+                                      size; 1..6 = 64x16 K4, 32x16 K4, 64x8 K2, 32x8 K4, 64x16 K2,
+                                      64x8 K4 (every choice gives the same result, bit for bit) */
+    int32_t reserved;              /* 0 */
 } ifl_adjoint_opts;
 
 typedef struct ifl_adjoint_report {

@@ -8,6 +8,8 @@
 #include "ifl_adjoint.hpp"
 #include "ifl_stencil.hpp"
 
+#include <utility>
+
 namespace ifl {
 
 namespace {
@@ -140,6 +142,120 @@ __global__ __launch_bounds__(kBlock) void k_adj_sweep_t(CaseParams P, const floa
             out[c] = acc;
         }
     });
+}
+
+// ---------------- ns (<= K) transposed red-black sweeps per launch, 2D ----------------
+// The transposed half sweep is local: a cell of the half sweep's colour hands its S to the source
+// adjoint and keeps phb - rlx phb; every other cell subtracts, per neighbour, the neighbour's
+// coefficient towards it times the neighbour's S (k_adj_sweep_t).  A workgroup owns a TX x TY
+// tile and keeps the tile plus a 2K-cell halo in registers -- per cell phb, aC, the coefficients
+// of its four neighbours towards it and the source adjoint, for the two cells (one of each
+// colour) of each of its pairs -- and exchanges only S through LDS, one barrier per half sweep.
+// Half sweep h leaves the outer h rings of the staged region stale (their neighbours are not
+// staged), so after 2K half sweeps the tile is exact: each cell goes through the arithmetic of
+// k_adj_sweep_t, in the same order, and the result is bit-identical.
+template <int TX, int TY, int K>
+__global__ __launch_bounds__(kBlock) void k_adj_tile2d(CaseParams P, const float *in, float *out, float *bb,
+                                                       const float *coef, float rlx, int ns) {
+    constexpr int H = 2 * K;
+    constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
+    constexpr int NPAIR = HX2 * HY;
+    constexpr int NP = (NPAIR + kBlock - 1) / kBlock;   // pairs per thread
+    constexpr int WS = HX + 2;                          // LDS row: the staged row and a pad cell each side
+    constexpr int DUMMY = (HY + 2) * WS;                // S slot of the pairs past the region
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "even tile origins keep the global red-black colours");
+    __shared__ float lS[DUMMY + 1];
+    const int nx = P.nx, ny = P.ny, N = P.N;
+    const int tilesX = (nx + TX - 1) / TX;
+    const int by = (int)blockIdx.x / tilesX, bx = (int)blockIdx.x - by * tilesX;
+    const int ox = bx * TX - H, oy = by * TY - H;       // origin of the staged region
+    const int t = threadIdx.x;
+
+    // [m][c]: the colour-c cell of the thread's m-th pair
+    float ph[NP][2], ac[NP][2], kE[NP][2], kW[NP][2], kN[NP][2], kS[NP][2], sb[NP][2];
+    int li[NP][2];          // LDS slot of the cell's S
+    unsigned fl[NP][2];     // neighbours in the grid: 1 east, 2 west, 4 north, 8 south; 16 owned cell
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const int e = t + kBlock * m;
+        const int ly = e / HX2, px = e - ly * HX2;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            ph[m][c] = 0.0f;
+            ac[m][c] = 1.0f;
+            kE[m][c] = kW[m][c] = kN[m][c] = kS[m][c] = 0.0f;
+            sb[m][c] = 0.0f;
+            fl[m][c] = 0u;
+            li[m][c] = DUMMY;
+            if (NPAIR % kBlock == 0 || e < NPAIR) {
+                const int lx = 2 * px + (c ^ (ly & 1));   // colour (lx + ly) & 1 == c
+                li[m][c] = (ly + 1) * WS + lx + 1;
+                const int gx = ox + lx, gy = oy + ly;
+                if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+                    const int g = gx + nx * gy;
+                    ph[m][c] = in[g];
+                    ac[m][c] = coef[g];
+                    unsigned f = 0u;
+                    if (gx < nx - 1) { f |= 1u; kE[m][c] = coef[g + 1 + N * sW]; }
+                    if (gx > 0) { f |= 2u; kW[m][c] = coef[g - 1 + N * sE]; }
+                    if (gy < ny - 1) { f |= 4u; kN[m][c] = coef[g + nx + N * sS]; }
+                    if (gy > 0) { f |= 8u; kS[m][c] = coef[g - nx + N * sN]; }
+                    if (lx >= H && lx < H + TX && ly >= H && ly < H + TY) {
+                        f |= 16u;
+                        sb[m][c] = bb[g];
+                    }
+                    fl[m][c] = f;
+                }
+            }
+        }
+    }
+    // the pad ring is read only by stale cells; zero it so they stay finite
+    for (int r = t; r < 2 * WS + 2 * HY; r += kBlock) {
+        int s;
+        if (r < WS) s = r;
+        else if (r < 2 * WS) s = (HY + 1) * WS + (r - WS);
+        else if (r < 2 * WS + HY) s = (r - 2 * WS + 1) * WS;
+        else s = (r - 2 * WS - HY + 1) * WS + WS - 1;
+        lS[s] = 0.0f;
+    }
+
+    auto half = [&](auto colc) {
+        constexpr int col = decltype(colc)::value, oc = col ^ 1;
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const float x = ph[m][col];
+            const float S = rlx * x / ac[m][col];
+            lS[li[m][col]] = S;
+            sb[m][col] += S;
+            ph[m][col] = x - rlx * x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int l = li[m][oc];
+            const unsigned f = fl[m][oc];
+            float acc = ph[m][oc];
+            if (f & 1u) acc -= kE[m][oc] * lS[l + 1];
+            if (f & 2u) acc -= kW[m][oc] * lS[l - 1];
+            if (f & 4u) acc -= kN[m][oc] * lS[l + WS];
+            if (f & 8u) acc -= kS[m][oc] * lS[l - WS];
+            ph[m][oc] = acc;
+        }
+    };
+    for (int s = 0; s < ns; ++s) {
+        half(std::integral_constant<int, 1>{});
+        half(std::integral_constant<int, 0>{});
+    }
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (fl[m][c] & 16u) {
+                const int ly = li[m][c] / WS - 1, lx = li[m][c] - (ly + 1) * WS - 1;
+                const int g = (ox + lx) + nx * (oy + ly);
+                out[g] = ph[m][c];
+                bb[g] = sb[m][c];
+            }
 }
 
 // ---------------- T3 (+ divergence of T4): u*-bar, p-bar partial, aC-bar, theta-bar ----------------
@@ -396,6 +512,33 @@ void launch_adj_sweep_t(const Lc &L, const AdjBufs &A, const float *in, float *o
     (void)A;
     IFL_ADJ_LAUNCH(L, k_adj_sweep_t, in, out, bb, coef, rlx, col);
 }
+// 2D transposed sweeps: tile shapes and sweeps per launch
+namespace {
+template <int TX, int TY, int K>
+void adjTile(const Lc &L, const float *in, float *out, float *bb, const float *coef, float rlx, int ns) {
+    const CaseParams &P = *L.P;
+    const unsigned tiles = (unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY));
+    k_adj_tile2d<TX, TY, K><<<tiles, kBlock, 0, L.s>>>(P, in, out, bb, coef, rlx, ns);
+}
+}  // namespace
+
+int adj_tile_sweeps(int shape) {
+    static constexpr int k[kAdjShapes] = {4, 4, 2, 4, 2, 4};
+    return shape >= 0 && shape < kAdjShapes ? k[shape] : 0;
+}
+
+void launch_adj_tile2d(const Lc &L, int shape, const float *in, float *out, float *bb, const float *coef, float rlx,
+                       int ns) {
+    switch (shape) {
+    case 0: adjTile<64, 16, 4>(L, in, out, bb, coef, rlx, ns); break;
+    case 1: adjTile<32, 16, 4>(L, in, out, bb, coef, rlx, ns); break;
+    case 2: adjTile<64, 8, 2>(L, in, out, bb, coef, rlx, ns); break;
+    case 3: adjTile<32, 8, 4>(L, in, out, bb, coef, rlx, ns); break;
+    case 4: adjTile<64, 16, 2>(L, in, out, bb, coef, rlx, ns); break;
+    default: adjTile<64, 8, 4>(L, in, out, bb, coef, rlx, ns); break;
+    }
+}
+
 void launch_adj_rc(const Lc &L, const AdjBufs &A) { IFL_ADJ_LAUNCH(L, k_adj_rc, A); }
 void launch_adj_coef(const Lc &L, const AdjBufs &A) { IFL_ADJ_LAUNCH(L, k_adj_coef, A); }
 void launch_adj_face(const Lc &L, const AdjBufs &A) { IFL_ADJ_LAUNCH(L, k_adj_face, A); }

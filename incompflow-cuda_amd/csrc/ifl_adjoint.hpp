@@ -3,7 +3,8 @@
 // One adjoint iteration lambda <- G_x^T lambda + J_x^T transposes one SIMPLE outer iteration at
 // its fixed point (u* = smoothed velocity, p' = correction, reverse stage order):
 //   k_adj_pcorr      p'-bar from the corrections (and b'-bar := 0)
-//   k_adj_sweep_t    x 2 n_p   transposed red-black half sweeps of the p' system
+//   k_adj_sweep_t    x 2 n_p   transposed red-black half sweeps of the p' system (2D: k_adj_tile2d,
+//                              up to four sweeps per launch)
 //   k_adj_rc         u*-bar, the Rhie-Chow pressure terms, aC-bar, inlet theta-bar (divergence)
 //   k_adj_sweep_t    x 2 n_d   transposed half sweeps of each momentum component
 //   k_adj_coef       coefficient adjoints before the BC rows, new p-bar, mu-bar partials
@@ -45,6 +46,12 @@ struct AdjBufs {
 void launch_adj_pcorr(const Lc &L, const AdjBufs &A);
 void launch_adj_sweep_t(const Lc &L, const AdjBufs &A, const float *in, float *out, float *bb, const float *coef,
                         float rlx, int col);
+// 2D: ns (<= adj_tile_sweeps(shape)) transposed sweeps in one launch; shapes 0..5 are the tiles
+// 64x16, 32x16, 64x8, 32x8, 64x16, 64x8 with 4, 4, 2, 4, 2, 4 sweeps per launch
+constexpr int kAdjShapes = 6;
+int adj_tile_sweeps(int shape);
+void launch_adj_tile2d(const Lc &L, int shape, const float *in, float *out, float *bb, const float *coef, float rlx,
+                       int ns);
 void launch_adj_rc(const Lc &L, const AdjBufs &A);
 void launch_adj_coef(const Lc &L, const AdjBufs &A);
 void launch_adj_face(const Lc &L, const AdjBufs &A);

@@ -118,7 +118,8 @@ OBJ_PRESSURE_DROP, OBJ_LINEAR = 0, 1
 
 class AdjointOpts(C.Structure):
     _fields_ = [("objective", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double), ("grad_tol", C.c_double),
-                ("weights", C.POINTER(C.c_float) * 4), ("warm_start", C.c_int32), ("check_every", C.c_int32)]
+                ("weights", C.POINTER(C.c_float) * 4), ("warm_start", C.c_int32), ("check_every", C.c_int32),
+                ("tile", C.c_int32), ("reserved", C.c_int32)]
 
 
 class AdjointReport(C.Structure):

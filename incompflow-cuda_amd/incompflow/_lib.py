@@ -149,10 +149,11 @@ class Solver:
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))
 
     def adjoint(self, objective=_abi.OBJ_PRESSURE_DROP, face=_abi.WEST, weights=None, max_iter=0, tol=0.0,
-                warm_start=False, check_every=0, profile_gradient=True, grad_tol=0.0):
+                warm_start=False, check_every=0, profile_gradient=True, grad_tol=0.0, tile=0):
         """Gradient of the objective at the current (converged) state w.r.t. the normal-velocity
         profile of `face` (an inlet) and mu (ifl_adjoint).  weights: dict field id -> array
-        (OBJ_LINEAR).  tol < 0 runs exactly max_iter adjoint iterations."""
+        (OBJ_LINEAR).  tol < 0 runs exactly max_iter adjoint iterations.  tile: 2D transposed-sweep
+        tile (0 = by grid size, 1..6 as ifl_adjoint_opts.tile)."""
         o = _abi.AdjointOpts()
         lib().ifl_adjoint_opts_default(C.byref(o))
         o.objective = objective
@@ -164,6 +165,7 @@ class Solver:
         o.warm_start = 1 if warm_start else 0
         if check_every:
             o.check_every = check_every
+        o.tile = tile
         keep = []
         if weights:
             for q, fid in enumerate((_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W, _abi.FIELD_P)):

@@ -84,6 +84,48 @@ def test_adjoint_iterations_equal_the_oracle_bit_for_bit(name):
     s.close()
 
 
+_WIDE = {}
+
+
+def _wide_oracle():
+    """300x100 channel (interior tiles for every 2D tile shape), u with 7 sweeps and p' with 13
+    (launches with fewer sweeps than the tile's K, odd and even launch counts)."""
+    if not _WIDE:
+        c, prof = channel_case(300, 100, noise=1e-2)
+        c.stop_rule = _abi.STOP_NONE
+        c.n_iter[0] = 7
+        c.n_iter[3] = 13
+        o = Oracle(c)
+        o.set_face_profile(_abi.WEST, prof)
+        o.solve(30, history=False)
+        o.adj_setup(0)
+        o.adj_iterate(9)
+        _WIDE.update(cfg=c, prof=prof, lam=_split(c, o.adj_lambda()), grad=o.adj_gradient())
+    return _WIDE
+
+
+@pytest.mark.parametrize("tile", range(7))
+def test_adjoint_tiles_equal_the_oracle_on_interior_tiles(tile):
+    w = _wide_oracle()
+    c = w["cfg"]
+    _log(f"[adjoint] 300x100 channel, tile {tile}: forward 30 iterations, then 9 adjoint iterations")
+    with Solver(c) as s:
+        s.set_face_profile(_abi.WEST, w["prof"])
+        s.solve(30, history=False)
+        r = s.adjoint(_abi.OBJ_PRESSURE_DROP, _abi.WEST, max_iter=9, tol=-1.0, tile=tile)
+        assert r["iterations"] == 9
+        for f in LAMBDA_FIELDS:
+            if f in (_abi.FIELD_W, _abi.FIELD_WF):
+                continue
+            got = s.adjoint_lambda(f, shaped=False)
+            want = w["lam"][f].astype(np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (tile, f, np.abs(got - want).max())
+    g = w["grad"]
+    assert abs(r["dJ_dmu"] - g[-1]) <= 1e-9 * abs(g[-1]), (r["dJ_dmu"], g[-1])
+    _, _, poff = layout(c)
+    assert np.array_equal(r["dJ_dprofile"], g[poff[_abi.WEST]:poff[_abi.WEST + 1]])
+
+
 def test_gradient_matches_differences_of_converged_fp64_solves():
     # FP32 stalls at a scaled residual of ~4e-5 on this grid (reached by iteration ~3000): run
     # the forward past it, and the adjoint to its own FP32 floor
