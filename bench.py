@@ -108,6 +108,62 @@ def batch_leg(args, incompflow, _abi):
             "speedup": seq / tb, "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
 
 
+# algorithmic bytes per cell of one launch (FP32 words x 4): the compulsory reads and writes
+KERNEL_WORDS_3D = {"mom_assemble": 14, "rhie_chow": 8, "pcorr_assemble": 13, "update": 16}
+
+
+def cavity3d_leg(args, incompflow, _abi):
+    """3D lid-driven cavity 256^3 (Re 400): time per outer iteration and per kernel, with each
+    kernel's algorithmic GB/s; the FP32 oracle's time per iteration on the host cores beside it."""
+    n = args.c3_n
+    c = _abi.cavity3d_config(n=n)
+    c.p_refine = 0
+    c.stop_rule = _abi.STOP_NONE
+    cells = n ** 3
+    log(f"[bench] cavity3d {n}^3: {args.c3_steps} outer iterations")
+    s = incompflow.Solver(c)
+    s.solve(3, history=False)
+    device_sync()
+    t0 = time.perf_counter()
+    r = s.solve(args.c3_steps, history=True)
+    device_sync()
+    wall = time.perf_counter() - t0
+    sweeps = sum(sum(rec.sweeps) for rec in r["history"]) / args.c3_steps
+    ms = 1000.0 * wall / args.c3_steps
+    kern = {}
+    reps = 20
+    us, _ = s.time_kernel(_abi.KERNEL_SWEEP, reps=reps)
+    # p' sweep: field in and out, b', the +x/+y/+z coefficient planes (24 B/cell); the survey's
+    # byte model of a stored 7-point sweep is 40 B/cell
+    kern["pcorr_sweep"] = {"us": us, "own_gbs": 24.0 * cells / (us * 1e3), "model_gbs": 40.0 * cells / (us * 1e3)}
+    us, _ = s.time_kernel(_abi.KERNEL_MOM_SWEEP, reps=reps)
+    # u, v, w together: 3 fields in and out, 3 sources, the 7 coefficient planes (64 B/cell)
+    kern["momentum_sweep_uvw"] = {"us": us, "own_gbs": 64.0 * cells / (us * 1e3),
+                                  "model_gbs": 3 * 40.0 * cells / (us * 1e3)}
+    for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
+                      ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE)):
+        us, _ = s.time_kernel(kid, reps=reps)
+        kern[name] = {"us": us, "model_gbs": 4.0 * KERNEL_WORDS_3D[name] * cells / (us * 1e3)}
+    s.close()
+    log(f"[bench] cavity3d {ms:.3f} ms/outer iteration ({sweeps:.0f} sweeps), p' sweep "
+        f"{kern['pcorr_sweep']['us']:.1f} us, u+v+w sweep {kern['momentum_sweep_uvw']['us']:.1f} us")
+    out = {"workload": f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
+                       "FP64 reductions", "cells": cells, "ms_per_step": ms, "sweeps_per_step": sweeps,
+           "cell_updates_per_s": cells * sweeps / (ms * 1e-3), "kernels": kern, "cpu_ms_per_step": None}
+    if not args.no_cpu and args.c3_cpu_iters > 0:
+        from oracle import oracle as orc
+        o = orc.Oracle(c)
+        o.solve(1, history=False)
+        t0 = time.perf_counter()
+        o.solve(args.c3_cpu_iters, history=False)
+        out["cpu_ms_per_step"] = 1000.0 * (time.perf_counter() - t0) / args.c3_cpu_iters
+        out["cpu_threads"] = orc.get_threads()
+        out["gpu_vs_cpu_per_step"] = out["cpu_ms_per_step"] / ms
+        o.close()
+        log(f"[bench] cavity3d cpu oracle {out['cpu_ms_per_step']:.1f} ms/iteration")
+    return out
+
+
 def adjoint_leg(args, incompflow, _abi):
     import numpy as np
     c = _abi.adjoint_channel_config(1024, 256)
@@ -186,6 +242,10 @@ def main():
     ap.add_argument("--batch-max-iter", type=int, default=20000)
     ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="estimate")
     ap.add_argument("--batch-probe-iters", type=int, default=300)
+    ap.add_argument("--no-cavity3d", action="store_true")
+    ap.add_argument("--c3-n", type=int, default=256)
+    ap.add_argument("--c3-steps", type=int, default=20)
+    ap.add_argument("--c3-cpu-iters", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
     ap.add_argument("--adjoint-fd", action="store_true")
@@ -234,10 +294,9 @@ def main():
     kb = max(1, args.sweep_block) if cfg.dim == 2 else 1   # sweeps per launch of the p' kernel
     alg_bytes = kb * sweep_bytes_per_cell(cfg.dim) * cells / per   # one launch updates 1/per of the cells
     achieved = alg_bytes / (us * 1e-6) / 1e9
-    # the kernel's own compulsory traffic per launch: 2D p' in/out, b', the +x and +y coefficient
-    # planes (the -x/-y ones are the neighbours'), once for all kb sweeps of the launch;
-    # 3D field in/out, source and the 7 stored coefficient planes, one sweep per launch
-    own_bytes = (20.0 * cells if cfg.dim == 2 else 40.0 * cells) / per
+    # the kernel's own compulsory traffic per launch: p' in/out, b' and the coefficient planes
+    # towards +x, +y (+z) (those towards -x, -y, -z are the neighbours'), once for all kb sweeps
+    own_bytes = (20.0 * cells if cfg.dim == 2 else 24.0 * cells) / per
     own_gbs = own_bytes / (us * 1e-6) / 1e9
     # HBM traffic of the same kernel on the same workload, from the committed PMC summary
     # (FETCH_SIZE and WRITE_SIZE runs, scaled by a known-size copy): null on other workloads
@@ -250,7 +309,8 @@ def main():
         traffic_source = os.path.relpath(tfile, ROOT)
     other = {}
     for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
-                      ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE)):
+                      ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE),
+                      ("momentum_sweep", _abi.KERNEL_MOM_SWEEP)):
         other[name] = round(s.time_kernel(kid, reps=max(10, args.kernel_reps // 4))[0], 3)
     s.close()
     log(f"[bench] sweep launch {us:.3f} us -> {achieved:.1f} GB/s algorithmic")
@@ -311,6 +371,9 @@ def main():
     adjoint = None
     if not args.no_adjoint and cfg.dim == 2:
         adjoint = adjoint_leg(args, incompflow, _abi)
+    cavity3d = None
+    if not args.no_cavity3d and cfg.dim == 2:
+        cavity3d = cavity3d_leg(args, incompflow, _abi)
 
     out = {
         "metric": METRIC,
@@ -347,6 +410,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "adjoint": adjoint,
+        "cavity3d": cavity3d,
     }
     if cpu:
         out["gpu_vs_cpu_per_step"] = cpu["ms_per_step"] / ms_per_step

@@ -201,7 +201,9 @@ enum {
     IFL_KERNEL_RHIE_CHOW = 2,
     IFL_KERNEL_PCORR_ASSEMBLE = 3,
     IFL_KERNEL_UPDATE = 4,
-    IFL_KERNEL_COUNT = 5
+    IFL_KERNEL_MOM_SWEEP = 5,     /* one launch of the momentum smoother (u, v and, in 3D, w swept
+                                     together on their shared matrix) */
+    IFL_KERNEL_COUNT = 6
 };
 
 /* Measurement hook (no reference counterpart): launch one kernel `reps` times on the handle's

@@ -182,12 +182,12 @@ struct SmoothMode {
 void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMode &md = SmoothMode{}) {
     const ifl_config &c = *pl.cfg;
     const int K = sweepBlockOf(c, pl.dim, slots[0]);
-    const bool fused = pl.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS;
+    const bool fused = c.smoother == IFL_SMOOTHER_RBGS;      // the equations of slots[] in one launch
     if (!fused && neq > 1) {
         for (int k = 0; k < neq; ++k) enqueueSmoothers(pl, &slots[k], 1, md);
         return;
     }
-    int n[2], nL[2], flip[2], nLmax = 0;
+    int n[3], nL[3], flip[3], nLmax = 0;
     for (int k = 0; k < neq; ++k) {
         const int q = slots[k] < 4 ? slots[k] : 3;
         n[k] = c.n_iter[q];
@@ -196,7 +196,7 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
         nLmax = std::max(nLmax, nL[k]);
     }
     for (int L = 0; L < nLmax; ++L) {
-        ifl::SweepSel se[2];
+        ifl::SweepSel se[3];
         int m = 0;
         for (int k = 0; k < neq; ++k) {
             const int kk = std::min(K, n[k] - L * K);
@@ -210,12 +210,10 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
         if (m == 1) se[1] = se[0];
         if (fused) {
             const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR;
-            ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
+            if (pl.dim == 2) ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
+            else ifl::launch_rb_sweep3d(pl.L, kind, se, m, L);
         } else {
-            for (int k = 0; k < m; ++k) {
-                if (c.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(pl.L, se[k], L);
-                else ifl::launch_rb_sweep3d(pl.L, se[k], L);
-            }
+            for (int k = 0; k < m; ++k) ifl::launch_sweep_jacobi(pl.L, se[k], L);
         }
     }
     for (int k = 0; k < neq; ++k) {
@@ -243,9 +241,9 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
 }
 
 void enqueueMomentumSmoothers(const Plan &pl, const SmoothMode &md = SmoothMode{}) {
-    if (pl.dim == 2 && pl.cfg->smoother == IFL_SMOOTHER_RBGS) {      // u and v share the matrix
-        const int slots[2] = {0, 1};
-        enqueueSmoothers(pl, slots, 2, md);
+    if (pl.cfg->smoother == IFL_SMOOTHER_RBGS) {      // u, v (and w) share the matrix: one sweep kernel
+        const int slots[3] = {0, 1, 2};
+        enqueueSmoothers(pl, slots, pl.dim, md);
         return;
     }
     for (int q = 0; q < pl.dim; ++q) enqueueSmoothers(pl, &q, 1, md);   // :178, :180, :183
@@ -737,7 +735,17 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
                 const ifl::SweepSel e{3, 3, K, 0};
                 if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(L, e, 0);
                 else if (pl.dim == 2) ifl::launch_rb_sweep2d(L, ifl::COEF_PCORR, e, e, 1, 0, K);
-                else ifl::launch_rb_sweep3d(L, e, 0);
+                else ifl::launch_rb_sweep3d(L, ifl::COEF_PCORR, &e, 1, 0);
+                break;
+            }
+            case IFL_KERNEL_MOM_SWEEP: {
+                // one launch of the momentum smoother as the outer iteration runs it
+                ifl::SweepSel se[3];
+                const int K = sweepBlockOf(h->cfg, pl.dim, 0);
+                for (int q = 0; q < 3; ++q) se[q] = ifl::SweepSel{q, q, K, 0};
+                if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(L, se[0], 0);
+                else if (pl.dim == 2) ifl::launch_rb_sweep2d(L, ifl::COEF_MOMENTUM, se[0], se[1], 2, 0, K);
+                else ifl::launch_rb_sweep3d(L, ifl::COEF_MOMENTUM, se, 3, 0);
                 break;
             }
             case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(L); break;

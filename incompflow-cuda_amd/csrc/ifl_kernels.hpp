@@ -57,6 +57,7 @@ void launch_copy_back(const Lc &L, int slot, int dir);   // dir 0: partner -> fi
 // 2D fused red-black sweeps (K = block sweeps per launch), one or two equations sharing a matrix
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block);
 // 3D z-marching red-black sweep
-void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch);
+// 3D red-black: neq (1..3) momentum equations together, or the p' system (kind COEF_PCORR, neq 1)
+void launch_rb_sweep3d(const Lc &L, int kind, const SweepSel *e, int neq, int launch);
 
 }  // namespace ifl

@@ -1,16 +1,23 @@
 // sweep3d.hip - one full red-black Gauss-Seidel sweep per launch for 3D grids.
 //
 // A workgroup owns a TX x TY column of the grid over ZC planes [z0, z1).  It marches up the
-// column with a rolling window of four planes of the old field in LDS (2-cell halo in x and y)
-// and at step t relaxes the red cells of plane t+1 (on the tile plus a 1-cell ring), then the
-// black cells of plane t (on the tile), then writes plane t:
+// column with a rolling window of four planes of the field in LDS (2-cell halo in x and y) and
+// at step t relaxes the red cells of plane t+1 (on the tile plus a 1-cell ring), then the black
+// cells of plane t (on the tile), then writes plane t:
 //   red(t+1) reads black values of planes t..t+2, which are still old;
 //   black(t) reads red values of planes t-1..t+1, which are already new.
 // The red planes z0-1 and z1 and the ring cells are recomputed from the same old values their
 // owners use, so the launch equals the reference's red pass followed by its black pass bit for
 // bit (GaussSeidelIterateKernel, kernels.cuh:926-1005, in red-black order, 7-point stencil).
-// Reads come from one buffer and writes go to the other (ping-pong).  Coefficients are the
-// stored (1+2dim) planes of the assembled matrix.
+// Reads come from one buffer and writes go to the other (ping-pong).
+//
+// The momentum equations (u, v, w: one matrix, the seven stored planes) are swept together,
+// NEQ of them per launch, so each coefficient is read once for all of them.  The p' system
+// reads only its +x, +y and +z planes: its matrix is symmetric bit for bit (the coefficient of
+// a cell towards -x is the +x coefficient of its west neighbour, kernels.cuh:517-529 computes
+// both from the same commutative sum) and its diagonal is -(E+W+N+S) - (T+B) as assembled.
+// The coefficients and sources of the cells a step relaxes are loaded one step ahead, into
+// registers, so their latency overlaps the previous step.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
@@ -19,127 +26,242 @@ namespace ifl {
 
 constexpr int Z_TX = 64, Z_TY = 8, Z_ZC = 16;
 
-template <int TX, int TY, int ZC, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e, int launch) {
+template <int TX, int TY, int ZC, int NEQ, bool PC, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1,
+                                                        SweepSel e2, int launch) {
     constexpr int HX = TX + 4, HY = TY + 4, PL = HX * HY;
     constexpr int NPL = (PL + kBlock - 1) / kBlock;
     constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;       // red cells of tile + ring per plane
     constexpr int BW = TX / 2, NB = TY * BW;                 // black cells of the tile per plane
-    __shared__ float win[4][HY][HX];                         // planes z mod 4
+    constexpr int NRT = (NR + kBlock - 1) / kBlock, NBT = (NB + kBlock - 1) / kBlock;
+    constexpr int NA = PC ? 6 : 7;      // p': E, W, N, S, T, B (C from them); momentum: C, E, W, N, S, T, B
+    static_assert(!PC || NEQ == 1, "one p' equation per launch");
+    __shared__ float win[NEQ][4][HY][HX];                    // plane z in slot z & 3
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH, blk);
     if (caseDone(P)) return;
-    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N, nxy = nx * ny;
     const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
     const int bx = blk.flat % gxc, byz = blk.flat / gxc;
     const int by = byz % gyc, bz = byz / gyc;
-    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
-    const bool odd = ((launch + e.flip) & 1) != 0;
-    const float *ein = odd ? alt : fld;
-    float *eout = odd ? fld : alt;
-    const float *esrc = slotSource(P, e.slot), *coef = slotCoef(P, e.slot);
-    const float erlx = P.relax[e.q];
+    const SweepSel eqs[3] = {e0, e1, e2};
+    const float *ein[NEQ], *esrc[NEQ];
+    float *eout[NEQ];
+    float erlx[NEQ];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) {
+        float *fld = slotField(P, eqs[q].slot), *alt = slotAlt(P, eqs[q].slot);
+        const bool odd = ((launch + eqs[q].flip) & 1) != 0;
+        ein[q] = odd ? alt : fld;
+        eout[q] = odd ? fld : alt;
+        esrc[q] = slotSource(P, eqs[q].slot);
+        erlx[q] = P.relax[eqs[q].q];
+    }
+    const float *coef = PC ? P.pcoef : P.mcoef;
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
     const int z1 = min(z0 + ZC, nz);
     const int tid = threadIdx.x;
 
-
-    // issue the loads of the first four planes, then decide whether this sweep runs
-    float reg[4][NPL];
-    auto loadPlane = [&](int z, float (&r)[NPL]) {
+    auto wslot = [](int z) { return z & 3; };
+    auto loadPlane = [&](int z, float (&r)[NEQ][NPL]) {
 #pragma unroll
         for (int m = 0; m < NPL; ++m) {
             const int q = tid + m * kBlock;
             const int ly = q / HX, lx = q - ly * HX;
             const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
             const bool in = q < PL && z >= 0 && z < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-            r[m] = in ? ein[gx + nx * (gy + ny * z)] : 0.0f;
+            const int g = gx + nx * (gy + ny * z);
+#pragma unroll
+            for (int e = 0; e < NEQ; ++e) r[e][m] = in ? ein[e][g] : 0.0f;
         }
     };
-    auto storePlane = [&](int z, const float (&r)[NPL]) {
-        float *w = &win[z & 3][0][0];
+    auto storePlane = [&](int z, const float (&r)[NEQ][NPL]) {
 #pragma unroll
-        for (int m = 0; m < NPL; ++m) {
-            const int q = tid + m * kBlock;
-            if (q < PL) w[q] = r[m];
+        for (int e = 0; e < NEQ; ++e) {
+            float *w = &win[e][wslot(z)][0][0];
+#pragma unroll
+            for (int m = 0; m < NPL; ++m) {
+                const int q = tid + m * kBlock;
+                if (q < PL) w[q] = r[e][m];
+            }
         }
     };
+
+    // the operands of one relaxed cell: its coefficients and the equations' sources
+    struct Ops {
+        float a[NA];
+        float b[NEQ];
+    };
+    // cell m of this thread in the red (ring) or black (tile) set of plane z; false if off the grid
+    auto redCell = [&](int z, int m, int &gx, int &gy) {
+        const int q = tid + m * kBlock;
+        const int r = q / RW, kk = q - r * RW;
+        gy = y0 - 1 + r;
+        gx = x0 - 1 + 2 * kk + ((x0 - 1 + gy + z) & 1);
+        return q < NR && z >= 0 && z < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    };
+    auto blackCell = [&](int z, int m, int &gx, int &gy) {
+        const int q = tid + m * kBlock;
+        const int r = q / BW, kk = q - r * BW;
+        gy = y0 + r;
+        gx = x0 + 2 * kk + ((x0 + gy + z + 1) & 1);
+        return q < NB && z < nz && gx < nx && gy < ny;
+    };
+    auto loadOps = [&](bool ok, int gx, int gy, int z, Ops &o) {
+        const int c = gx + nx * (gy + ny * z);
+        if constexpr (PC) {
+            o.a[0] = ok ? coef[c + N * sE] : 0.0f;
+            o.a[1] = ok && gx > 0 ? coef[c - 1 + N * sE] : 0.0f;
+            o.a[2] = ok ? coef[c + N * sN] : 0.0f;
+            o.a[3] = ok && gy > 0 ? coef[c - nx + N * sN] : 0.0f;
+            o.a[4] = ok ? coef[c + N * sT] : 0.0f;
+            o.a[5] = ok && z > 0 ? coef[c - nxy + N * sT] : 0.0f;
+        } else {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) loadPlane(z0 - 2 + s, reg[s]);
-    const bool run = launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q]);
+            for (int s = 0; s < 7; ++s) o.a[s] = ok ? coef[c + N * s] : 0.0f;
+        }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) storePlane(z0 - 2 + s, reg[s]);
+        for (int e = 0; e < NEQ; ++e) o.b[e] = ok ? esrc[e][c] : 0.0f;
+    };
+    auto loadRed = [&](int z, Ops (&o)[NRT]) {
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) {
+            int gx, gy;
+            const bool ok = redCell(z, m, gx, gy);
+            loadOps(ok, gx, gy, z, o[m]);
+        }
+    };
+    auto loadBlack = [&](int z, Ops (&o)[NBT]) {
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) {
+            int gx, gy;
+            const bool ok = blackCell(z, m, gx, gy);
+            loadOps(ok, gx, gy, z, o[m]);
+        }
+    };
+
+    // issue the loads of the first four planes and the first operands, then decide which
+    // equations sweep
+    {
+        float reg[4][NEQ][NPL];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) loadPlane(z0 - 2 + s, reg[s]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) storePlane(z0 - 2 + s, reg[s]);
+    }
+    bool run[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; ++e) run[e] = launchRuns(P, blk, eqs[e].slot, launch, eqs[e].kPrev, blk.n, P.tol[eqs[e].q]);
     __syncthreads();
 
-    double acc = 0.0;
-    auto relax = [&](int gx, int gy, int z, bool owned) {
-        const int c = gx + nx * (gy + ny * z);
+    double acc[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; ++e) acc[e] = 0.0;
+    auto relax = [&](int gx, int gy, int z, bool owned, const Ops &o) {
         const int lx = gx - x0 + 2, ly = gy - y0 + 2;
-        float(*wc)[HX] = win[z & 3];
-        const float phi_C = wc[ly][lx];
-        const float phi_E = wc[ly][lx + 1], phi_W = wc[ly][lx - 1];
-        const float phi_N = wc[ly + 1][lx], phi_S = wc[ly - 1][lx];
-        const float phi_T = (z != nz - 1) ? win[(z + 1) & 3][ly][lx] : 0.0f;
-        const float phi_B = (z != 0) ? win[(z - 1) & 3][ly][lx] : 0.0f;
-        float newPhi = esrc[c] - coef[c + N * sE] * phi_E - coef[c + N * sW] * phi_W - coef[c + N * sN] * phi_N
-                       - coef[c + N * sS] * phi_S;
-        newPhi -= coef[c + N * sT] * phi_T + coef[c + N * sB] * phi_B;
-        newPhi /= coef[c];
-        const float d = erlx * (newPhi - phi_C);
-        wc[ly][lx] = phi_C + d;
-        if (owned) acc += (double)d * (double)d;
-    };
-    auto redPlane = [&](int z) {
-        if (z < 0 || z >= nz) return;
-        const bool ownPlane = z >= z0 && z < z1;
-        for (int q = tid; q < NR; q += kBlock) {
-            const int r = q / RW, kk = q - r * RW;
-            const int gy = y0 - 1 + r;
-            const int gx = x0 - 1 + 2 * kk + ((x0 - 1 + gy + z) & 1);
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                relax(gx, gy, z, ownPlane && gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY);
+        float aC, aE, aW, aN, aS, aT, aB;
+        if constexpr (PC) {
+            aE = o.a[0];
+            aW = o.a[1];
+            aN = o.a[2];
+            aS = o.a[3];
+            aT = o.a[4];
+            aB = o.a[5];
+            aC = -(aE + aW + aN + aS);
+            aC += -(aT + aB);
+        } else {
+            aC = o.a[0];
+            aE = o.a[1];
+            aW = o.a[2];
+            aN = o.a[3];
+            aS = o.a[4];
+            aT = o.a[5];
+            aB = o.a[6];
+        }
+#pragma unroll
+        for (int e = 0; e < NEQ; ++e) {
+            if (!run[e]) continue;
+            float(*wc)[HX] = win[e][wslot(z)];
+            const float phi_C = wc[ly][lx];
+            const float phi_E = wc[ly][lx + 1], phi_W = wc[ly][lx - 1];
+            const float phi_N = wc[ly + 1][lx], phi_S = wc[ly - 1][lx];
+            const float phi_T = (z != nz - 1) ? win[e][wslot(z + 1)][ly][lx] : 0.0f;
+            const float phi_B = (z != 0) ? win[e][wslot(z - 1)][ly][lx] : 0.0f;
+            float newPhi = o.b[e] - aE * phi_E - aW * phi_W - aN * phi_N - aS * phi_S;
+            newPhi -= aT * phi_T + aB * phi_B;
+            newPhi /= aC;
+            const float d = erlx[e] * (newPhi - phi_C);
+            wc[ly][lx] = phi_C + d;
+            if (owned) acc[e] += (double)d * (double)d;
         }
     };
-    auto blackPlane = [&](int z) {
-        for (int q = tid; q < NB; q += kBlock) {
-            const int r = q / BW, kk = q - r * BW;
-            const int gy = y0 + r;
-            const int gx = x0 + 2 * kk + ((x0 + gy + z + 1) & 1);
-            if (gx < nx && gy < ny) relax(gx, gy, z, true);
+    auto redPlane = [&](int z, const Ops (&o)[NRT]) {
+        const bool ownPlane = z >= z0 && z < z1;
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) {
+            int gx, gy;
+            if (redCell(z, m, gx, gy))
+                relax(gx, gy, z, ownPlane && gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY, o[m]);
+        }
+    };
+    auto blackPlane = [&](int z, const Ops (&o)[NBT]) {
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) {
+            int gx, gy;
+            if (blackCell(z, m, gx, gy)) relax(gx, gy, z, true, o[m]);
         }
     };
 
-    if (run) {
-        redPlane(z0 - 1);
-        redPlane(z0);
-        __syncthreads();
+    Ops red[NRT], blk0[NBT];
+    {
+        Ops r0[NRT];
+        loadRed(z0 - 1, r0);
+        loadRed(z0, red);
+        redPlane(z0 - 1, r0);
+        redPlane(z0, red);
     }
-    for (int t = z0; t < z1; ++t) {
-        float nxt[NPL];
-        loadPlane(t + 2, nxt);                 // old plane t+2 (needed by red(t+1))
-        if (run) {
-            // plane t+2 replaces plane t-2 (slot (t+2)&3), which no step reads any more
-            storePlane(t + 2, nxt);
-            __syncthreads();
-            redPlane(t + 1);
-            __syncthreads();
-            blackPlane(t);
-            __syncthreads();
-        } else {
-            storePlane(t + 2, nxt);
-            __syncthreads();
-        }
-        // write plane t of the tile
+    loadRed(z0 + 1, red);
+    loadBlack(z0, blk0);
+    // write plane z of the tile
+    auto writePlane = [&](int z) {
         for (int q = tid; q < TX * TY; q += kBlock) {
             const int r = q / TX, xx = q - r * TX;
             const int gx = x0 + xx, gy = y0 + r;
-            if (gx < nx && gy < ny) eout[gx + nx * (gy + ny * t)] = win[t & 3][r + 2][xx + 2];
+            if (gx < nx && gy < ny) {
+#pragma unroll
+                for (int e = 0; e < NEQ; ++e) eout[e][gx + nx * (gy + ny * z)] = win[e][wslot(z)][r + 2][xx + 2];
+            }
         }
+    };
+    __syncthreads();
+    // Step t: store old plane t+2 over plane t-2 (read by no later phase), barrier, red(t+1)
+    // (planes t..t+2), barrier, black(t) (planes t-1..t+1), barrier, write plane t.  The store of
+    // step t+1 (over plane t-1) follows black(t) by a barrier.
+    for (int t = z0; t < z1; ++t) {
+        // prefetch: old plane t+2 (needed by red(t+1)) and the operands of the next step
+        float nxt[NEQ][NPL];
+        loadPlane(t + 2, nxt);
+        Ops redN[NRT], blkN[NBT];
+        loadRed(t + 2, redN);
+        loadBlack(t + 1, blkN);
+        storePlane(t + 2, nxt);
         __syncthreads();
+        redPlane(t + 1, red);
+        __syncthreads();
+        blackPlane(t, blk0);
+        __syncthreads();
+        writePlane(t);
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) red[m] = redN[m];
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) blk0[m] = blkN[m];
     }
-    if (run) {
-        double a1[1] = {acc};
-        storeBlockSums<1>(a1, sweepPart(P, e.slot, launch, 0) + blk.flat);
+    {
+        double tot[NEQ];
+        blockSum<NEQ>(acc, tot);
+#pragma unroll
+        for (int e = 0; e < NEQ; ++e)
+            if (tid == e && run[e]) sweepPart(P, eqs[e].slot, launch, 0)[blk.flat] = tot[e];
     }
 }
 
@@ -149,15 +271,25 @@ int zmarch_blocks(const CaseParams &P) {
 
 int zmarch_dimx(const CaseParams &P) { return (P.nx + Z_TX - 1) / Z_TX; }
 
-void launch_rb_sweep3d(const Lc &L, const SweepSel &e, int launch) {
+template <int NEQ, bool PC, bool BATCH>
+static void launchZ(const Lc &L, const SweepSel *e, int launch) {
     const CaseParams &P = *L.P;
-    if (L.batch) {
-        k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, true><<<dim3((unsigned)L.total[MAP_ZMARCH]), kBlock, 0, L.s>>>(P, L.M, e, launch);
-        return;
-    }
-    dim3 grid((unsigned)((P.nx + Z_TX - 1) / Z_TX), (unsigned)((P.ny + Z_TY - 1) / Z_TY),
-              (unsigned)((P.nz + Z_ZC - 1) / Z_ZC));
-    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, false><<<grid, kBlock, 0, L.s>>>(P, L.M, e, launch);
+    const dim3 grid = BATCH ? dim3((unsigned)L.total[MAP_ZMARCH]) : dim3((unsigned)zmarch_blocks(P));
+    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
+                                                                          e[NEQ > 2 ? 2 : 0], launch);
+}
+
+template <bool BATCH>
+static void launchZ(const Lc &L, int kind, const SweepSel *e, int neq, int launch) {
+    if (kind == COEF_PCORR) launchZ<1, true, BATCH>(L, e, launch);
+    else if (neq == 3) launchZ<3, false, BATCH>(L, e, launch);
+    else if (neq == 2) launchZ<2, false, BATCH>(L, e, launch);
+    else launchZ<1, false, BATCH>(L, e, launch);
+}
+
+void launch_rb_sweep3d(const Lc &L, int kind, const SweepSel *e, int neq, int launch) {
+    if (L.batch) launchZ<true>(L, kind, e, neq, launch);
+    else launchZ<false>(L, kind, e, neq, launch);
 }
 
 }  // namespace ifl

@@ -32,7 +32,7 @@ FIELD_NAMES = ["u", "v", "w", "p", "temp", "pcorr", "uf", "vf", "wf", "mcoef", "
 
 TILE_AUTO, TILE_64x16, TILE_64x8, TILE_32x8, TILE_32x4 = range(5)
 
-(KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE) = range(5)
+(KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE, KERNEL_MOM_SWEEP) = range(6)
 
 
 class Config(C.Structure):
