@@ -24,7 +24,7 @@
 
 namespace ifl {
 
-constexpr int Z_TX = 64, Z_TY = 8, Z_ZC = 16;
+constexpr int Z_TX = 128, Z_TY = 8, Z_ZC = 32;   // 256^3: 8.11 ms per outer iteration, against 9.27 ms at 64x8x16
 
 template <int TX, int TY, int ZC, int NEQ, bool PC, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1,

@@ -275,9 +275,11 @@ def test_sweep_blocks_on_interior_tiles_with_refinement(block):
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
-@pytest.mark.parametrize("shape", [(70, 20, 40), (9, 17, 33)])
+@pytest.mark.parametrize("shape", [(150, 20, 70), (70, 20, 40), (9, 17, 33)])
 def test_3d_zmarch_multiple_tiles_and_chunks(shape):
-    """The 3D fused sweep with several x/y tiles and z chunks (chunk edges recompute red planes)."""
+    """The 3D fused sweep with several x/y tiles and z chunks (chunk edges recompute red planes;
+    the tile is 128 x 8 cells over 32 planes), u, v, w with different sweep counts (launches of
+    three, two and one equations)."""
     cfg = CASES["duct3d"]()
     cfg.nx, cfg.ny, cfg.nz = shape
     cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[2], cfg.n_iter[3] = 3, 2, 5, 7
