@@ -16,8 +16,8 @@
 // reads only its +x, +y and +z planes: its matrix is symmetric bit for bit (the coefficient of
 // a cell towards -x is the +x coefficient of its west neighbour, kernels.cuh:517-529 computes
 // both from the same commutative sum) and its diagonal is -(E+W+N+S) - (T+B) as assembled.
-// The coefficients and sources of the cells a step relaxes are loaded one step ahead, into
-// registers, so their latency overlaps the previous step.
+// The coefficients and sources of the cells a step relaxes are loaded ahead, into registers,
+// so their latency overlaps the previous steps.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
@@ -212,16 +212,20 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
         }
     };
 
-    Ops red[NRT], blk0[NBT];
+    // The red and black operands of one plane are loaded in the same step, so the second load
+    // finds the cache lines the first brought in (the cells of a colour sit two floats apart);
+    // the black ones are used two steps later.
+    Ops red[NRT], blk0[NBT], blk1[NBT];
     {
         Ops r0[NRT];
         loadRed(z0 - 1, r0);
         loadRed(z0, red);
+        loadBlack(z0, blk0);
         redPlane(z0 - 1, r0);
         redPlane(z0, red);
     }
     loadRed(z0 + 1, red);
-    loadBlack(z0, blk0);
+    loadBlack(z0 + 1, blk1);
     // write plane z of the tile
     auto writePlane = [&](int z) {
         for (int q = tid; q < TX * TY; q += kBlock) {
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
         loadPlane(t + 2, nxt);
         Ops redN[NRT], blkN[NBT];
         loadRed(t + 2, redN);
-        loadBlack(t + 1, blkN);
+        loadBlack(t + 2, blkN);
         storePlane(t + 2, nxt);
         __syncthreads();
         redPlane(t + 1, red);
@@ -254,7 +258,10 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
 #pragma unroll
         for (int m = 0; m < NRT; ++m) red[m] = redN[m];
 #pragma unroll
-        for (int m = 0; m < NBT; ++m) blk0[m] = blkN[m];
+        for (int m = 0; m < NBT; ++m) {
+            blk0[m] = blk1[m];
+            blk1[m] = blkN[m];
+        }
     }
     {
         double tot[NEQ];
