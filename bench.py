@@ -221,6 +221,20 @@ def adjoint_leg(args, incompflow, _abi):
     return out
 
 
+def copy_probe():
+    """A plain device copy of known size: what the HBM counters read for it scales their readings
+    on the solver's kernels."""
+    import torch
+    n = 64 * 1024 * 1024          # 64 Mi floats = 256 MiB each way
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"[bench] copied 5 x {n * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +256,8 @@ def main():
     ap.add_argument("--batch-max-iter", type=int, default=20000)
     ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="estimate")
     ap.add_argument("--batch-probe-iters", type=int, default=300)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--no-cavity3d", action="store_true")
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-steps", type=int, default=20)
@@ -250,6 +266,9 @@ def main():
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
     ap.add_argument("--adjoint-fd", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU slab decomposition is a design, not an implementation.", file=sys.stderr)
@@ -302,8 +321,10 @@ def main():
     # (FETCH_SIZE and WRITE_SIZE runs, scaled by a known-size copy): null on other workloads
     traffic = None
     traffic_source = None
-    tfile = os.path.join(ROOT, "profiles", "k_rb_pc2d_cavity1024_k4_traffic.json")
-    if cfg.dim == 2 and n == 1024 and kb == 4 and args.tile in (0, 1) and os.path.exists(tfile):
+    tile = (args.tile or _abi.default_tile(cfg.nx, cfg.ny)) if cfg.dim == 2 else 0
+    tname = {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16"}.get(tile)
+    tfile = os.path.join(ROOT, "profiles", f"k_rb_pc2d_{tname}_cavity1024_k4_traffic.json")
+    if cfg.dim == 2 and n == 1024 and kb == 4 and tname and os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh)["traffic_bytes_per_launch"]
         traffic_source = os.path.relpath(tfile, ROOT)
@@ -394,7 +415,8 @@ def main():
                    (f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
                     f"FP64 reductions"),
                    "cells": cells, "sweeps_per_step": sweeps / args.steps, "sweep_block": kb,
-                   "tile": cfg.tile},
+                   "tile": {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16", _abi.TILE_64x8: "64x8",
+                            _abi.TILE_32x8: "32x8", _abi.TILE_32x4: "32x4"}.get(tile, "3d")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": ("k_rb_pc2d (fused red-black sweeps of p')" if cfg.dim == 2 else

@@ -89,7 +89,7 @@ int validate(const ifl_config &c) {
         return fail(IFL_ERR_ARG, "smoother must be IFL_SMOOTHER_RBGS or IFL_SMOOTHER_JACOBI on the device");
     if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
     if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
-    if (c.tile < 0 || c.tile > 4) return fail(IFL_ERR_ARG, "tile must be 0..4");
+    if (c.tile < 0 || c.tile > ifl::TILE_128x32) return fail(IFL_ERR_ARG, "tile must be 0..5");
     if (c.sweep_block < 0 || c.sweep_block > 4 || c.sweep_block == 3)
         return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2 or 4");
     return IFL_OK;
@@ -132,7 +132,7 @@ void fillParams(ifl_solver *h) {
 int partStride(const ifl::CaseParams &P) {
     int n = std::max(ifl::cell_blocks(P), ifl::zmarch_blocks(P));
     if (P.dim == 2)
-        for (int t = ifl::TILE_64x16; t <= ifl::TILE_32x4; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
+        for (int t = ifl::TILE_64x16; t <= ifl::TILE_128x32; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
     return n;
 }
 
@@ -920,7 +920,7 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
     if ((what & IFL_TUNE_LAUNCH) && h->P.dim == 2 && c0.smoother == IFL_SMOOTHER_RBGS) {
         bestUs = 1e300;
         Settings x = best;
-        for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_32x4; ++tile)
+        for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_128x32; ++tile)
             for (int K = 1; K <= 4; K *= 2) {
                 x.tile = tile;
                 x.K = K;

@@ -114,11 +114,11 @@ __device__ __forceinline__ double waveSum(double x) {
     return x;
 }
 
-// Reduce K per-thread values over the workgroup (all threads must call); every thread returns
-// the K totals.
-template <int K>
+// Reduce K per-thread values over the workgroup of NW waves (all threads must call); every
+// thread returns the K totals.
+template <int K, int NW = kWaves>
 __device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {
-    __shared__ double lds[kWaves][K];
+    __shared__ double lds[NW][K];
     __shared__ double bc[K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -130,7 +130,7 @@ __device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K])
     if (threadIdx.x < K) {
         double r = lds[0][threadIdx.x];
 #pragma unroll
-        for (int q = 1; q < kWaves; ++q) r += lds[q][threadIdx.x];
+        for (int q = 1; q < NW; ++q) r += lds[q][threadIdx.x];
         bc[threadIdx.x] = r;
     }
     __syncthreads();

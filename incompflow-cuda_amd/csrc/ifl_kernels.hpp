@@ -19,7 +19,8 @@ struct SweepSel {
 };
 
 enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2 };
-enum { TILE_AUTO = 0, TILE_64x16 = 1, TILE_64x8 = 2, TILE_32x8 = 3, TILE_32x4 = 4 };
+// 2D sweep tiles; 128x32 runs 1024-thread workgroups, the others 256
+enum { TILE_AUTO = 0, TILE_64x16 = 1, TILE_64x8 = 2, TILE_32x8 = 3, TILE_32x4 = 4, TILE_128x32 = 5 };
 enum { RED_MOM = 1, RED_MASS = 2, RED_UPD = 4, RED_FINISH = 8 };
 
 // Launch context: one case (P, grids from P) or a batch (device map, total workgroups per kind).

@@ -30,7 +30,26 @@ FIELD_NAMES = ["u", "v", "w", "p", "temp", "pcorr", "uf", "vf", "wf", "mcoef", "
  STAGE_SMOOTH_V, STAGE_SMOOTH_W, STAGE_RHIE_CHOW, STAGE_PCORR_ASSEMBLE, STAGE_SMOOTH_P,
  STAGE_UPDATE, STAGE_OUTER) = range(12)
 
-TILE_AUTO, TILE_64x16, TILE_64x8, TILE_32x8, TILE_32x4 = range(5)
+TILE_AUTO, TILE_64x16, TILE_64x8, TILE_32x8, TILE_32x4, TILE_128x32 = range(6)
+_TILE_SHAPE = {TILE_64x16: (64, 16), TILE_64x8: (64, 8), TILE_32x8: (32, 8), TILE_32x4: (32, 4), TILE_128x32: (128, 32)}
+
+
+def tile_count(nx, ny, tile):
+    tx, ty = _TILE_SHAPE[tile]
+    return -(-nx // tx) * -(-ny // ty)
+
+
+def default_tile(nx, ny):
+    """The 2D sweep tile a single case gets with tile = 0 (the library's rule, sweep2d.hip)."""
+    if tile_count(nx, ny, TILE_128x32) >= 256:
+        return TILE_128x32
+    if tile_count(nx, ny, TILE_64x16) >= 768:
+        return TILE_64x16
+    if tile_count(nx, ny, TILE_64x8) >= 512:
+        return TILE_64x8
+    if tile_count(nx, ny, TILE_32x8) >= 256:
+        return TILE_32x8
+    return TILE_32x4
 
 (KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE, KERNEL_MOM_SWEEP) = range(6)
 

@@ -208,8 +208,8 @@ def test_bad_config_rejected():
         Solver(cfg)
 
 
-@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_64x8, _abi.TILE_32x8, _abi.TILE_32x4])
-@pytest.mark.parametrize("shape", [(70, 37), (33, 18)])
+@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_64x8, _abi.TILE_32x8, _abi.TILE_32x4, _abi.TILE_128x32])
+@pytest.mark.parametrize("shape", [(70, 37), (33, 18), (150, 45)])
 def test_fused_sweep_tiles_ragged_grids(tile, shape):
     """Every tile variant of the fused sweep on grids that are not tile multiples."""
     cfg = _abi.channel_config(shape[0], shape[1], 6)
@@ -236,7 +236,7 @@ def test_odd_and_unequal_sweep_counts():
 
 
 @pytest.mark.parametrize("block", [2, 4])
-@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_32x4])
+@pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_32x4, _abi.TILE_128x32])
 @pytest.mark.parametrize("name", ["channel2d", "cavity2d"])
 def test_two_sweeps_per_launch(tile, name, block):
     """sweep_block=2 and 4 (two or four red-black sweeps per launch over a 4- or 8-cell halo; the
@@ -258,20 +258,22 @@ def test_two_sweeps_per_launch(tile, name, block):
 
 
 @pytest.mark.parametrize("block", [2, 4])
-def test_sweep_blocks_on_interior_tiles_with_refinement(block):
+@pytest.mark.parametrize("tile,shape", [(_abi.TILE_64x16, (256, 96)), (_abi.TILE_128x32, (400, 128))])
+def test_sweep_blocks_on_interior_tiles_with_refinement(block, tile, shape):
     """A grid with interior tiles (their ring lies inside the grid, the unchecked fast path),
     20 p' sweeps (five launches of four: an odd count, started from the partner buffer) and the
     refinement, whose correction is added to p' by its last launch; 12 outer iterations."""
     cfg = CASES["cavity2d"]()
-    cfg.nx, cfg.ny = 256, 96
+    cfg.nx, cfg.ny = shape
     cfg.sweep_block = block
+    cfg.tile = tile
     cfg.p_refine = 1
-    _log(f"[interior] block={block} grid=256x96")
+    _log(f"[interior] block={block} tile={tile} grid={shape}")
     g = Solver(cfg)
     o = Oracle(cfg)
     rg = g.solve(12)
     ro = o.solve(12)
-    _compare_all(g, o, cfg, f"interior block {block}")
+    _compare_all(g, o, cfg, f"interior block {block} tile {tile}")
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
