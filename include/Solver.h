@@ -85,6 +85,11 @@ public:
         ifl_compat::check(ifl_write_vtk(h_, ("output/" + filepath).c_str()));
     }
 
+    // No reference counterpart: binary state files (ifl_save_state / ifl_load_state) to continue
+    // a solve or warm start a case of the same grid; paths are used as given.
+    void saveState(const std::string &path) const { ifl_compat::check(ifl_save_state(h_, path.c_str())); }
+    void loadState(const std::string &path) { ifl_compat::check(ifl_load_state(h_, path.c_str())); }
+
     // The C-ABI handle, for field access (ifl_get_field) and the other entry points.
     ifl_solver *handle() const { return h_; }
 

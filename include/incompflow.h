@@ -185,6 +185,41 @@ size_t ifl_field_size(const ifl_solver *h, int field);
  * the reference prefixes "output/", which ifl::Solver in Solver.h reproduces). */
 int ifl_write_vtk(ifl_solver *h, const char *path);
 
+/* ---- Binary state export / import (no reference counterpart) ----
+ * The reference's constructor takes only uniform initial values (Solver.h:46) and its only output,
+ * writeVTK (Solver.cu:301-367), is ASCII and write-only, so a run cannot be continued or warm
+ * started.  A state file holds what the next outer iteration reads: the fields u, v, w, p, temp,
+ * uf, vf, wf (FP32) and the outer-iteration monitor (iteration count, the running maxima of the
+ * stop rules, the scaled-residual normalisers).  Loading it into a handle with the same dim and
+ * grid size, then solving on, gives the same iterates bit for bit as the solve that wrote it would
+ * have continued with; loading it into a handle whose other settings differ (viscosity, boundary
+ * values, inlet profiles, relaxation, sweeps) is a warm start of that case from the saved state.
+ *
+ * File layout (little-endian): one ifl_state_header, then the eight fields in the order u, v, w,
+ * p, temp, uf, vf, wf, each as ifl_get_field returns it (float32, its full element count). */
+#define IFL_STATE_MAGIC "IFLSTATE"
+#define IFL_STATE_VERSION 1
+
+typedef struct ifl_state_header {
+    char magic[8];               /* IFL_STATE_MAGIC, no terminating 0 */
+    uint32_t version;            /* IFL_STATE_VERSION */
+    int32_t dim, nx, ny, nz;
+    int32_t last_converged;      /* the stop rule held at the last iteration */
+    int64_t iter;                /* outer iterations executed */
+    double norm_max[7];          /* running maxima of the p, u, v, w, uf, vf, wf increment norms */
+    double res_max[4];           /* scaled-residual normalisers: continuity, u, v, w momentum */
+    double last_scaled;          /* scaled residual of the last iteration */
+    uint64_t payload_bytes;      /* bytes of field data after the header */
+    uint64_t checksum;           /* FNV-1a (64-bit) of the field data */
+} ifl_state_header;
+
+/* Write the handle's state to path (created or replaced). */
+int ifl_save_state(ifl_solver *h, const char *path);
+/* Read a state file into the handle.  Fails with IFL_ERR_ARG (and leaves the handle unchanged)
+ * when dim/nx/ny/nz differ from the handle's, and with IFL_ERR_IO when the file is missing,
+ * truncated, of another version or fails its checksum. */
+int ifl_load_state(ifl_solver *h, const char *path);
+
 /* Run one stage on the handle's device state (parity tests).  out (optional, >= 8 doubles)
  * receives stage-specific FP64 values:
  *   SMOOTH_U/V/W/P : [0] sweeps executed, [1] sum of squared increments of the last sweep

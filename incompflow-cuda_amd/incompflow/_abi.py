@@ -304,3 +304,82 @@ def adjoint_inlet_profile(cfg, u_max=0.02, sigma=1e-3, seed=20261203):
         xi = math.sqrt(-2.0 * math.log(max(u1, 1e-300))) * math.cos(2.0 * math.pi * u2)
         out.append(4.0 * u_max * y * (H - y) / (H * H) + sigma * xi)
     return out
+
+
+# ---- binary state files (ifl_save_state / ifl_load_state) ----
+STATE_MAGIC = b"IFLSTATE"
+STATE_VERSION = 1
+STATE_FIELDS = (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_UF, FIELD_VF, FIELD_WF)
+
+
+class StateHeader(C.LittleEndianStructure):
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("dim", C.c_int32), ("nx", C.c_int32),
+                ("ny", C.c_int32), ("nz", C.c_int32), ("last_converged", C.c_int32), ("iter", C.c_int64),
+                ("norm_max", C.c_double * 7), ("res_max", C.c_double * 4), ("last_scaled", C.c_double),
+                ("payload_bytes", C.c_uint64), ("checksum", C.c_uint64)]
+
+
+def fnv1a64(data):
+    """FNV-1a (64-bit) of a bytes-like object: the checksum of a state file's field data."""
+    h = 0xcbf29ce484222325
+    for x in bytes(data):
+        h = ((h ^ x) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _state_sizes(dim, nx, ny, nz):
+    c = Config()
+    c.dim, c.nx, c.ny, c.nz = dim, nx, ny, nz
+    return [(f, c.field_size(f), c.shape(f)) for f in STATE_FIELDS]
+
+
+def read_state(path, verify=True):
+    """Parse a state file without the library: (header, {field id: float32 array, shaped})."""
+    import numpy as np
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    if len(raw) < C.sizeof(StateHeader):
+        raise ValueError(f"{path}: shorter than a state header")
+    hd = StateHeader.from_buffer_copy(raw[:C.sizeof(StateHeader)])
+    if hd.magic != STATE_MAGIC or hd.version != STATE_VERSION:
+        raise ValueError(f"{path}: not a version-{STATE_VERSION} state file")
+    payload = raw[C.sizeof(StateHeader):]
+    sizes = _state_sizes(hd.dim, hd.nx, hd.ny, hd.nz)
+    if len(payload) != hd.payload_bytes or hd.payload_bytes != 4 * sum(n for _, n, _ in sizes):
+        raise ValueError(f"{path}: payload size does not match the grid")
+    if verify and fnv1a64(payload) != hd.checksum:
+        raise ValueError(f"{path}: checksum mismatch")
+    data = np.frombuffer(payload, dtype="<f4")
+    out, off = {}, 0
+    for f, n, shp in sizes:
+        out[f] = data[off:off + n].reshape(shp).copy()
+        off += n
+    return hd, out
+
+
+def write_state(path, dim, nx, ny, nz, fields, iter=0, norm_max=None, res_max=None, last_scaled=0.0,
+                last_converged=0):
+    """Write a state file from host arrays (fields: {field id: array}; missing fields are zero)."""
+    import numpy as np
+    sizes = _state_sizes(dim, nx, ny, nz)
+    parts = []
+    for f, n, _ in sizes:
+        a = np.zeros(n, dtype="<f4") if f not in fields else np.ascontiguousarray(fields[f], dtype="<f4").ravel()
+        if a.size != n:
+            raise ValueError(f"field {FIELD_NAMES[f]}: {a.size} values, the grid needs {n}")
+        parts.append(a.tobytes())
+    payload = b"".join(parts)
+    hd = StateHeader()
+    hd.magic, hd.version = STATE_MAGIC, STATE_VERSION
+    hd.dim, hd.nx, hd.ny, hd.nz = dim, nx, ny, nz
+    hd.last_converged, hd.iter = last_converged, iter
+    for q, v in enumerate(norm_max or [0.0] * 7):
+        hd.norm_max[q] = v
+    for q, v in enumerate(res_max or [0.0] * 4):
+        hd.res_max[q] = v
+    hd.last_scaled = last_scaled
+    hd.payload_bytes = len(payload)
+    hd.checksum = fnv1a64(payload)
+    with open(path, "wb") as fh:
+        fh.write(bytes(hd))
+        fh.write(payload)

@@ -27,6 +27,8 @@ EXPORTS = [
     ("ifl_set_field", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
     ("ifl_field_size", C.c_size_t, [C.c_void_p, C.c_int]),
     ("ifl_write_vtk", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("ifl_save_state", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("ifl_load_state", C.c_int, [C.c_void_p, C.c_char_p]),
     ("ifl_run_stage", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
     ("ifl_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     ("ifl_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(_abi.TuneResult)]),
@@ -147,6 +149,14 @@ class Solver:
 
     def write_vtk(self, path):
         _check(lib().ifl_write_vtk(self.h, os.fsencode(path)))
+
+    def save_state(self, path):
+        """Binary state file: fields and outer-iteration monitor (ifl_save_state)."""
+        _check(lib().ifl_save_state(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        """Continue from (or warm start with) a state file of the same grid (ifl_load_state)."""
+        _check(lib().ifl_load_state(self.h, os.fsencode(path)))
 
     def adjoint(self, objective=_abi.OBJ_PRESSURE_DROP, face=_abi.WEST, weights=None, max_iter=0, tol=0.0,
                 warm_start=False, check_every=0, profile_gradient=True, grad_tol=0.0, tile=0):
