@@ -14,6 +14,8 @@ batch      : the seeded 64-case uneven batch (2D cavities and channels, 64..512 
              the batch equals its single solve bit for bit, so both runs execute the same iterations;
              the sequential time is measured in full with --batch-sequential full, else it is each
              case's measured single-solve time per iteration times its iteration count.
+energy     : the same cavity with heated walls and the temperature equation on: ms per outer
+             iteration with and without it, and the temperature sweep launch (unless --no-energy).
 adjoint    : the 1024x256 channel with a noisy parabolic inlet solved to the scaled residual
              --adjoint-forward-target (1e-5: FP32 stalls near 9.4e-6 on this grid), then the discrete
              adjoint of the pressure drop until dJ/dmu settles; wall times of both, and with
@@ -221,6 +223,42 @@ def adjoint_leg(args, incompflow, _abi):
     return out
 
 
+def energy_leg(args, incompflow, _abi, base_ms):
+    """The 2D cavity with heated walls (south 373 K, north 293 K, east and west adiabatic, k / c_p =
+    mu / 0.71): ms per outer iteration with the temperature equation on (10 sweeps per iteration),
+    against the same case without it, and the temperature sweep launch with its algorithmic GB/s."""
+    n = args.n
+    c = _abi.cavity_config(n=n, p_refine=args.p_refine)
+    c.stop_rule = _abi.STOP_NONE
+    c.sweep_block = args.sweep_block
+    c.tile = args.tile
+    c.energy, c.n_iter_t, c.relax_t, c.tol_t = 1, 10, 0.7, 1e-5
+    c.conductivity, c.specific_heat = c.viscosity / 0.71, 1.0
+    for f in range(6):
+        c.temp_bc_type[f], c.temp_bc_val[f] = _abi.TEMP_NEUMANN, 0.0
+    c.temp_bc_type[_abi.NORTH], c.temp_bc_val[_abi.NORTH] = _abi.TEMP_DIRICHLET, 293.0
+    c.temp_bc_type[_abi.SOUTH], c.temp_bc_val[_abi.SOUTH] = _abi.TEMP_DIRICHLET, 373.0
+    c.init[4] = 293.0
+    s = incompflow.Solver(c)
+    s.solve(args.warmup, history=False)
+    device_sync()
+    t0 = time.perf_counter()
+    r = s.solve(args.steps, history=True)
+    device_sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    us, _ = s.time_kernel(_abi.KERNEL_TEMP_SWEEP, reps=max(10, args.kernel_reps // 4))
+    s.close()
+    kb = min(2, max(1, args.sweep_block))
+    cells = n * n
+    log(f"[bench] energy: {ms:.4f} ms/step with temperature ({base_ms:.4f} without), T sweep launch {us:.2f} us")
+    return {"workload": f"2D cavity {n}x{n}, Re 100, walls south 373 K / north 293 K, east/west adiabatic, "
+                        "k/c_p = mu/0.71, 10 temperature sweeps per outer iteration",
+            "ms_per_step": ms, "ms_per_step_without": base_ms, "overhead": ms / base_ms - 1.0,
+            "sweeps_t_per_step": sum(x.sweeps_t for x in r["history"]) / args.steps,
+            "temp_sweep_us": us, "temp_sweep_model_gbs": kb * 32.0 * cells / (us * 1e3),
+            "temp_sweep_own_gbs": 20.0 * cells / (us * 1e3)}
+
+
 def copy_probe():
     """A plain device copy of known size: what the HBM counters read for it scales their readings
     on the solver's kernels."""
@@ -263,6 +301,7 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--c3-cpu-iters", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
     ap.add_argument("--adjoint-fd", action="store_true")
     args = ap.parse_args()
@@ -395,6 +434,9 @@ def main():
     cavity3d = None
     if not args.no_cavity3d and cfg.dim == 2:
         cavity3d = cavity3d_leg(args, incompflow, _abi)
+    energy = None
+    if not args.no_energy and cfg.dim == 2:
+        energy = energy_leg(args, incompflow, _abi, ms_per_step)
 
     out = {
         "metric": METRIC,
@@ -433,6 +475,7 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "cavity3d": cavity3d,
+        "energy": energy,
     }
     if cpu:
         out["gpu_vs_cpu_per_step"] = cpu["ms_per_step"] / ms_per_step

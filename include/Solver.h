@@ -42,6 +42,17 @@ inline ifl_config config_from_header(scalar u, scalar v, scalar w, scalar p, sca
     c.tol[0] = tol_u; c.tol[1] = tol_v; c.tol[2] = tol_w; c.tol[3] = tol_p;
     c.tol_mass = tol_mass;
     c.init[0] = u; c.init[1] = v; c.init[2] = w; c.init[3] = p; c.init[4] = temp;
+    // config.h:19-20, 45-63: thermal properties and TempBCs.  The reference never solves for
+    // temperature; defining IFL_SOLVE_ENERGY before this header turns the equation on.
+    c.conductivity = thermalConductivity;
+    c.specific_heat = specificHeatCapacity;
+    for (int f = 0; f < 6; ++f) {
+        c.temp_bc_type[f] = TempBCs::type[f];
+        c.temp_bc_val[f] = TempBCs::val[f];
+    }
+#ifdef IFL_SOLVE_ENERGY
+    c.energy = 1;
+#endif
     c.stop_rule = IFL_STOP_FAITHFUL;   // Solver.cu:237-268 as written
     c.log_every = 1000;                // Solver.cu:270
     return c;

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define IFL_ABI_VERSION 1
+#define IFL_ABI_VERSION 2
 
 /* ---- status codes ---- */
 enum {
@@ -49,6 +49,9 @@ enum { IFL_SLOT_C = 0, IFL_SLOT_E = 1, IFL_SLOT_W = 2, IFL_SLOT_N = 3, IFL_SLOT_
 
 /* ---- velocity BC codes, include/constants.h:34-36 ---- */
 enum { IFL_WALL = 0, IFL_INLET = 1, IFL_OUTLET = 2 };
+
+/* ---- temperature BC codes, include/config.h:46-53 (TempBCs::type) ---- */
+enum { IFL_TEMP_DIRICHLET = 0, IFL_TEMP_NEUMANN = 1 };
 
 /* ---- smoother (linear solver) ---- */
 enum {
@@ -76,15 +79,18 @@ enum {
     IFL_FIELD_PCOEF = 10, /* pressure-correction matrix, (1+2dim) planes of N */
     IFL_FIELD_BU = 11, IFL_FIELD_BV = 12, IFL_FIELD_BW = 13,
     IFL_FIELD_BP = 14,    /* mass imbalance b' */
-    IFL_FIELD_COUNT = 15
+    IFL_FIELD_TCOEF = 15, /* temperature matrix, (1+2dim) planes of N (energy on) */
+    IFL_FIELD_BT = 16,    /* temperature source (energy on) */
+    IFL_FIELD_COUNT = 17
 };
 
 /* ---- stages for ifl_run_stage: one SIMPLE outer iteration is 1..9 in order ---- */
 enum {
     IFL_STAGE_INIT_FACES = 0,   /* Solver.cu:369-408  initFaceVel (first iteration only) */
     IFL_STAGE_FACE_BC = 1,      /* Solver.cu:410-465  applyBCsToFaceVel */
-    IFL_STAGE_MOM_ASSEMBLE = 2, /* Solver.cu:167-175  coefficients + sources + BC rows */
-    IFL_STAGE_MOM_RESIDUAL = 3, /* FP64 ||b - A phi|| per momentum component (no reference) */
+    IFL_STAGE_MOM_ASSEMBLE = 2, /* Solver.cu:167-175  coefficients + sources + BC rows (and the
+                                   temperature matrix and source, energy on) */
+    IFL_STAGE_MOM_RESIDUAL = 3, /* FP64 ||b - A phi|| per momentum component and T (no reference) */
     IFL_STAGE_SMOOTH_U = 4,     /* Solver.cu:178 */
     IFL_STAGE_SMOOTH_V = 5,     /* Solver.cu:180 */
     IFL_STAGE_SMOOTH_W = 6,     /* Solver.cu:183 */
@@ -93,7 +99,8 @@ enum {
     IFL_STAGE_SMOOTH_P = 9,     /* Solver.cu:196 (+ FP64-defect refinement cycles) */
     IFL_STAGE_UPDATE = 10,      /* Solver.cu:198      updateField */
     IFL_STAGE_OUTER = 11,       /* one whole outer iteration incl. the stop test */
-    IFL_STAGE_COUNT = 12
+    IFL_STAGE_SMOOTH_T = 12,    /* temperature sweeps (energy on; after SMOOTH_U..W in an iteration) */
+    IFL_STAGE_COUNT = 13
 };
 
 /* Runtime case description; field-for-field the reference's config.h (line cited). */
@@ -125,7 +132,22 @@ typedef struct ifl_config {
     int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2 or 4; the momentum
                                     equations use at most 2): a stop test met inside a block takes
                                     effect at the block's end (1 = the reference's per-sweep exit) */
-    int32_t reserved[5];
+    /* Energy (temperature) transport.  The reference declares the thermal properties and the
+     * temperature BC table (config.h:19-23, 45-63) but never solves for temperature; here the
+     * steady equation  div(rho u T) = div((k / c_p) grad T)  is discretised like the momentum
+     * equations (kernels.cuh:116-131 with mu -> k / c_p; Dirichlet faces take the wall row of
+     * kernels.cuh:304-320, Neumann faces the zero-gradient row of :321-331) on the face
+     * velocities of the outer iteration, and swept after the momentum equations.  Temperature
+     * is passive: nothing of the flow depends on it. */
+    int32_t energy;              /* 1: solve for temperature in every outer iteration (0: carry it) */
+    int32_t n_iter_t;            /* temperature sweeps per outer iteration */
+    double conductivity;         /* config.h:19 thermalConductivity */
+    double specific_heat;        /* config.h:20 specificHeatCapacity */
+    double relax_t;              /* under-relaxation of the temperature sweeps */
+    double tol_t;                /* sweep stop tolerance of the temperature sweeps */
+    int32_t temp_bc_type[6];     /* config.h:46-53 TempBCs::type[face]: IFL_TEMP_DIRICHLET / NEUMANN */
+    double temp_bc_val[6];       /* config.h:55-62 TempBCs::val[face] (Dirichlet values) */
+    int32_t reserved[4];
 } ifl_config;
 
 /* One record per outer iteration (device-written, FP64). */
@@ -134,11 +156,14 @@ typedef struct ifl_iter_record {
     double uf_norm, vf_norm, wf_norm;
     double mass_norm;                       /* RMS of b' before correction */
     double res_cont, res_u, res_v, res_w;   /* FP64 L2: ||b'||, ||b_u - A u|| ... before sweeps */
-    double scaled;                          /* max_x res_x / max(res_x over iterations 1..5) */
+    double scaled;                          /* max_x res_x / max(res_x over iterations 1..5), x over
+                                               continuity, momentum (and temperature, energy on) */
     double rel_p, rel_u, rel_v, rel_w, rel_uf, rel_vf, rel_wf; /* stop-rule relative norms */
     int32_t sweeps[4];                      /* smoother sweeps executed: u, v, w, p' (incl. refine) */
     int32_t converged;                      /* stop rule met at this iteration */
-    int32_t pad;
+    int32_t sweeps_t;                       /* temperature sweeps executed (energy on) */
+    double res_t;                           /* FP64 L2 ||b_T - A_T T|| before the sweeps (energy on;
+                                               part of the scaled residual then) */
 } ifl_iter_record;
 
 typedef struct ifl_report {
@@ -207,7 +232,7 @@ typedef struct ifl_state_header {
     int32_t last_converged;      /* the stop rule held at the last iteration */
     int64_t iter;                /* outer iterations executed */
     double norm_max[7];          /* running maxima of the p, u, v, w, uf, vf, wf increment norms */
-    double res_max[4];           /* scaled-residual normalisers: continuity, u, v, w momentum */
+    double res_max[5];           /* scaled-residual normalisers: continuity, u, v, w momentum, T */
     double last_scaled;          /* scaled residual of the last iteration */
     uint64_t payload_bytes;      /* bytes of field data after the header */
     uint64_t checksum;           /* FNV-1a (64-bit) of the field data */
@@ -222,8 +247,8 @@ int ifl_load_state(ifl_solver *h, const char *path);
 
 /* Run one stage on the handle's device state (parity tests).  out (optional, >= 8 doubles)
  * receives stage-specific FP64 values:
- *   SMOOTH_U/V/W/P : [0] sweeps executed, [1] sum of squared increments of the last sweep
- *   MOM_RESIDUAL   : [0..2] ||b_u - A u||, ||b_v - A v||, ||b_w - A w|| (L2)
+ *   SMOOTH_U/V/W/P/T : [0] sweeps executed, [1] sum of squared increments of the last sweep
+ *   MOM_RESIDUAL   : [0..3] ||b_u - A u||, ||b_v - A v||, ||b_w - A w||, ||b_T - A_T T|| (L2)
  *   PCORR_ASSEMBLE : [0] sum of b'^2
  *   UPDATE         : [0..6] sums of squared increments of p, u, v, w, uf, vf, wf
  *   OUTER          : [0] converged flag, [1] scaled residual */
@@ -239,7 +264,8 @@ enum {
     IFL_KERNEL_UPDATE = 4,
     IFL_KERNEL_MOM_SWEEP = 5,     /* one launch of the momentum smoother (u, v and, in 3D, w swept
                                      together on their shared matrix) */
-    IFL_KERNEL_COUNT = 6
+    IFL_KERNEL_TEMP_SWEEP = 6,    /* one launch of the temperature smoother (energy on) */
+    IFL_KERNEL_COUNT = 7
 };
 
 /* Measurement hook (no reference counterpart): launch one kernel `reps` times on the handle's

@@ -52,13 +52,12 @@ struct ifl_solver {
     hipStream_t stream = nullptr;
     char *arena = nullptr;
     size_t arenaBytes = 0;
-    float *temp = nullptr;
     float *prof[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
     bool graphDirty = true;
     long long iter = 0;   // host mirror of CaseState::iter
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float tolf[4];
+    float tolf[5];
     int tile = 0;     // 2D sweep tile variant
     struct AdjHost *adj = nullptr;   // discrete adjoint buffers, made by the first ifl_adjoint call
 };
@@ -92,6 +91,15 @@ int validate(const ifl_config &c) {
     if (c.tile < 0 || c.tile > ifl::TILE_128x32) return fail(IFL_ERR_ARG, "tile must be 0..5");
     if (c.sweep_block < 0 || c.sweep_block > 4 || c.sweep_block == 3)
         return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2 or 4");
+    if (c.energy != 0 && c.energy != 1) return fail(IFL_ERR_ARG, "energy must be 0 or 1");
+    if (c.energy) {
+        if (c.n_iter_t < 0 || c.n_iter_t > kMaxSweeps) return fail(IFL_ERR_ARG, "n_iter_t out of range");
+        if (!(c.conductivity >= 0) || !(c.specific_heat > 0))
+            return fail(IFL_ERR_ARG, "energy needs conductivity >= 0 and specific_heat > 0");
+        for (int f = 0; f < 6; ++f)
+            if (c.temp_bc_type[f] != IFL_TEMP_DIRICHLET && c.temp_bc_type[f] != IFL_TEMP_NEUMANN)
+                return fail(IFL_ERR_ARG, "temp_bc_type must be 0 (Dirichlet) or 1 (Neumann)");
+    }
     return IFL_OK;
 }
 
@@ -121,16 +129,26 @@ void fillParams(ifl_solver *h) {
         h->tolf[q] = (float)c.tol[q];
         P.tol[q] = (double)h->tolf[q];
     }
+    P.relax[4] = (float)c.relax_t;
+    h->tolf[4] = (float)c.tol_t;
+    P.tol[4] = (double)h->tolf[4];
     P.tolMass = (double)(float)c.tol_mass;
     P.tolRes = c.tol_residual;
     P.stopRule = c.stop_rule;
     P.histCap = kHistCap;
+    // temperature (config.h:19-23, 45-63): diffusion coefficient k / c_p in float arithmetic
+    P.energy = c.energy;
+    P.gam = (float)c.conductivity / (float)c.specific_heat;
+    for (int f = 0; f < 6; ++f) {
+        P.tbct[f] = c.temp_bc_type[f];
+        P.tbcv[f] = (float)c.temp_bc_val[f];
+    }
 }
 
 // Records per (slot, launch parity, sweep) partial array: one per workgroup of the largest sweep
 // launch (fused 2D tiles: the smallest tile gives the most; 3D z-march; Jacobi cell grid).
 int partStride(const ifl::CaseParams &P) {
-    int n = std::max(ifl::cell_blocks(P), ifl::zmarch_blocks(P));
+    int n = std::max({ifl::cell_blocks(P), ifl::zmarch_blocks(P, false), ifl::zmarch_blocks(P, true)});
     if (P.dim == 2)
         for (int t = ifl::TILE_64x16; t <= ifl::TILE_128x32; ++t) n = std::max(n, ifl::tile_count(P.nx, P.ny, t));
     return n;
@@ -144,23 +162,30 @@ struct Plan {
 };
 
 // Sweeps per launch of a smoother call: sweep_block for p' (2D red-black), at most 2 for the
-// momentum equations (their fused launch stages two fields and the face velocities), 1 otherwise.
+// momentum and temperature equations (their fused launch stages the face velocities), 1 otherwise.
 int sweepBlockOf(const ifl_config &c, int dim, int slot) {
     if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS || c.sweep_block <= 1) return 1;
-    return slot < 3 ? std::min(c.sweep_block, 2) : c.sweep_block;
+    return (slot < 3 || slot == ifl::kSlotT) ? std::min(c.sweep_block, 2) : c.sweep_block;
 }
 
+// Equation index of a smoother slot (relax / tol / sweep counts): u, v, w, p' (p' and its
+// refinement correction), T.
+int slotEq(int slot) { return slot < 3 ? slot : slot == ifl::kSlotT ? 4 : 3; }
+int slotSweeps(const ifl_config &c, int slot) { return slot == ifl::kSlotT ? c.n_iter_t : c.n_iter[slotEq(slot)]; }
+
 // Workgroup kind of a smoother launch and, for one case, its workgroup count.
-int sweepKind(const Plan &pl) {
+int sweepKind(const Plan &pl, int slot) {
     if (pl.cfg->smoother == IFL_SMOOTHER_JACOBI) return ifl::MAP_CELL;
-    return pl.dim == 2 ? ifl::MAP_TILE : ifl::MAP_ZMARCH;
+    if (pl.dim == 2) return ifl::MAP_TILE;
+    return (slot == 3 || slot == 4) ? ifl::MAP_ZMARCH_P : ifl::MAP_ZMARCH;
 }
-int sweepRecordsSingle(const Plan &pl) {
+int sweepRecordsSingle(const Plan &pl, int slot) {
     const ifl::CaseParams &P = *pl.L.P;
-    switch (sweepKind(pl)) {
+    switch (sweepKind(pl, slot)) {
         case ifl::MAP_CELL: return ifl::cell_blocks(P);
         case ifl::MAP_TILE: return ifl::tile_count(P.nx, P.ny, pl.L.tile);
-        default: return ifl::zmarch_blocks(P);
+        case ifl::MAP_ZMARCH_P: return ifl::zmarch_blocks(P, true);
+        default: return ifl::zmarch_blocks(P, false);
     }
 }
 
@@ -189,10 +214,11 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
     }
     int n[3], nL[3], flip[3], nLmax = 0;
     for (int k = 0; k < neq; ++k) {
-        const int q = slots[k] < 4 ? slots[k] : 3;
-        n[k] = c.n_iter[q];
+        n[k] = slotSweeps(c, slots[k]);
         nL[k] = (n[k] + K - 1) / K;
-        flip[k] = (md.fast && nL[k] % 2 == 1) ? 1 : 0;
+        // the temperature has no stage that keeps its partner buffer equal to it: it never starts
+        // there, and an odd launch count is copied back
+        flip[k] = (md.fast && nL[k] % 2 == 1 && slots[k] != ifl::kSlotT) ? 1 : 0;
         nLmax = std::max(nLmax, nL[k]);
     }
     for (int L = 0; L < nLmax; ++L) {
@@ -202,14 +228,15 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
             const int kk = std::min(K, n[k] - L * K);
             if (kk <= 0) continue;
             const int kp = L > 0 ? std::min(K, n[k] - (L - 1) * K) : 0;
-            se[m] = ifl::SweepSel{slots[k], slots[k] < 4 ? slots[k] : 3, kk, kp};
+            se[m] = ifl::SweepSel{slots[k], slotEq(slots[k]), kk, kp};
             se[m].flip = flip[k];
             se[m].accum = (md.accumLast && fused && L == nL[k] - 1) ? 1 : 0;
             ++m;
         }
         if (m == 1) se[1] = se[0];
         if (fused) {
-            const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM : ifl::COEF_PCORR;
+            const int kind = slots[0] < 3 ? ifl::COEF_MOMENTUM
+                             : slots[0] == ifl::kSlotT ? ifl::COEF_TEMP : ifl::COEF_PCORR;
             if (pl.dim == 2) ifl::launch_rb_sweep2d(pl.L, kind, se[0], se[1], m, L, K);
             else ifl::launch_rb_sweep3d(pl.L, kind, se, m, L);
         } else {
@@ -217,24 +244,25 @@ void enqueueSmoothers(const Plan &pl, const int *slots, int neq, const SmoothMod
         }
     }
     for (int k = 0; k < neq; ++k) {
-        if (!md.fast) {
+        if (!md.fast || slots[k] == ifl::kSlotT) {
             if (nL[k] % 2 == 1) ifl::launch_copy_back(pl.L, slots[k], 0);
             else if (slots[k] < 3 && nL[k] > 0) ifl::launch_copy_back(pl.L, slots[k], 1);
         }
         if (n[k] > 0) {
-            const int q = slots[k] < 4 ? slots[k] : 3;
+            const int q = slotEq(slots[k]);
             const int kLast = n[k] - (nL[k] - 1) * K;
             if (md.fast && md.pending) {
                 ifl::Finish &f = *md.pending;
-                f.kind = sweepKind(pl);
-                f.nprev = sweepRecordsSingle(pl);
+                f.kind = sweepKind(pl, slots[k]);
+                f.nprev = sweepRecordsSingle(pl, slots[k]);
                 f.slot[f.n] = slots[k];
                 f.q[f.n] = q;
                 f.nL[f.n] = nL[k];
                 f.kLast[f.n] = kLast;
                 ++f.n;
             } else {
-                ifl::launch_smooth_finish(pl.L, slots[k], q, nL[k], kLast, sweepKind(pl), sweepRecordsSingle(pl));
+                ifl::launch_smooth_finish(pl.L, slots[k], q, nL[k], kLast, sweepKind(pl, slots[k]),
+                                          sweepRecordsSingle(pl, slots[k]));
             }
         }
     }
@@ -247,6 +275,13 @@ void enqueueMomentumSmoothers(const Plan &pl, const SmoothMode &md = SmoothMode{
         return;
     }
     for (int q = 0; q < pl.dim; ++q) enqueueSmoothers(pl, &q, 1, md);   // :178, :180, :183
+}
+
+// The temperature sweeps (energy on), after the momentum sweeps on the same face velocities.
+void enqueueTemperatureSmoother(const Plan &pl, const SmoothMode &md = SmoothMode{}) {
+    if (!pl.cfg->energy) return;
+    const int slot = ifl::kSlotT;
+    enqueueSmoothers(pl, &slot, 1, md);
 }
 
 // Solver.cu:196 plus the FP64-defect refinement cycles.  Returns the calls still to be closed
@@ -277,6 +312,7 @@ void enqueueOuter(const Plan &pl) {
     md.fast = true;
     md.pending = &fm;
     enqueueMomentumSmoothers(pl, md);                                // :178-184
+    enqueueTemperatureSmoother(pl, md);
     ifl::launch_rhie_chow(pl.L, fm);                                 // :186-187
     ifl::launch_pcorr_assemble(pl.L);                                // :189-193
     const ifl::Finish fp = enqueuePressureCorrection(pl, true);      // :196
@@ -382,6 +418,7 @@ void tally(ifl_solver *h, SolveTally &t, const ifl_iter_record *recs, long long 
         const long long local = it - t.it0;
         if (t.history && (size_t)local < t.histCap) t.history[local] = r;
         for (int q = 0; q < 4; ++q) t.sweeps += (double)r.sweeps[q] * (double)h->P.N;
+        t.sweeps += (double)r.sweeps_t * (double)h->P.N;
         const bool last = (local == t.maxIter - 1) || r.converged || (it == b - 1 && doneAtEnd);
         if (logEvery > 0 && (local == 0 || (it + 1) % logEvery == 0 || last)) printLog(r, it, h->P.dim);
     }
@@ -394,7 +431,7 @@ float *fieldPtr(ifl_solver *h, int id) {
         case IFL_FIELD_V: return P.v;
         case IFL_FIELD_W: return P.w;
         case IFL_FIELD_P: return P.p;
-        case IFL_FIELD_TEMP: return h->temp;
+        case IFL_FIELD_TEMP: return P.temp;
         case IFL_FIELD_PCORR: return P.pc;
         case IFL_FIELD_UF: return P.uf;
         case IFL_FIELD_VF: return P.vf;
@@ -405,6 +442,8 @@ float *fieldPtr(ifl_solver *h, int id) {
         case IFL_FIELD_BV: return P.bv;
         case IFL_FIELD_BW: return P.bw;
         case IFL_FIELD_BP: return P.bp;
+        case IFL_FIELD_TCOEF: return P.tcoef;      // null unless energy is on
+        case IFL_FIELD_BT: return P.bt;
         default: return nullptr;
     }
 }
@@ -416,7 +455,8 @@ size_t fieldSize(const ifl_config &c, int id) {
         case IFL_FIELD_VF: return nx * (ny + 1) * nz;
         case IFL_FIELD_WF: return nx * ny * (nz + 1);
         case IFL_FIELD_MCOEF:
-        case IFL_FIELD_PCOEF: return n * (1 + 2 * c.dim);
+        case IFL_FIELD_PCOEF:
+        case IFL_FIELD_TCOEF: return n * (1 + 2 * c.dim);
         default: return (id >= 0 && id < IFL_FIELD_COUNT) ? n : 0;
     }
 }
@@ -449,6 +489,19 @@ void ifl_config_default(ifl_config *c) {
     c->tol[3] = 1e-3;
     c->tol_mass = 1e-9;
     c->init[4] = 273;   // Solver.h:46 default temp
+    // config.h:19-20, 45-63 (declared, not solved by the reference: energy stays 0)
+    c->conductivity = 1;
+    c->specific_heat = 1;
+    const int ttypes[6] = {IFL_TEMP_NEUMANN, IFL_TEMP_NEUMANN, IFL_TEMP_DIRICHLET, IFL_TEMP_DIRICHLET,
+                           IFL_TEMP_NEUMANN, IFL_TEMP_NEUMANN};
+    const double tvals[6] = {0.0, 0.0, 293.0, 373.0, 0.0, 0.0};
+    for (int f = 0; f < 6; ++f) {
+        c->temp_bc_type[f] = ttypes[f];
+        c->temp_bc_val[f] = tvals[f];
+    }
+    c->n_iter_t = 10;
+    c->relax_t = 0.75;
+    c->tol_t = 1e-5;
     c->smoother = IFL_SMOOTHER_RBGS;
     c->stop_rule = IFL_STOP_FAITHFUL;
     c->tol_residual = 1e-5;
@@ -472,10 +525,10 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     P.partStride = (int)pstride;
     // arena layout
     struct Part { void **dst; size_t bytes; };
-    float *temp = nullptr;
+    const size_t nT = cfg->energy ? N : 0;     // temperature matrix, source and partner: energy only
     std::vector<Part> parts = {
         {(void **)&P.u, N * 4}, {(void **)&P.v, N * 4}, {(void **)&P.w, N * 4}, {(void **)&P.p, N * 4},
-        {(void **)&temp, N * 4}, {(void **)&P.pc, N * 4},
+        {(void **)&P.temp, N * 4}, {(void **)&P.pc, N * 4},
         {(void **)&P.uf, (nx + 1) * ny * nz * 4}, {(void **)&P.vf, nx * (ny + 1) * nz * 4},
         {(void **)&P.wf, nx * ny * (nz + 1) * 4},
         {(void **)&P.mcoef, ncoef * 4}, {(void **)&P.pcoef, ncoef * 4},
@@ -483,8 +536,9 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
         {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
         {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
         {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
+        {(void **)&P.tcoef, nT * (1 + 2 * P.dim) * 4}, {(void **)&P.bt, nT * 4}, {(void **)&P.alt[5], nT * 4},
         {(void **)&P.spart, (size_t)ifl::kSlots * 2 * ifl::kMaxBlock * pstride * 8},
-        {(void **)&P.partMom, 3 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
+        {(void **)&P.partMom, 4 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
         {(void **)&P.st, sizeof(ifl::CaseState)},
         {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
     };
@@ -502,15 +556,14 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     }
     size_t off = 0;
     for (auto &p : parts) {
-        *p.dst = h->arena + off;
+        *p.dst = p.bytes ? h->arena + off : nullptr;
         off += align256(p.bytes);
     }
-    h->temp = temp;
     // Solver.cu:13-104: uniform initial u, v, w, p, temp; everything else zero.
     e = hipMemsetAsync(h->arena, 0, total, h->stream);
     std::vector<float> host(N);
     // u, v, w also into their smoothers' partner buffers, which stay equal to them
-    float *initDst[8] = {P.u, P.v, P.w, P.p, temp, P.alt[0], P.alt[1], P.alt[2]};
+    float *initDst[8] = {P.u, P.v, P.w, P.p, P.temp, P.alt[0], P.alt[1], P.alt[2]};
     const int initSrc[8] = {0, 1, 2, 3, 4, 0, 1, 2};
     for (int q = 0; q < 8 && e == hipSuccess; ++q) {
         std::fill(host.begin(), host.end(), (float)cfg->init[initSrc[q]]);
@@ -570,7 +623,7 @@ size_t ifl_field_size(const ifl_solver *h, int field) { return h ? fieldSize(h->
 int ifl_get_field(ifl_solver *h, int field, float *dst, size_t n) {
     if (!h || !dst) return fail(IFL_ERR_ARG, "null argument");
     float *src = fieldPtr(h, field);
-    if (!src) return fail(IFL_ERR_ARG, "bad field id");
+    if (!src) return fail(IFL_ERR_ARG, field == IFL_FIELD_TCOEF || field == IFL_FIELD_BT ? "field needs energy = 1" : "bad field id");
     if (n != fieldSize(h->cfg, field)) return fail(IFL_ERR_ARG, "size mismatch");
     IFL_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToHost, h->stream));
     IFL_HIP(hipStreamSynchronize(h->stream));
@@ -580,7 +633,7 @@ int ifl_get_field(ifl_solver *h, int field, float *dst, size_t n) {
 int ifl_set_field(ifl_solver *h, int field, const float *src, size_t n) {
     if (!h || !src) return fail(IFL_ERR_ARG, "null argument");
     float *dst = fieldPtr(h, field);
-    if (!dst) return fail(IFL_ERR_ARG, "bad field id");
+    if (!dst) return fail(IFL_ERR_ARG, field == IFL_FIELD_TCOEF || field == IFL_FIELD_BT ? "field needs energy = 1" : "bad field id");
     if (n != fieldSize(h->cfg, field)) return fail(IFL_ERR_ARG, "size mismatch");
     IFL_HIP(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, h->stream));
     if (field == IFL_FIELD_U || field == IFL_FIELD_V || field == IFL_FIELD_W)   // and the partner buffer
@@ -669,6 +722,10 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
             enqueueSmoothers(pl, &slot, 1);
             break;
         }
+        case IFL_STAGE_SMOOTH_T:
+            if (!P.energy) return fail(IFL_ERR_ARG, "SMOOTH_T needs energy = 1");
+            enqueueTemperatureSmoother(pl);
+            break;
         case IFL_STAGE_RHIE_CHOW: ifl::launch_rhie_chow(L); break;
         case IFL_STAGE_PCORR_ASSEMBLE:
             ifl::launch_pcorr_assemble(L);
@@ -690,7 +747,11 @@ int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {
     if (st != IFL_OK) return st;
     switch (stage) {
         case IFL_STAGE_MOM_RESIDUAL:
-            for (int q = 0; q < 3; ++q) tmp[q] = std::sqrt(s.resSq[q]);
+            for (int q = 0; q < 4; ++q) tmp[q] = std::sqrt(s.resSq[q]);
+            break;
+        case IFL_STAGE_SMOOTH_T:
+            tmp[0] = s.sm[ifl::kSlotT].sweeps;
+            tmp[1] = s.sm[ifl::kSlotT].lastSum;
             break;
         case IFL_STAGE_SMOOTH_U:
         case IFL_STAGE_SMOOTH_V:
@@ -748,6 +809,15 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
                 else ifl::launch_rb_sweep3d(L, ifl::COEF_MOMENTUM, se, 3, 0);
                 break;
             }
+            case IFL_KERNEL_TEMP_SWEEP: {
+                // one launch of the temperature smoother as the outer iteration runs it
+                const int K = sweepBlockOf(h->cfg, pl.dim, ifl::kSlotT);
+                const ifl::SweepSel e{ifl::kSlotT, 4, K, 0};
+                if (h->cfg.smoother == IFL_SMOOTHER_JACOBI) ifl::launch_sweep_jacobi(L, e, 0);
+                else if (pl.dim == 2) ifl::launch_rb_sweep2d(L, ifl::COEF_TEMP, e, e, 1, 0, K);
+                else ifl::launch_rb_sweep3d(L, ifl::COEF_TEMP, &e, 1, 0);
+                break;
+            }
             case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(L); break;
             case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(L); break;
             case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(L); break;
@@ -756,6 +826,7 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
         }
     };
     if (kernel < 0 || kernel >= IFL_KERNEL_COUNT) return fail(IFL_ERR_ARG, "bad kernel id");
+    if (kernel == IFL_KERNEL_TEMP_SWEEP && !h->P.energy) return fail(IFL_ERR_ARG, "the temperature sweep needs energy = 1");
     once();   // warm-up
     IFL_HIP(hipGetLastError());
     // one event pair per launch: the per-dispatch device duration, as a kernel trace reports it
@@ -830,7 +901,7 @@ namespace {
 
 constexpr int kStateFields[8] = {IFL_FIELD_U, IFL_FIELD_V, IFL_FIELD_W, IFL_FIELD_P,
                                  IFL_FIELD_TEMP, IFL_FIELD_UF, IFL_FIELD_VF, IFL_FIELD_WF};
-static_assert(sizeof(ifl_state_header) == 152, "ifl_state_header layout");
+static_assert(sizeof(ifl_state_header) == 160, "ifl_state_header layout");
 
 uint64_t fnv1a(const unsigned char *p, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
     for (size_t i = 0; i < n; ++i) {
@@ -871,7 +942,7 @@ int ifl_save_state(ifl_solver *h, const char *path) {
     hd.iter = s.iter;
     const double nm[7] = {s.pNormMax, s.uNormMax, s.vNormMax, s.wNormMax, s.ufNormMax, s.vfNormMax, s.wfNormMax};
     for (int q = 0; q < 7; ++q) hd.norm_max[q] = nm[q];
-    for (int q = 0; q < 4; ++q) hd.res_max[q] = s.resMax[q];
+    for (int q = 0; q < 5; ++q) hd.res_max[q] = s.resMax[q];
     hd.last_scaled = s.lastScaled;
     hd.payload_bytes = data.size() * 4;
     hd.checksum = fnv1a(reinterpret_cast<const unsigned char *>(data.data()), data.size() * 4);
@@ -914,7 +985,7 @@ int ifl_load_state(ifl_solver *h, const char *path) {
     s.lastConverged = hd.last_converged;
     double *nm[7] = {&s.pNormMax, &s.uNormMax, &s.vNormMax, &s.wNormMax, &s.ufNormMax, &s.vfNormMax, &s.wfNormMax};
     for (int q = 0; q < 7; ++q) *nm[q] = hd.norm_max[q];
-    for (int q = 0; q < 4; ++q) s.resMax[q] = hd.res_max[q];
+    for (int q = 0; q < 5; ++q) s.resMax[q] = hd.res_max[q];
     s.lastScaled = hd.last_scaled;
     s.done = 0;
     size_t off = 0;
@@ -1353,11 +1424,13 @@ int batchValidate(const ifl_config *c, int n) {
         if (st != IFL_OK) return fail(st, "case " + std::to_string(i) + ": " + g_err);
         const ifl_config &a = c[0], &b = c[i];
         bool same = a.dim == b.dim && a.smoother == b.smoother && a.p_refine == b.p_refine
-                    && a.sweep_block == b.sweep_block && a.tile == b.tile;
+                    && a.sweep_block == b.sweep_block && a.tile == b.tile && a.energy == b.energy
+                    && (!a.energy || a.n_iter_t == b.n_iter_t);
         for (int q = 0; q < 4; ++q) same = same && a.n_iter[q] == b.n_iter[q];
         if (!same)
             return fail(IFL_ERR_ARG, "case " + std::to_string(i)
-                                         + ": dim, smoother, n_iter, p_refine, sweep_block and tile must agree across a batch");
+                                         + ": dim, smoother, n_iter, p_refine, sweep_block, tile, energy and n_iter_t "
+                                           "must agree across a batch");
     }
     return IFL_OK;
 }
@@ -1393,7 +1466,8 @@ int batchBuild(ifl_batch *b, const std::vector<int> &ids) {
                 case ifl::MAP_CELL: tot += ifl::cell_blocks(P); break;
                 case ifl::MAP_TILE: tot += P.dim == 2 ? ifl::tile_count(P.nx, P.ny, b->tile ? b->tile : ifl::TILE_64x16) : 0; break;
                 case ifl::MAP_COPY: tot += ifl::copy_blocks(P); break;
-                case ifl::MAP_ZMARCH: tot += P.dim == 3 ? ifl::zmarch_blocks(P) : 0; break;
+                case ifl::MAP_ZMARCH: tot += P.dim == 3 ? ifl::zmarch_blocks(P, false) : 0; break;
+                case ifl::MAP_ZMARCH_P: tot += P.dim == 3 ? ifl::zmarch_blocks(P, true) : 0; break;
                 default: tot += 1; break;
             }
         }
@@ -1433,7 +1507,8 @@ int batchBuild(ifl_batch *b, const std::vector<int> &ids) {
                     dx = P.dim == 2 ? ifl::tile_dimx(P.nx, L.tile) : 1;
                     break;
                 case ifl::MAP_COPY: cnt = ifl::copy_blocks(P); dx = cnt; break;
-                case ifl::MAP_ZMARCH: cnt = P.dim == 3 ? ifl::zmarch_blocks(P) : 0; dx = ifl::zmarch_dimx(P); break;
+                case ifl::MAP_ZMARCH: cnt = P.dim == 3 ? ifl::zmarch_blocks(P, false) : 0; dx = ifl::zmarch_dimx(P, false); break;
+                case ifl::MAP_ZMARCH_P: cnt = P.dim == 3 ? ifl::zmarch_blocks(P, true) : 0; dx = ifl::zmarch_dimx(P, true); break;
                 default: break;
             }
             start[c] = (int)acc;

@@ -22,7 +22,8 @@ namespace ifl {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kSlots = 5;              // smoother call slots: u, v, w, p', p' refinement
+constexpr int kSlots = 6;              // smoother call slots: u, v, w, p', p' refinement, temperature
+constexpr int kSlotT = 5;              // the temperature slot (its equation index is 4)
 constexpr int kRows = 4;               // rows per thread in the cell kernels (64 x 16 cells per workgroup)
 
 enum { E = 0, W = 1, N = 2, S = 3, T = 4, B = 5 };                // constants.h:7-12
@@ -45,12 +46,12 @@ struct CaseState {
     int pad0;
     long long iter;         // outer iterations executed
     SmoothState sm[kSlots];
-    int sweepsIter[4];      // sweeps executed in the current outer iteration (u, v, w, p')
+    int sweepsIter[5];      // sweeps executed in the current outer iteration (u, v, w, p', T)
     double mass;            // sum b'^2 (calcPresCorrSrcTerm)
-    double resSq[3];        // sum (b - A phi)^2 of u, v, w before the sweeps
+    double resSq[4];        // sum (b - A phi)^2 of u, v, w and T before the sweeps
     double upd[7];          // sums of squared increments p,u,v,w,uf,vf,wf (updateField)
     double pNormMax, uNormMax, vNormMax, wNormMax, ufNormMax, vfNormMax, wfNormMax;
-    double resMax[4];
+    double resMax[5];       // scaled-residual normalisers: continuity, u, v, w, T
     double lastScaled;
     int lastConverged;
     int pad;
@@ -64,15 +65,20 @@ struct CaseParams {
     int bct[6];
     float bcv[6][3];
     const float *prof[6];
-    float relax[4];
-    double tol[4];
+    float relax[5];          // u, v, w, p', T
+    double tol[5];
     double tolMass, tolRes;
     int stopRule;
     int histCap;
-    float *u, *v, *w, *p, *pc, *uf, *vf, *wf;
+    int energy;              // the temperature equation is solved (its buffers below exist)
+    float gam;               // k / c_p: the diffusion coefficient of the temperature equation
+    int tbct[6];             // temperature BC per face: IFL_TEMP_DIRICHLET / IFL_TEMP_NEUMANN
+    float tbcv[6];           // Dirichlet values
+    float *tcoef, *bt;       // temperature matrix ((1+2dim) planes) and source
+    float *u, *v, *w, *p, *pc, *uf, *vf, *wf, *temp;
     float *mcoef, *pcoef, *bu, *bv, *bw, *bp;
     float *tmp, *tmp2;       // p' defect and refinement correction
-    float *alt[5];           // ping-pong partners of u, v, w, p', refinement correction
+    float *alt[6];           // ping-pong partners of u, v, w, p', refinement correction, T
     float *rinv;             // 1/aC of the momentum matrix (p' coefficients are rebuilt from it)
     double *spart;           // sweep partials: [slot][launch parity][sweep in launch][partStride]
     double *partMom;         // 3 per cell workgroup: momentum residual sums
@@ -167,7 +173,8 @@ __device__ __forceinline__ int caseDone(const CaseParams &P) { return P.st->done
 // case's own) or on a batch: the grid is the concatenation of every case's workgroups, and a
 // workgroup finds its case in a per-workgroup table.  Either way the kernel body sees the case's
 // parameters and the workgroup's position inside that case's own grid.
-enum { MAP_CELL = 0, MAP_TILE = 1, MAP_ONE = 2, MAP_COPY = 3, MAP_ZMARCH = 4, MAP_KINDS = 5 };
+// MAP_ZMARCH: 3D momentum/temperature z-march columns; MAP_ZMARCH_P: 3D p' columns (own shape)
+enum { MAP_CELL = 0, MAP_TILE = 1, MAP_ONE = 2, MAP_COPY = 3, MAP_ZMARCH = 4, MAP_ZMARCH_P = 5, MAP_KINDS = 6 };
 
 struct BatchMap {
     int ncase;
@@ -224,20 +231,21 @@ __device__ __forceinline__ int kindCount(const BatchMap &M, int kind, const Case
 
 __device__ __forceinline__ bool leader(const Blk &b) { return b.flat == 0 && threadIdx.x == 0; }
 
-// The field a smoother slot works on (u, v, w, p', refinement correction) and its source.
+// The field a smoother slot works on (u, v, w, p', refinement correction, T) and its source.
 __device__ __forceinline__ float *slotField(const CaseParams &P, int slot) {
-    return slot == 0 ? P.u : slot == 1 ? P.v : slot == 2 ? P.w : slot == 3 ? P.pc : P.tmp2;
+    return slot == 0 ? P.u : slot == 1 ? P.v : slot == 2 ? P.w : slot == 3 ? P.pc : slot == 4 ? P.tmp2 : P.temp;
 }
 // The slot's ping-pong partner (a select chain: indexing the kernel-argument array with a
 // runtime value would copy the whole argument block to scratch memory).
 __device__ __forceinline__ float *slotAlt(const CaseParams &P, int slot) {
-    return slot == 0 ? P.alt[0] : slot == 1 ? P.alt[1] : slot == 2 ? P.alt[2] : slot == 3 ? P.alt[3] : P.alt[4];
+    return slot == 0 ? P.alt[0] : slot == 1 ? P.alt[1] : slot == 2 ? P.alt[2] : slot == 3 ? P.alt[3]
+         : slot == 4 ? P.alt[4] : P.alt[5];
 }
 __device__ __forceinline__ const float *slotSource(const CaseParams &P, int slot) {
-    return slot == 0 ? P.bu : slot == 1 ? P.bv : slot == 2 ? P.bw : slot == 3 ? P.bp : P.tmp;
+    return slot == 0 ? P.bu : slot == 1 ? P.bv : slot == 2 ? P.bw : slot == 3 ? P.bp : slot == 4 ? P.tmp : P.bt;
 }
 __device__ __forceinline__ const float *slotCoef(const CaseParams &P, int slot) {
-    return slot < 3 ? P.mcoef : P.pcoef;
+    return slot < 3 ? P.mcoef : slot == kSlotT ? P.tcoef : P.pcoef;
 }
 
 // Sweep partials: [slot][launch parity][sweep in the launch, < kMaxBlock][partStride].
@@ -310,10 +318,10 @@ struct Finish {
     int n = 0;
     int kind = 0;      // workgroup kind of the calls' launches (their partial counts)
     int nprev = 0;     // their workgroup count (one case)
-    int slot[3] = {0, 0, 0};
-    int q[3] = {0, 0, 0};
-    int nL[3] = {0, 0, 0};
-    int kLast[3] = {0, 0, 0};
+    int slot[4] = {0, 0, 0, 0};
+    int q[4] = {0, 0, 0, 0};
+    int nL[4] = {0, 0, 0, 0};
+    int kLast[4] = {0, 0, 0, 0};
 };
 
 __device__ __forceinline__ bool launchRuns(const CaseParams &P, const Blk &b, int slot, int L, int kPrev, int nprev,

@@ -18,7 +18,7 @@ struct SweepSel {
     int accum = 0;   // 1: also add the result into p' (last launch of a refinement correction)
 };
 
-enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2 };
+enum { COEF_STORED = 0, COEF_MOMENTUM = 1, COEF_PCORR = 2, COEF_TEMP = 3 };
 // 2D sweep tiles; 128x32 runs 1024-thread workgroups, the others 256
 enum { TILE_AUTO = 0, TILE_64x16 = 1, TILE_64x8 = 2, TILE_32x8 = 3, TILE_32x4 = 4, TILE_128x32 = 5 };
 enum { RED_MOM = 1, RED_MASS = 2, RED_UPD = 4, RED_FINISH = 8 };
@@ -28,7 +28,7 @@ struct Lc {
     const CaseParams *P = nullptr;   // the case (batch: any case, only its dim/tile are read)
     BatchMap M{};
     bool batch = false;
-    int total[MAP_KINDS] = {0, 0, 0, 0, 0};
+    int total[MAP_KINDS] = {};
     int tile = TILE_64x16;           // 2D sweep tile
     hipStream_t s = nullptr;
 };
@@ -38,8 +38,8 @@ int copy_blocks(const CaseParams &P);
 int tile_count(int nx, int ny, int tile);
 int tile_dimx(int nx, int tile);
 int default_tile(int nx, int ny);
-int zmarch_blocks(const CaseParams &P);
-int zmarch_dimx(const CaseParams &P);
+int zmarch_blocks(const CaseParams &P, bool pc);   // pc: the p' sweep's column shape
+int zmarch_dimx(const CaseParams &P, bool pc);
 int cell_dimx(const CaseParams &P);
 
 void launch_init_faces(const Lc &L);

@@ -6,12 +6,13 @@
 
 namespace ifl {
 
-// Momentum link coefficients from the six face velocities, calcMomLinkCoefKernel
-// (kernels.cuh:116-131; T3 fixed: aT/aB in their own planes).
+// Link coefficients of a transported quantity with diffusion coefficient mu from the six face
+// velocities: calcMomLinkCoefKernel (kernels.cuh:116-131; T3 fixed: aT/aB in their own planes)
+// with mu = P.mu for momentum and mu = P.gam (k / c_p) for temperature.
 template <int DIM>
-__device__ __forceinline__ void momLinkCoef(const CaseParams &P, float ue, float uw, float vn, float vs, float wt,
-                                            float wb, float (&a)[7]) {
-    const float rho = P.rho, mu = P.mu;
+__device__ __forceinline__ void linkCoef(const CaseParams &P, float mu, float ue, float uw, float vn, float vs,
+                                         float wt, float wb, float (&a)[7]) {
+    const float rho = P.rho;
     a[sE] = (rho * min0(ue) * P.Ax) - mu * P.Ax / P.dx;
     a[sW] = (rho * min0(-uw) * P.Ax) - mu * P.Ax / P.dx;
     a[sN] = (rho * min0(vn) * P.Ay) - mu * P.Ay / P.dy;
@@ -26,6 +27,12 @@ __device__ __forceinline__ void momLinkCoef(const CaseParams &P, float ue, float
     }
 }
 
+template <int DIM>
+__device__ __forceinline__ void momLinkCoef(const CaseParams &P, float ue, float uw, float vn, float vs, float wt,
+                                            float wb, float (&a)[7]) {
+    linkCoef<DIM>(P, P.mu, ue, uw, vn, vs, wt, wb, a);
+}
+
 __device__ __forceinline__ int bcExSlot(int f) {
     return f == E ? sE : f == W ? sW : f == N ? sN : f == S ? sS : f == T ? sT : sB;
 }
@@ -33,17 +40,37 @@ __device__ __forceinline__ int bcInSlot(int f) {
     return f == E ? sW : f == W ? sE : f == N ? sS : f == S ? sN : f == T ? sB : sT;
 }
 
-// Coefficient part of the boundary row of face f, applyBCsToMomEqKernel (kernels.cuh:304-331).
-__device__ __forceinline__ void bcRowCoef(const CaseParams &P, int f, float (&a)[7]) {
+// Coefficient part of the boundary row of face f, applyBCsToMomEqKernel (kernels.cuh:304-331):
+// a face with a prescribed value (velocity wall/inlet, temperature Dirichlet) or a zero-gradient
+// face (velocity outlet, temperature Neumann).
+__device__ __forceinline__ void bcRowCoefKind(bool given, int f, float (&a)[7]) {
     const int ex = bcExSlot(f), in = bcInSlot(f);
     const float aEx = a[ex];
-    if (ghostCopy(P, f)) {          // wall / inlet: 2nd-order one-sided wall gradient
+    if (given) {                    // wall / inlet: 2nd-order one-sided wall gradient
         a[sC] -= 2 * aEx;
         a[in] += aEx / 3;
         a[ex] = 0;
     } else {                        // outlet: zero gradient
         a[sC] += aEx;
         a[ex] = 0;
+    }
+}
+
+__device__ __forceinline__ void bcRowCoef(const CaseParams &P, int f, float (&a)[7]) {
+    bcRowCoefKind(ghostCopy(P, f), f, a);
+}
+__device__ __forceinline__ bool tempDirichlet(const CaseParams &P, int f) { return P.tbct[f] == IFL_TEMP_DIRICHLET; }
+
+// Boundary rows of the temperature equation of cell (i,j,k), in the same order.
+template <int DIM>
+__device__ __forceinline__ void tempBcRowsCoef(const CaseParams &P, int i, int j, int k, float (&a)[7]) {
+    if (i == P.nx - 1) bcRowCoefKind(tempDirichlet(P, E), E, a);
+    if (i == 0) bcRowCoefKind(tempDirichlet(P, W), W, a);
+    if (j == P.ny - 1) bcRowCoefKind(tempDirichlet(P, N), N, a);
+    if (j == 0) bcRowCoefKind(tempDirichlet(P, S), S, a);
+    if (DIM == 3) {
+        if (k == P.nz - 1) bcRowCoefKind(tempDirichlet(P, T), T, a);
+        if (k == 0) bcRowCoefKind(tempDirichlet(P, B), B, a);
     }
 }
 

@@ -116,14 +116,23 @@ __device__ __forceinline__ double momResidual(const CaseParams &P, const float *
     return r;
 }
 
+// Boundary row of face f of the temperature equation: the momentum rows with the face's
+// temperature BC (Dirichlet value, or zero gradient).
+template <int DIM>
+__device__ __forceinline__ void tempBcRow(const CaseParams &P, int f, float (&a)[7], float &b) {
+    const bool given = tempDirichlet(P, f);
+    if (given) b -= (float)8 / 3 * a[bcExSlot(f)] * P.tbcv[f];
+    bcRowCoefKind(given, f, a);
+}
+
 template <int DIM, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap M) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
     if (leader(blk))
-        for (int q = 0; q < 4; ++q) P.st->sweepsIter[q] = 0;   // a new outer iteration starts
-    double rs[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < 5; ++q) P.st->sweepsIter[q] = 0;   // a new outer iteration starts
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
     forCells(P, blk, [&](int i, int j, int k, int c) {
         const int nx = P.nx, ny = P.ny, nz = P.nz;
         applyFaceBCs<DIM>(P, i, j, k, c);
@@ -162,8 +171,26 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
             const double r2 = momResidual<DIM>(P, P.w, a, b[2], i, j, k, c);
             rs[2] += r2 * r2;
         }
+        if (P.energy) {      // the temperature equation on the same face velocities
+            float t[7];
+            linkCoef<DIM>(P, P.gam, ue, uw, vn, vs, wt, wb, t);
+            float bt = 0.0f;
+            if (i == nx - 1) tempBcRow<DIM>(P, E, t, bt);
+            if (i == 0) tempBcRow<DIM>(P, W, t, bt);
+            if (j == ny - 1) tempBcRow<DIM>(P, N, t, bt);
+            if (j == 0) tempBcRow<DIM>(P, S, t, bt);
+            if (DIM == 3) {
+                if (k == nz - 1) tempBcRow<DIM>(P, T, t, bt);
+                if (k == 0) tempBcRow<DIM>(P, B, t, bt);
+            }
+#pragma unroll
+            for (int s = 0; s < 1 + 2 * DIM; ++s) P.tcoef[c + P.N * s] = t[s];
+            P.bt[c] = bt;
+            const double r3 = momResidual<DIM>(P, P.temp, t, bt, i, j, k, c);
+            rs[3] += r3 * r3;
+        }
     });
-    storeBlockSums<3>(rs, P.partMom + 3 * blk.flat);
+    storeBlockSums<4>(rs, P.partMom + 4 * blk.flat);
 }
 
 // ---------------- one relaxed Gauss-Seidel/Jacobi cell update ----------------
@@ -453,12 +480,14 @@ __device__ void finishOuter(const CaseParams &P) {
     rec.res_u = sqrt(S.resSq[0]);
     rec.res_v = sqrt(S.resSq[1]);
     rec.res_w = sqrt(S.resSq[2]);
-    const double rs[4] = {rec.res_cont, rec.res_u, rec.res_v, rec.res_w};
+    rec.res_t = sqrt(S.resSq[3]);
+    const double rs[5] = {rec.res_cont, rec.res_u, rec.res_v, rec.res_w, rec.res_t};
     if (S.iter < 5)
-        for (int q = 0; q < 4; ++q) S.resMax[q] = fmax(S.resMax[q], rs[q]);
+        for (int q = 0; q < 5; ++q) S.resMax[q] = fmax(S.resMax[q], rs[q]);
     double sc = 0.0;
-    for (int q = 0; q < (P.dim == 3 ? 4 : 3); ++q)
-        if (S.resMax[q] > 0) sc = fmax(sc, rs[q] / S.resMax[q]);
+    for (int q = 0; q < 5; ++q)
+        if ((q < 3 || (q == 3 && P.dim == 3) || (q == 4 && P.energy)) && S.resMax[q] > 0)
+            sc = fmax(sc, rs[q] / S.resMax[q]);
     rec.scaled = sc;
     S.pNormMax = fmax(S.pNormMax, rec.p_norm);
     if (P.stopRule == IFL_STOP_FAITHFUL) {
@@ -493,13 +522,13 @@ __device__ void finishOuter(const CaseParams &P) {
         conv = sc < P.tolRes;
     }
     for (int q = 0; q < 4; ++q) rec.sweeps[q] = S.sweepsIter[q];
+    rec.sweeps_t = S.sweepsIter[4];
     rec.converged = conv ? 1 : 0;
-    rec.pad = 0;
     S.lastScaled = sc;
     S.lastConverged = rec.converged;
     if (P.hist) P.hist[S.iter % P.histCap] = rec;
     const double chk[] = {rec.p_norm, rec.u_norm, rec.v_norm, rec.w_norm, rec.uf_norm, rec.vf_norm,
-                          rec.wf_norm, rec.mass_norm, rec.res_cont, rec.res_u, rec.res_v, rec.res_w};
+                          rec.wf_norm, rec.mass_norm, rec.res_cont, rec.res_u, rec.res_v, rec.res_w, rec.res_t};
     bool finite = true;
     for (double x : chk) finite = finite && isfinite(x);
     S.iter += 1;
@@ -517,10 +546,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P0, BatchMap
     const int nblk = kindCount<BATCH>(M, MAP_CELL, P, nSingle);
     CaseState &S = *P.st;
     if (mask & RED_MOM) {
-        double t[3];
-        sumPartials<3>(P.partMom, nblk, t);
+        double t[4];
+        sumPartials<4>(P.partMom, nblk, t);
         if (threadIdx.x == 0)
-            for (int q = 0; q < 3; ++q) S.resSq[q] = t[q];
+            for (int q = 0; q < 4; ++q) S.resSq[q] = t[q];
     }
     if (mask & RED_MASS) {
         double t[1];

@@ -12,6 +12,8 @@
 //   * k_rb_tile2d: the momentum equations.  Coefficients are recomputed from the four face
 //     velocities and the boundary rows (the functions of the assembly kernel, ifl_stencil.hpp),
 //     one set for u and v, which are swept together in one launch since they share the matrix.
+//     The temperature equation (COEF_TEMP) runs through the same kernel with its own diffusion
+//     coefficient and boundary rows.
 //   * k_rb_pc2d: the p' system, with the coefficients towards +x and +y of the stored p' matrix
 //     (the coefficient towards -x of a cell is the +x one of its west neighbour, bit for bit).
 // Every global load of the tile is issued first; the stop decision, which sums the previous
@@ -33,7 +35,8 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
     constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0, SX2 = SX / 2;   // cells ever updated (sources)
-    constexpr bool MOM = KIND == COEF_MOMENTUM;
+    // the momentum and temperature matrices are rebuilt from the face velocities
+    constexpr bool MOM = KIND == COEF_MOMENTUM || KIND == COEF_TEMP;
     static_assert(MOM, "the p' system has its own kernel (k_rb_pc2d)");
     constexpr int UX = SX + 1, UY = SY;                        // uf faces of the updated cells
     constexpr int VX = SX, VY = SY + 1;                        // vf faces of the updated cells
@@ -180,8 +183,13 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
             const int sx = gx - x0 + R0, sy = gy - y0 + R0;
             const float uw = suf[sy][sx], ue = suf[sy][sx + 1];
             const float vs = svf[sy][sx], vn = svf[sy + 1][sx];
-            momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
-            momBcRowsCoef<2>(P, gx, gy, 0, a);
+            if constexpr (KIND == COEF_TEMP) {
+                linkCoef<2>(P, P.gam, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                tempBcRowsCoef<2>(P, gx, gy, 0, a);
+            } else {
+                momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                momBcRowsCoef<2>(P, gx, gy, 0, a);
+            }
         }
         const bool owned = gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY;
         const int j = p >> 1;
@@ -505,7 +513,8 @@ static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel
         return;
     }
     if constexpr (K <= 2) {
-        if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+        if (kind == COEF_TEMP) k_rb_tile2d<TX, TY, K, 1, COEF_TEMP, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+        else if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
         else k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
     } else {
         (void)neq;

@@ -24,7 +24,13 @@
 
 namespace ifl {
 
-constexpr int Z_TX = 128, Z_TY = 8, Z_ZC = 32;   // 256^3: 8.11 ms per outer iteration, against 9.27 ms at 64x8x16
+// Column shapes, measured per launch on the 256^3 cavity (us; p' sweep / u+v+w sweep):
+//   128x8x32 162 / 310, 128x8x16 173 / 334, 64x8x32 149 / 461, 64x8x16 154 / 393,
+//   128x16x16 239 / 448, 256x4x32 186 / 522, 256x8x16 253 / 479.
+// The momentum (and temperature) sweep moves three fields and seven coefficient planes per cell
+// and prefers wide columns; the p' sweep (one field, three planes) prefers more workgroups.
+constexpr int Z_TX = 128, Z_TY = 8, Z_ZC = 32;     // momentum, temperature
+constexpr int ZP_TX = 64, ZP_TY = 8, ZP_ZC = 32;   // p'
 
 template <int TX, int TY, int ZC, int NEQ, bool PC, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1,
@@ -38,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     static_assert(!PC || NEQ == 1, "one p' equation per launch");
     __shared__ float win[NEQ][4][HY][HX];                    // plane z in slot z & 3
     Blk blk;
-    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH, blk);
+    const CaseParams &P = caseOf<BATCH>(P0, M, PC ? MAP_ZMARCH_P : MAP_ZMARCH, blk);
     if (caseDone(P)) return;
     const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N, nxy = nx * ny;
     const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
         esrc[q] = slotSource(P, eqs[q].slot);
         erlx[q] = P.relax[eqs[q].q];
     }
-    const float *coef = PC ? P.pcoef : P.mcoef;
+    const float *coef = PC ? P.pcoef : slotCoef(P, e0.slot);    // momentum or temperature planes
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
     const int z1 = min(z0 + ZC, nz);
     const int tid = threadIdx.x;
@@ -272,18 +278,22 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     }
 }
 
-int zmarch_blocks(const CaseParams &P) {
+int zmarch_blocks(const CaseParams &P, bool pc) {
+    if (pc) return ((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY) * ((P.nz + ZP_ZC - 1) / ZP_ZC);
     return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
 }
 
-int zmarch_dimx(const CaseParams &P) { return (P.nx + Z_TX - 1) / Z_TX; }
+int zmarch_dimx(const CaseParams &P, bool pc) { return pc ? (P.nx + ZP_TX - 1) / ZP_TX : (P.nx + Z_TX - 1) / Z_TX; }
 
 template <int NEQ, bool PC, bool BATCH>
 static void launchZ(const Lc &L, const SweepSel *e, int launch) {
     const CaseParams &P = *L.P;
-    const dim3 grid = BATCH ? dim3((unsigned)L.total[MAP_ZMARCH]) : dim3((unsigned)zmarch_blocks(P));
-    k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
-                                                                          e[NEQ > 2 ? 2 : 0], launch);
+    const dim3 grid = BATCH ? dim3((unsigned)L.total[PC ? MAP_ZMARCH_P : MAP_ZMARCH]) : dim3((unsigned)zmarch_blocks(P, PC));
+    if constexpr (PC)
+        k_rb_zmarch3d<ZP_TX, ZP_TY, ZP_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[0], e[0], launch);
+    else
+        k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
+                                                                              e[NEQ > 2 ? 2 : 0], launch);
 }
 
 template <bool BATCH>

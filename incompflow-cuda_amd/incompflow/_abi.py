@@ -6,7 +6,7 @@ Field-for-field correspondence with the reference's compile-time contract:
 """
 import ctypes as C
 
-IFL_ABI_VERSION = 1
+IFL_ABI_VERSION = 2
 
 # status codes
 IFL_OK, IFL_ERR_ARG, IFL_ERR_DEVICE, IFL_ERR_DIVERGED, IFL_ERR_NOT_CONVERGED, IFL_ERR_IO, IFL_ERR_NOMEM = range(7)
@@ -17,18 +17,20 @@ EAST, WEST, NORTH, SOUTH, TOP, BOTTOM = range(6)
 SLOT_C, SLOT_E, SLOT_W, SLOT_N, SLOT_S, SLOT_T, SLOT_B = range(7)
 # BC codes (constants.h:34-36)
 WALL, INLET, OUTLET = range(3)
+# temperature BC codes (config.h:46-53)
+TEMP_DIRICHLET, TEMP_NEUMANN = range(2)
 
 SMOOTHER_RBGS, SMOOTHER_JACOBI, SMOOTHER_LEX = range(3)
 STOP_NONE, STOP_FAITHFUL, STOP_FIXED, STOP_SCALED = range(4)
 
 (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_PCORR, FIELD_UF, FIELD_VF, FIELD_WF,
- FIELD_MCOEF, FIELD_PCOEF, FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP) = range(15)
+ FIELD_MCOEF, FIELD_PCOEF, FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP, FIELD_TCOEF, FIELD_BT) = range(17)
 FIELD_NAMES = ["u", "v", "w", "p", "temp", "pcorr", "uf", "vf", "wf", "mcoef", "pcoef",
-               "bu", "bv", "bw", "bp"]
+               "bu", "bv", "bw", "bp", "tcoef", "bt"]
 
 (STAGE_INIT_FACES, STAGE_FACE_BC, STAGE_MOM_ASSEMBLE, STAGE_MOM_RESIDUAL, STAGE_SMOOTH_U,
  STAGE_SMOOTH_V, STAGE_SMOOTH_W, STAGE_RHIE_CHOW, STAGE_PCORR_ASSEMBLE, STAGE_SMOOTH_P,
- STAGE_UPDATE, STAGE_OUTER) = range(12)
+ STAGE_UPDATE, STAGE_OUTER, STAGE_SMOOTH_T) = range(13)
 
 TILE_AUTO, TILE_64x16, TILE_64x8, TILE_32x8, TILE_32x4, TILE_128x32 = range(6)
 _TILE_SHAPE = {TILE_64x16: (64, 16), TILE_64x8: (64, 8), TILE_32x8: (32, 8), TILE_32x4: (32, 4), TILE_128x32: (128, 32)}
@@ -51,7 +53,8 @@ def default_tile(nx, ny):
         return TILE_32x8
     return TILE_32x4
 
-(KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE, KERNEL_MOM_SWEEP) = range(6)
+(KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE, KERNEL_MOM_SWEEP,
+ KERNEL_TEMP_SWEEP) = range(7)
 
 
 class Config(C.Structure):
@@ -79,7 +82,15 @@ class Config(C.Structure):
         ("check_every", C.c_int32),
         ("tile", C.c_int32),
         ("sweep_block", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("energy", C.c_int32),
+        ("n_iter_t", C.c_int32),
+        ("conductivity", C.c_double),
+        ("specific_heat", C.c_double),
+        ("relax_t", C.c_double),
+        ("tol_t", C.c_double),
+        ("temp_bc_type", C.c_int32 * 6),
+        ("temp_bc_val", C.c_double * 6),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def copy(self):
@@ -91,7 +102,7 @@ class Config(C.Structure):
         nx, ny, nz, dim = self.nx, self.ny, self.nz, self.dim
         n = nx * ny * nz
         if field in (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_PCORR,
-                     FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP):
+                     FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP, FIELD_BT):
             return n
         if field == FIELD_UF:
             return (nx + 1) * ny * nz
@@ -99,7 +110,7 @@ class Config(C.Structure):
             return nx * (ny + 1) * nz
         if field == FIELD_WF:
             return nx * ny * (nz + 1)
-        if field in (FIELD_MCOEF, FIELD_PCOEF):
+        if field in (FIELD_MCOEF, FIELD_PCOEF, FIELD_TCOEF):
             return n * (1 + 2 * dim)
         raise ValueError(field)
 
@@ -112,7 +123,7 @@ class Config(C.Structure):
             return (nz, ny + 1, nx)
         if field == FIELD_WF:
             return (nz + 1, ny, nx)
-        if field in (FIELD_MCOEF, FIELD_PCOEF):
+        if field in (FIELD_MCOEF, FIELD_PCOEF, FIELD_TCOEF):
             return (1 + 2 * dim, nz, ny, nx)
         return (nz, ny, nx)
 
@@ -122,7 +133,7 @@ class IterRecord(C.Structure):
         "p_norm", "u_norm", "v_norm", "w_norm", "uf_norm", "vf_norm", "wf_norm", "mass_norm",
         "res_cont", "res_u", "res_v", "res_w", "scaled",
         "rel_p", "rel_u", "rel_v", "rel_w", "rel_uf", "rel_vf", "rel_wf")] + [
-        ("sweeps", C.c_int32 * 4), ("converged", C.c_int32), ("pad", C.c_int32)]
+        ("sweeps", C.c_int32 * 4), ("converged", C.c_int32), ("sweeps_t", C.c_int32), ("res_t", C.c_double)]
 
 
 class Report(C.Structure):
@@ -176,6 +187,13 @@ def default_config():
     c.p_refine = 0
     c.log_every = 0
     c.check_every = 0
+    # config.h:19-20, 45-63: thermal properties and TempBCs (declared, not solved by the reference)
+    c.energy = 0
+    c.n_iter_t, c.relax_t, c.tol_t = 10, 0.75, 1e-5
+    c.conductivity, c.specific_heat = 1.0, 1.0
+    for f, (t, v) in enumerate(((TEMP_NEUMANN, 0.0), (TEMP_NEUMANN, 0.0), (TEMP_DIRICHLET, 293.0),
+                                (TEMP_DIRICHLET, 373.0), (TEMP_NEUMANN, 0.0), (TEMP_NEUMANN, 0.0))):
+        c.temp_bc_type[f], c.temp_bc_val[f] = t, v
     return c
 
 
@@ -315,7 +333,7 @@ STATE_FIELDS = (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_UF, FIELD_
 class StateHeader(C.LittleEndianStructure):
     _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("dim", C.c_int32), ("nx", C.c_int32),
                 ("ny", C.c_int32), ("nz", C.c_int32), ("last_converged", C.c_int32), ("iter", C.c_int64),
-                ("norm_max", C.c_double * 7), ("res_max", C.c_double * 4), ("last_scaled", C.c_double),
+                ("norm_max", C.c_double * 7), ("res_max", C.c_double * 5), ("last_scaled", C.c_double),
                 ("payload_bytes", C.c_uint64), ("checksum", C.c_uint64)]
 
 
@@ -375,7 +393,7 @@ def write_state(path, dim, nx, ny, nz, fields, iter=0, norm_max=None, res_max=No
     hd.last_converged, hd.iter = last_converged, iter
     for q, v in enumerate(norm_max or [0.0] * 7):
         hd.norm_max[q] = v
-    for q, v in enumerate(res_max or [0.0] * 4):
+    for q, v in enumerate(res_max or [0.0] * 5):
         hd.res_max[q] = v
     hd.last_scaled = last_scaled
     hd.payload_bytes = len(payload)

@@ -109,6 +109,11 @@ int run_stage(oracle::Simple<Real> *s, int stage, double *out, size_t n_out) {
             tmp[1] = s->lastSweepSum;
             break;
         }
+        case IFL_STAGE_SMOOTH_T:
+            if (!s->energy) { g_err = "SMOOTH_T needs energy = 1"; return IFL_ERR_ARG; }
+            tmp[0] = s->smoothTemperature();
+            tmp[1] = s->lastSweepSum;
+            break;
         case IFL_STAGE_RHIE_CHOW: s->RhieChowInterpolate(); break;
         case IFL_STAGE_PCORR_ASSEMBLE:
             s->calcPresCorrLinkCoef();

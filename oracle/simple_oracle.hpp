@@ -68,17 +68,26 @@ public:
     int pRefine = 0;
     int sweepBlock = 1;      // sweeps per block: a met stop test ends the call at the block's end
     int numOuterIter = 1;
+    // temperature (config.h:19-23, 45-63; declared but never solved by the reference): the
+    // momentum discretisation with mu -> k / c_p and the TempBCs rows (build addition)
+    bool energy = false;
+    Real gam = 0;
+    int tbct[6];
+    Real tbcv[6];
+    int nIterT = 0;
+    Real relaxT = 0, tolT = 0;
 
     // ---- state (Solver.h:10-45) ----
     std::vector<Real> u, v, w, p, temp, pc, uf, vf, wf;
     std::vector<Real> ucoef, vcoef, wcoef, pcoef;
     std::vector<Real> bu, bv, bw, bp;
+    std::vector<Real> tcoef, bt;      // temperature matrix and source (energy on)
 
     // ---- monitor ----
     int64_t iter = 0;  // outer iterations executed on this object
     double pNormMax = 0, uNormMax = 0, vNormMax = 0, wNormMax = 0;
     double ufNormMax = 0, vfNormMax = 0, wfNormMax = 0;
-    double resMax[4] = {0, 0, 0, 0};  // continuity, u, v, w over the first 5 iterations
+    double resMax[5] = {0, 0, 0, 0, 0};  // continuity, u, v, w, T over the first 5 iterations
     int lastSweeps = 0;
     double lastSweepSum = 0;
 
@@ -120,6 +129,16 @@ public:
         // sweeps per launch on the device: sweep_block for p' (2D red-black), at most 2 for momentum
         sweepBlock = (c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block > 1) ? c.sweep_block : 1;
         numOuterIter = c.num_outer_iter;
+        energy = c.energy != 0;
+        gam = (Real)c.conductivity / (Real)c.specific_heat;
+        for (int f = 0; f < 6; ++f) {
+            tbct[f] = c.temp_bc_type[f];
+            tbcv[f] = (Real)c.temp_bc_val[f];
+            if (energy && tbct[f] != IFL_TEMP_DIRICHLET && tbct[f] != IFL_TEMP_NEUMANN) return fail("bad temp_bc_type");
+        }
+        nIterT = c.n_iter_t;
+        relaxT = (Real)c.relax_t;
+        tolT = (Real)c.tol_t;
         // Solver.cu:13-104: uniform initial u,v,w,p,temp; everything else zero.
         u.assign(Nc, (Real)c.init[0]);
         v.assign(Nc, (Real)c.init[1]);
@@ -133,6 +152,7 @@ public:
         int64_t ncoef = Nc * (1 + 2 * dim);
         ucoef.assign(ncoef, 0); vcoef.assign(ncoef, 0); wcoef.assign(ncoef, 0); pcoef.assign(ncoef, 0);
         bu.assign(Nc, 0); bv.assign(Nc, 0); bw.assign(Nc, 0); bp.assign(Nc, 0);
+        if (energy) { tcoef.assign(ncoef, 0); bt.assign(Nc, 0); }
         return true;
     }
 
@@ -298,11 +318,61 @@ public:
         }
     }
 
-    // Solver.cu:167-175: one momentum assembly.
+    // Temperature matrix and source (build addition): calcMomLinkCoefKernel (kernels.cuh:116-131)
+    // with mu -> k / c_p, no interior source, then per face in launch order E, W, N, S (T, B)
+    // the row of applyBCsToMomEqKernel for a given value (kernels.cuh:304-320) on Dirichlet faces
+    // and the zero-gradient row (:321-331) on Neumann faces.
+    void tempBcRow(int f, int64_t c) {
+        static const int exSlot[6] = {aE, aW, aN, aS, aT, aB};
+        static const int inSlot[6] = {aW, aE, aS, aN, aB, aT};
+        int64_t iC = slot(c, aC), iIn = slot(c, inSlot[f]), iEx = slot(c, exSlot[f]);
+        if (tbct[f] == IFL_TEMP_DIRICHLET) {
+            bt[c] -= (Real)8 / 3 * tcoef[iEx] * tbcv[f];
+            tcoef[iC] -= 2 * tcoef[iEx];
+            tcoef[iIn] += tcoef[iEx] / 3;
+            tcoef[iEx] = 0;
+        } else {
+            tcoef[iC] += tcoef[iEx];
+            tcoef[iEx] = 0;
+        }
+    }
+    void assembleTemperature() {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    int64_t c = cid(i, j, k);
+                    Real ue = uf[ufi(i + 1, j, k)], uw = uf[ufi(i, j, k)];
+                    Real vn = vf[vfi(i, j + 1, k)], vs = vf[vfi(i, j, k)];
+                    tcoef[slot(c, aE)] = (density * std::min(ue, (Real)0) * areaX) - gam * areaX / dx;
+                    tcoef[slot(c, aW)] = (density * std::min(-uw, (Real)0) * areaX) - gam * areaX / dx;
+                    tcoef[slot(c, aN)] = (density * std::min(vn, (Real)0) * areaY) - gam * areaY / dy;
+                    tcoef[slot(c, aS)] = (density * std::min(-vs, (Real)0) * areaY) - gam * areaY / dy;
+                    tcoef[slot(c, aC)] = -(tcoef[slot(c, aE)] + tcoef[slot(c, aW)] + tcoef[slot(c, aN)] + tcoef[slot(c, aS)]);
+                    if (dim == 3) {
+                        Real wt = wf[wfi(i, j, k + 1)], wb = wf[wfi(i, j, k)];
+                        tcoef[slot(c, aT)] = (density * std::min(wt, (Real)0) * areaZ) - gam * areaZ / dz;
+                        tcoef[slot(c, aB)] = (density * std::min(-wb, (Real)0) * areaZ) - gam * areaZ / dz;
+                        tcoef[slot(c, aC)] += -(tcoef[slot(c, aT)] + tcoef[slot(c, aB)]);
+                    }
+                    bt[c] = 0;
+                    if (i == nx - 1) tempBcRow(E, c);
+                    if (i == 0) tempBcRow(W, c);
+                    if (j == ny - 1) tempBcRow(N, c);
+                    if (j == 0) tempBcRow(S, c);
+                    if (dim == 3) {
+                        if (k == nz - 1) tempBcRow(T, c);
+                        if (k == 0) tempBcRow(B, c);
+                    }
+                }
+    }
+
+    // Solver.cu:167-175: one momentum assembly (and the temperature equation's, energy on).
     void assembleMomentum() {
         calcMomLinkCoef();
         calcMomSrcTerm();
         applyBCsToMomEq();
+        if (energy) assembleTemperature();
     }
 
     // ---- smoothers: GaussSeidelIterateKernel kernels.cuh:926-1005 / pointJacobiIterateKernel
@@ -621,12 +691,16 @@ public:
             for (int q = 0; q < 7; ++q) sums[q] += part[(size_t)r * 7 + q];
     }
 
-    // Momentum residual norms before the sweeps (build addition).
-    void momentumResidual(double res[3]) {
+    // Momentum (and temperature) residual norms before the sweeps (build addition).
+    void momentumResidual(double res[4]) {
         res[0] = defect(u.data(), ucoef.data(), bu.data(), nullptr);
         res[1] = defect(v.data(), vcoef.data(), bv.data(), nullptr);
         res[2] = dim == 3 ? defect(w.data(), wcoef.data(), bw.data(), nullptr) : 0.0;
+        res[3] = energy ? defect(temp.data(), tcoef.data(), bt.data(), nullptr) : 0.0;
     }
+
+    // The temperature sweeps (build addition): the momentum smoother call on T.
+    int smoothTemperature() { return iterateMomentum(temp.data(), tcoef.data(), bt.data(), nIterT, relaxT, tolT); }
 
     // ===================== one outer iteration, Solver.cu:160-290 =====================
     // Returns true when the stop rule holds.
@@ -635,11 +709,12 @@ public:
         if (iter == 0) initFaceVel();                    // :162-164
         applyBCsToFaceVel();                             // :165
         assembleMomentum();                              // :167-175
-        double res[3];
+        double res[4];
         momentumResidual(res);
         rec.sweeps[0] = iterateMomentum(u.data(), ucoef.data(), bu.data(), nIter[0], relax[0], tol[0]);   // :178
         rec.sweeps[1] = iterateMomentum(v.data(), vcoef.data(), bv.data(), nIter[1], relax[1], tol[1]);   // :180
         if (dim == 3) rec.sweeps[2] = iterateMomentum(w.data(), wcoef.data(), bw.data(), nIter[2], relax[2], tol[2]);
+        if (energy) rec.sweeps_t = smoothTemperature();
         RhieChowInterpolate();                           // :186-187
         calcPresCorrLinkCoef();                          // :189
         double mass = calcPresCorrSrcTerm();             // :191
@@ -650,7 +725,7 @@ public:
     }
 
     // Solver.cu:226-268 (norms in FP64) plus the scaled residual.
-    bool finishIteration(ifl_iter_record &rec, const double sums[7], double mass, const double res[3]) {
+    bool finishIteration(ifl_iter_record &rec, const double sums[7], double mass, const double res[4]) {
         const double Nd = (double)Nc;
         rec.p_norm = std::sqrt(sums[0] / Nd);
         rec.u_norm = std::sqrt(sums[1] / Nd);
@@ -666,12 +741,14 @@ public:
         rec.res_u = res[0];
         rec.res_v = res[1];
         rec.res_w = res[2];
-        double rs[4] = {rec.res_cont, rec.res_u, rec.res_v, rec.res_w};
+        rec.res_t = res[3];
+        double rs[5] = {rec.res_cont, rec.res_u, rec.res_v, rec.res_w, rec.res_t};
         if (iter < 5)
-            for (int q = 0; q < 4; ++q) resMax[q] = std::max(resMax[q], rs[q]);
+            for (int q = 0; q < 5; ++q) resMax[q] = std::max(resMax[q], rs[q]);
         double sc = 0;
-        for (int q = 0; q < (dim == 3 ? 4 : 3); ++q)
-            if (resMax[q] > 0) sc = std::max(sc, rs[q] / resMax[q]);
+        for (int q = 0; q < 5; ++q)
+            if ((q < 3 || (q == 3 && dim == 3) || (q == 4 && energy)) && resMax[q] > 0)
+                sc = std::max(sc, rs[q] / resMax[q]);
         rec.scaled = sc;
 
         pNormMax = std::max(pNormMax, rec.p_norm);       // :237
@@ -710,7 +787,7 @@ public:
 
     bool finiteRecord(const ifl_iter_record &r) const {
         const double v[] = {r.p_norm, r.u_norm, r.v_norm, r.w_norm, r.uf_norm, r.vf_norm, r.wf_norm,
-                            r.mass_norm, r.res_cont, r.res_u, r.res_v, r.res_w};
+                            r.mass_norm, r.res_cont, r.res_u, r.res_v, r.res_w, r.res_t};
         for (double x : v)
             if (!std::isfinite(x)) return false;
         return true;
@@ -750,6 +827,8 @@ public:
             case IFL_FIELD_BV: return &bv;
             case IFL_FIELD_BW: return &bw;
             case IFL_FIELD_BP: return &bp;
+            case IFL_FIELD_TCOEF: return energy ? &tcoef : nullptr;
+            case IFL_FIELD_BT: return energy ? &bt : nullptr;
             default: return nullptr;
         }
     }

@@ -30,7 +30,7 @@ def test_header_mirror_matches_c_header():
     for decl in re.findall(r"^\s+\w+\s+([\w\[\], ]+);", body, flags=re.M):
         names += [re.sub(r"\[\d+\]", "", d).strip() for d in decl.split(",")]
     assert names == [n for n, _ in _abi.StateHeader._fields_]
-    assert C.sizeof(_abi.StateHeader) == 152      # static_assert in the library
+    assert C.sizeof(_abi.StateHeader) == 160      # static_assert in the library
     assert '#define IFL_STATE_MAGIC "IFLSTATE"' in text and "#define IFL_STATE_VERSION 1" in text
 
 
@@ -43,18 +43,18 @@ def test_round_trip(tmp_path, dims):
     fields = {f: rng.standard_normal(cfg.field_size(f)).astype(np.float32) for f in _abi.STATE_FIELDS}
     path = tmp_path / "s.bin"
     _abi.write_state(path, dim, nx, ny, nz, fields, iter=123, norm_max=list(range(1, 8)),
-                     res_max=[0.5, 0.25, 0.125, 0.0625], last_scaled=3e-4, last_converged=1)
+                     res_max=[0.5, 0.25, 0.125, 0.0625, 0.03125], last_scaled=3e-4, last_converged=1)
     size = os.path.getsize(path)
-    assert size == 152 + 4 * sum(cfg.field_size(f) for f in _abi.STATE_FIELDS)
+    assert size == 160 + 4 * sum(cfg.field_size(f) for f in _abi.STATE_FIELDS)
     hd, got = _abi.read_state(path)
     assert (hd.dim, hd.nx, hd.ny, hd.nz, hd.iter, hd.last_converged) == (dim, nx, ny, nz, 123, 1)
-    assert list(hd.norm_max) == list(range(1, 8)) and list(hd.res_max) == [0.5, 0.25, 0.125, 0.0625]
+    assert list(hd.norm_max) == list(range(1, 8)) and list(hd.res_max) == [0.5, 0.25, 0.125, 0.0625, 0.03125]
     for f in _abi.STATE_FIELDS:
         assert got[f].shape == cfg.shape(f)
         assert np.array_equal(got[f].ravel().view(np.uint32), fields[f].view(np.uint32))
     # field order on disk: u first, wf last
     raw = open(path, "rb").read()
-    assert raw[152:156] == fields[_abi.FIELD_U][:1].tobytes()
+    assert raw[160:164] == fields[_abi.FIELD_U][:1].tobytes()
     assert raw[-4:] == fields[_abi.FIELD_WF][-1:].tobytes()
 
 
