@@ -129,9 +129,9 @@ typedef struct ifl_config {
     int32_t check_every;         /* outer iterations between host reads of the stop flag (0 = auto) */
     int32_t tile;                /* 2D sweep tile: 0 auto, 1 64x16, 2 64x8, 3 32x8, 4 32x4,
                                     5 128x32 (1024-thread workgroups) */
-    int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2 or 4; the momentum
-                                    equations use at most 2): a stop test met inside a block takes
-                                    effect at the block's end (1 = the reference's per-sweep exit) */
+    int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2 or 4, every equation): a
+                                    stop test met inside a block takes effect at the block's end
+                                    (1 = the reference's per-sweep exit) */
     /* Energy (temperature) transport.  The reference declares the thermal properties and the
      * temperature BC table (config.h:19-23, 45-63) but never solves for temperature; here the
      * steady equation  div(rho u T) = div((k / c_p) grad T)  is discretised like the momentum

@@ -161,11 +161,11 @@ struct Plan {
     int dim;
 };
 
-// Sweeps per launch of a smoother call: sweep_block for p' (2D red-black), at most 2 for the
-// momentum and temperature equations (their fused launch stages the face velocities), 1 otherwise.
+// Sweeps per launch of a smoother call: sweep_block in 2D red-black, 1 otherwise.
 int sweepBlockOf(const ifl_config &c, int dim, int slot) {
+    (void)slot;
     if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS || c.sweep_block <= 1) return 1;
-    return (slot < 3 || slot == ifl::kSlotT) ? std::min(c.sweep_block, 2) : c.sweep_block;
+    return c.sweep_block;
 }
 
 // Equation index of a smoother slot (relax / tol / sweep counts): u, v, w, p' (p' and its

@@ -538,31 +538,45 @@ __device__ void finishOuter(const CaseParams &P) {
 }
 
 // One workgroup: totals of the cell-kernel partials (fixed order), then (optionally) the monitor.
+// All three partial arrays are read in one pass, four records per thread in flight at a time.
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P0, BatchMap M, int nSingle, int mask) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ONE, blk);
     if (caseDone(P)) return;
     const int nblk = kindCount<BATCH>(M, MAP_CELL, P, nSingle);
+    const bool dm = (mask & RED_MOM) != 0, dM = (mask & RED_MASS) != 0, du = (mask & RED_UPD) != 0;
+    double acc[12];
+#pragma unroll
+    for (int s = 0; s < 12; ++s) acc[s] = 0.0;
+    // record b adds in the order of b (thread t owns b = t, t + 256, ...; chunks of four keep the order)
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += 4 * kBlock) {
+        double r[4][12];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int b = b0 + q * kBlock;
+            const bool ok = b < nblk;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) r[q][s] = (ok && dm) ? P.partMom[4 * b + s] : 0.0;
+            r[q][4] = (ok && dM) ? P.partMass[b] : 0.0;
+#pragma unroll
+            for (int s = 0; s < 7; ++s) r[q][5 + s] = (ok && du) ? P.partUpd[7 * b + s] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int s = 0; s < 12; ++s) acc[s] += r[q][s];
+    }
+    double tot[12];
+    blockSum<12>(acc, tot);
+    if (threadIdx.x != 0) return;
     CaseState &S = *P.st;
-    if (mask & RED_MOM) {
-        double t[4];
-        sumPartials<4>(P.partMom, nblk, t);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < 4; ++q) S.resSq[q] = t[q];
-    }
-    if (mask & RED_MASS) {
-        double t[1];
-        sumPartials<1>(P.partMass, nblk, t);
-        if (threadIdx.x == 0) S.mass = t[0];
-    }
-    if (mask & RED_UPD) {
-        double t[7];
-        sumPartials<7>(P.partUpd, nblk, t);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < 7; ++q) S.upd[q] = t[q];
-    }
-    if ((mask & RED_FINISH) && threadIdx.x == 0) finishOuter(P);
+    if (dm)
+        for (int q = 0; q < 4; ++q) S.resSq[q] = tot[q];
+    if (dM) S.mass = tot[4];
+    if (du)
+        for (int q = 0; q < 7; ++q) S.upd[q] = tot[5 + q];
+    if (mask & RED_FINISH) finishOuter(P);
 }
 
 // ---------------- host-side launchers ----------------

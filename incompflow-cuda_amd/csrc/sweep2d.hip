@@ -512,13 +512,9 @@ static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel
         k_rb_pc2d<TX, TY, K, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, launch);
         return;
     }
-    if constexpr (K <= 2) {
-        if (kind == COEF_TEMP) k_rb_tile2d<TX, TY, K, 1, COEF_TEMP, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
-        else k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
-    } else {
-        (void)neq;
-    }
+    if (kind == COEF_TEMP) k_rb_tile2d<TX, TY, K, 1, COEF_TEMP, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    else if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    else k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
 }
 
 template <int K, bool BATCH>
@@ -551,11 +547,11 @@ int default_tile(int nx, int ny) {
 
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block) {
     if (L.batch) {
-        if (block == 4 && kind == COEF_PCORR) launchK<4, true>(L, kind, e0, e1, neq, launch);
+        if (block == 4) launchK<4, true>(L, kind, e0, e1, neq, launch);
         else if (block >= 2) launchK<2, true>(L, kind, e0, e1, neq, launch);
         else launchK<1, true>(L, kind, e0, e1, neq, launch);
     } else {
-        if (block == 4 && kind == COEF_PCORR) launchK<4, false>(L, kind, e0, e1, neq, launch);
+        if (block == 4) launchK<4, false>(L, kind, e0, e1, neq, launch);
         else if (block >= 2) launchK<2, false>(L, kind, e0, e1, neq, launch);
         else launchK<1, false>(L, kind, e0, e1, neq, launch);
     }

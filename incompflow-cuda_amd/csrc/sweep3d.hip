@@ -147,11 +147,13 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     };
 
     // issue the loads of the first four planes and the first operands, then decide which
-    // equations sweep
+    // equations sweep; plane z0+2 stays in flight, in registers, until the first step stores it
+    float pend[NEQ][NPL];
     {
         float reg[4][NEQ][NPL];
 #pragma unroll
         for (int s = 0; s < 4; ++s) loadPlane(z0 - 2 + s, reg[s]);
+        loadPlane(z0 + 2, pend);
 #pragma unroll
         for (int s = 0; s < 4; ++s) storePlane(z0 - 2 + s, reg[s]);
     }
@@ -248,13 +250,13 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     // (planes t..t+2), barrier, black(t) (planes t-1..t+1), barrier, write plane t.  The store of
     // step t+1 (over plane t-1) follows black(t) by a barrier.
     for (int t = z0; t < z1; ++t) {
-        // prefetch: old plane t+2 (needed by red(t+1)) and the operands of the next step
-        float nxt[NEQ][NPL];
-        loadPlane(t + 2, nxt);
+        // old plane t+2 (needed by red(t+1)) was loaded during the previous step; plane t+3 and the
+        // operands of the next step load during this one
+        storePlane(t + 2, pend);
+        loadPlane(t + 3, pend);
         Ops redN[NRT], blkN[NBT];
         loadRed(t + 2, redN);
         loadBlack(t + 2, blkN);
-        storePlane(t + 2, nxt);
         __syncthreads();
         redPlane(t + 1, red);
         __syncthreads();

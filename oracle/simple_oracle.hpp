@@ -126,7 +126,7 @@ public:
         stopRule = c.stop_rule;
         tolRes = c.tol_residual;
         pRefine = c.p_refine;
-        // sweeps per launch on the device: sweep_block for p' (2D red-black), at most 2 for momentum
+        // sweeps per launch on the device: sweep_block for every equation (2D red-black)
         sweepBlock = (c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block > 1) ? c.sweep_block : 1;
         numOuterIter = c.num_outer_iter;
         energy = c.energy != 0;
@@ -453,7 +453,7 @@ public:
         return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlock);
     }
     int iterateMomentum(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl) {
-        return iterateBlocked(fld, coef, src, n, rlx, tl, std::min(sweepBlock, 2));
+        return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlock);
     }
     int iterateBlocked(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl, int block) {
         double maxNorm = -1e20;
