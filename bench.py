@@ -231,6 +231,7 @@ def energy_leg(args, incompflow, _abi, base_ms):
     c = _abi.cavity_config(n=n, p_refine=args.p_refine)
     c.stop_rule = _abi.STOP_NONE
     c.sweep_block = args.sweep_block
+    c.sweep_block_mom = args.sweep_block_mom
     c.tile = args.tile
     c.energy, c.n_iter_t, c.relax_t, c.tol_t = 1, 10, 0.7, 1e-5
     c.conductivity, c.specific_heat = c.viscosity / 0.71, 1.0
@@ -248,7 +249,7 @@ def energy_leg(args, incompflow, _abi, base_ms):
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
     us, _ = s.time_kernel(_abi.KERNEL_TEMP_SWEEP, reps=max(10, args.kernel_reps // 4))
     s.close()
-    kb = min(2, max(1, args.sweep_block))
+    kb = args.sweep_block_mom if args.sweep_block_mom > 0 else min(2, max(1, args.sweep_block))
     cells = n * n
     log(f"[bench] energy: {ms:.4f} ms/step with temperature ({base_ms:.4f} without), T sweep launch {us:.2f} us")
     return {"workload": f"2D cavity {n}x{n}, Re 100, walls south 373 K / north 293 K, east/west adiabatic, "
@@ -286,6 +287,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=600)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--sweep-block", type=int, default=4)
+    ap.add_argument("--sweep-block-mom", type=int, default=4,
+                    help="momentum/temperature sweeps per launch on the 2D cavity (batch and adjoint legs keep 2)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
@@ -325,6 +328,7 @@ def main():
     else:
         cfg = _abi.cavity_config(n=n, p_refine=args.p_refine, tol_residual=args.target)
     cfg.sweep_block = args.sweep_block
+    cfg.sweep_block_mom = args.sweep_block_mom
     cfg.tile = args.tile
     cells = cfg.nx * cfg.ny * cfg.nz
 
@@ -456,7 +460,7 @@ def main():
                    if cfg.dim == 2 else
                    (f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
                     f"FP64 reductions"),
-                   "cells": cells, "sweeps_per_step": sweeps / args.steps, "sweep_block": kb,
+                   "cells": cells, "sweeps_per_step": sweeps / args.steps, "sweep_block": kb, "sweep_block_mom": cfg.sweep_block_mom,
                    "tile": {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16", _abi.TILE_64x8: "64x8",
                             _abi.TILE_32x8: "32x8", _abi.TILE_32x4: "32x4"}.get(tile, "3d")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

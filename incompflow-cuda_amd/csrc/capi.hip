@@ -91,6 +91,8 @@ int validate(const ifl_config &c) {
     if (c.tile < 0 || c.tile > ifl::TILE_128x32) return fail(IFL_ERR_ARG, "tile must be 0..5");
     if (c.sweep_block < 0 || c.sweep_block > 4 || c.sweep_block == 3)
         return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2 or 4");
+    if (c.sweep_block_mom < 0 || c.sweep_block_mom > 4 || c.sweep_block_mom == 3)
+        return fail(IFL_ERR_ARG, "sweep_block_mom must be 0, 1, 2 or 4");
     if (c.energy != 0 && c.energy != 1) return fail(IFL_ERR_ARG, "energy must be 0 or 1");
     if (c.energy) {
         if (c.n_iter_t < 0 || c.n_iter_t > kMaxSweeps) return fail(IFL_ERR_ARG, "n_iter_t out of range");
@@ -161,11 +163,16 @@ struct Plan {
     int dim;
 };
 
-// Sweeps per launch of a smoother call: sweep_block in 2D red-black, 1 otherwise.
+// Sweeps per launch of a smoother call in 2D red-black (1 otherwise): sweep_block for p' and its
+// refinement, sweep_block_mom for the momentum and temperature equations (0: the smaller of
+// sweep_block and 2).
 int sweepBlockOf(const ifl_config &c, int dim, int slot) {
-    (void)slot;
-    if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS || c.sweep_block <= 1) return 1;
-    return c.sweep_block;
+    if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS) return 1;
+    if (slot < 3 || slot == ifl::kSlotT) {
+        const int k = c.sweep_block_mom > 0 ? c.sweep_block_mom : std::min(std::max(c.sweep_block, 1), 2);
+        return std::max(k, 1);
+    }
+    return std::max(c.sweep_block, 1);
 }
 
 // Equation index of a smoother slot (relax / tol / sweep counts): u, v, w, p' (p' and its
@@ -1424,12 +1431,13 @@ int batchValidate(const ifl_config *c, int n) {
         if (st != IFL_OK) return fail(st, "case " + std::to_string(i) + ": " + g_err);
         const ifl_config &a = c[0], &b = c[i];
         bool same = a.dim == b.dim && a.smoother == b.smoother && a.p_refine == b.p_refine
-                    && a.sweep_block == b.sweep_block && a.tile == b.tile && a.energy == b.energy
+                    && a.sweep_block == b.sweep_block && a.sweep_block_mom == b.sweep_block_mom
+                    && a.tile == b.tile && a.energy == b.energy
                     && (!a.energy || a.n_iter_t == b.n_iter_t);
         for (int q = 0; q < 4; ++q) same = same && a.n_iter[q] == b.n_iter[q];
         if (!same)
             return fail(IFL_ERR_ARG, "case " + std::to_string(i)
-                                         + ": dim, smoother, n_iter, p_refine, sweep_block, tile, energy and n_iter_t "
+                                         + ": dim, smoother, n_iter, p_refine, sweep_block(_mom), tile, energy and n_iter_t "
                                            "must agree across a batch");
     }
     return IFL_OK;

@@ -90,7 +90,8 @@ class Config(C.Structure):
         ("tol_t", C.c_double),
         ("temp_bc_type", C.c_int32 * 6),
         ("temp_bc_val", C.c_double * 6),
-        ("reserved", C.c_int32 * 4),
+        ("sweep_block_mom", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def copy(self):

@@ -67,6 +67,7 @@ public:
     double tolRes = 1e-5;
     int pRefine = 0;
     int sweepBlock = 1;      // sweeps per block: a met stop test ends the call at the block's end
+    int sweepBlockMom = 1;   // the same for the momentum and temperature equations
     int numOuterIter = 1;
     // temperature (config.h:19-23, 45-63; declared but never solved by the reference): the
     // momentum discretisation with mu -> k / c_p and the TempBCs rows (build addition)
@@ -126,8 +127,11 @@ public:
         stopRule = c.stop_rule;
         tolRes = c.tol_residual;
         pRefine = c.p_refine;
-        // sweeps per launch on the device: sweep_block for every equation (2D red-black)
-        sweepBlock = (c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS && c.sweep_block > 1) ? c.sweep_block : 1;
+        // sweeps per launch on the device (2D red-black): sweep_block for p', sweep_block_mom for the
+        // momentum and temperature equations (0: the smaller of sweep_block and 2)
+        const bool blocked = c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS;
+        sweepBlock = (blocked && c.sweep_block > 1) ? c.sweep_block : 1;
+        sweepBlockMom = !blocked ? 1 : c.sweep_block_mom > 0 ? c.sweep_block_mom : std::min(sweepBlock, 2);
         numOuterIter = c.num_outer_iter;
         energy = c.energy != 0;
         gam = (Real)c.conductivity / (Real)c.specific_heat;
@@ -453,7 +457,7 @@ public:
         return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlock);
     }
     int iterateMomentum(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl) {
-        return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlock);
+        return iterateBlocked(fld, coef, src, n, rlx, tl, sweepBlockMom);
     }
     int iterateBlocked(Real *fld, const Real *coef, const Real *src, int n, Real rlx, Real tl, int block) {
         double maxNorm = -1e20;

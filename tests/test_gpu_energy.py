@@ -90,12 +90,12 @@ def test_energy_outer_iterations_bitwise(name, nT):
     assert [r.sweeps_t for r in rg["history"]] == [r.sweeps_t for r in ro["history"]]
 
 
-@pytest.mark.parametrize("tile,block", [(1, 2), (3, 1), (4, 4), (5, 2), (5, 4)])
-def test_energy_2d_tiles_and_sweep_blocks(tile, block):
+@pytest.mark.parametrize("tile,block,mblock", [(1, 2, 0), (3, 1, 0), (4, 4, 4), (5, 2, 0), (5, 4, 4), (1, 4, 1)])
+def test_energy_2d_tiles_and_sweep_blocks(tile, block, mblock):
     cfg = energy(cavity(150, 70), n=9)
-    cfg.tile, cfg.sweep_block = tile, block
+    cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = tile, block, mblock
     cfg.stop_rule = A.STOP_NONE
-    _log(f"[energy tiles] tile={tile} block={block}")
+    _log(f"[energy tiles] tile={tile} block={block} momentum block={mblock}")
     g, o = Solver(cfg), Oracle(cfg)
     g.solve(4)
     o.solve(4)

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import CASES, STATE_FIELDS, assert_bitwise, load_state, max_rel, random_state
+from helpers import CASES, STATE_FIELDS, assert_bitwise, cavity, load_state, max_rel, random_state
 from incompflow import _abi
 from incompflow import Solver, IflError
 from oracle.oracle import Oracle
@@ -255,6 +255,23 @@ def test_two_sweeps_per_launch(tile, name, block):
     _compare_all(g, o, cfg, f"block2 {name}")
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
     np.testing.assert_allclose(_rec_array(rg["history"]), _rec_array(ro["history"]), rtol=1e-11, atol=1e-300)
+
+
+@pytest.mark.parametrize("tile,shape", [(_abi.TILE_64x16, (150, 70)), (_abi.TILE_32x8, (40, 30)),
+                                        (_abi.TILE_128x32, (400, 128))])
+@pytest.mark.parametrize("mom_block", [1, 4])
+def test_momentum_sweep_block(tile, shape, mom_block):
+    """sweep_block_mom (momentum sweeps per launch) 1 or 4 with four p' sweeps per launch: the
+    blocked stop rule of the oracle, odd and unequal sweep counts, interior and boundary tiles."""
+    cfg = cavity(*shape)
+    cfg.sweep_block, cfg.sweep_block_mom, cfg.tile = 4, mom_block, tile
+    cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[3] = 7, 10, 12
+    cfg.tol[0] = 1e-2
+    _log(f"[mom block {mom_block}] tile={tile} shape={shape}")
+    g, o = Solver(cfg), Oracle(cfg)
+    rg, ro = g.solve(5), o.solve(5)
+    _compare_all(g, o, cfg, f"momentum block {mom_block}")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
 @pytest.mark.parametrize("block", [2, 4])

@@ -465,9 +465,12 @@ def test_sweep_block_only_delays_the_stop():
     h1 = Oracle(c1).solve(1)["history"][0]
     h2 = Oracle(c2).solve(1)["history"][0]
     assert h1.sweeps[1] == 1 and h2.sweeps[1] == 2
-    # sweep_block 4 applies to every equation: the v smoother's zero-increment stop waits for the
-    # end of its first block of four
+    # sweep_block 4 applies to p' only: the momentum equations keep blocks of 2 unless
+    # sweep_block_mom asks for 4, when the v smoother's stop waits for the end of a block of four
     c4 = cavity(12, 10)
     c4.sweep_block = 4
+    h4 = Oracle(c4).solve(1)["history"][0]
+    assert h4.sweeps[1] == 2
+    c4.sweep_block_mom = 4
     h4 = Oracle(c4).solve(1)["history"][0]
     assert h4.sweeps[1] == 4
