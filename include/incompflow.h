@@ -280,7 +280,8 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
 /* ---- Autotuner (no reference counterpart) ----
  * Picks, by measurement on the handle's own case, the settings that the reference fixes at
  * compile time (config.h:67-70) or that only exist here:
- *   IFL_TUNE_LAUNCH : 2D sweep tile and sweep_block, by the device time of one outer iteration;
+ *   IFL_TUNE_LAUNCH : 2D sweep tile, sweep_block and sweep_block_mom, by the device time of one
+ *                     outer iteration;
  *   IFL_TUNE_SWEEPS : sweeps per outer iteration (n_iter of u, v, w and of p'), by the drop of
  *                     the scaled residual per second over trial_iters outer iterations.
  * Trials start from the current state (a case that has run fewer than 20 outer iterations is
@@ -291,12 +292,12 @@ enum { IFL_TUNE_LAUNCH = 1, IFL_TUNE_SWEEPS = 2 };
 
 typedef struct ifl_tune_result {
     int32_t tile;              /* chosen tile (0 in 3D) */
-    int32_t sweep_block;       /* chosen sweeps per launch */
+    int32_t sweep_block;       /* chosen p' sweeps per launch */
     int32_t n_iter[4];         /* chosen sweeps per outer iteration (u, v, w, p') */
     double us_per_iter;        /* device time of one outer iteration with the choice */
     double decades_per_s;      /* scaled-residual decades per second with the choice (0: not measured) */
     int32_t trials;            /* configurations measured */
-    int32_t pad;
+    int32_t sweep_block_mom;   /* chosen momentum sweeps per launch (0: the default rule) */
 } ifl_tune_result;
 
 int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_result *out);

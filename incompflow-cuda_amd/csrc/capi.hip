@@ -1042,13 +1042,14 @@ int snapRestore(ifl_solver *h, const Snapshot &s) {
 }
 
 struct Settings {
-    int tile, K, nIter[4], stopRule;
+    int tile, K, nIter[4], stopRule, Km;
 };
 
 void applySettings(ifl_solver *h, const Settings &x) {
     h->cfg.tile = x.tile;
     if (h->P.dim == 2) h->tile = x.tile;
     h->cfg.sweep_block = x.K;
+    h->cfg.sweep_block_mom = x.Km;
     for (int q = 0; q < 4; ++q) h->cfg.n_iter[q] = x.nIter[q];
     h->cfg.stop_rule = x.stopRule;
     h->P.stopRule = x.stopRule;
@@ -1083,7 +1084,7 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
     std::memset(out, 0, sizeof(*out));
     const int trial = trial_iters > 0 ? trial_iters : 300;
     const ifl_config c0 = h->cfg;
-    Settings best{h->P.dim == 2 ? h->tile : 0, std::max(1, c0.sweep_block), {}, IFL_STOP_NONE};
+    Settings best{h->P.dim == 2 ? h->tile : 0, std::max(1, c0.sweep_block), {}, IFL_STOP_NONE, c0.sweep_block_mom};
     for (int q = 0; q < 4; ++q) best.nIter[q] = c0.n_iter[q];
     int st;
     double ms = 0, sc = 0;
@@ -1108,17 +1109,21 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         bestUs = 1e300;
         Settings x = best;
         for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_128x32; ++tile)
-            for (int K = 1; K <= 4; K *= 2) {
-                x.tile = tile;
-                x.K = K;
-                double us;
-                if ((st = timeIter(x, us)) != IFL_OK) return st;
-                if (us < bestUs) {
-                    bestUs = us;
-                    best.tile = tile;
-                    best.K = K;
+            for (int K = 1; K <= 4; K *= 2)
+                for (int Km : {0, 4}) {          // momentum: the default rule, or four per launch
+                    if (Km == 4 && K < 4) continue;
+                    x.tile = tile;
+                    x.K = K;
+                    x.Km = Km;
+                    double us;
+                    if ((st = timeIter(x, us)) != IFL_OK) return st;
+                    if (us < bestUs) {
+                        bestUs = us;
+                        best.tile = tile;
+                        best.K = K;
+                        best.Km = Km;
+                    }
                 }
-            }
     }
     if (what & IFL_TUNE_SWEEPS) {
         const int uvOpts[3] = {5, 10, 20}, pOpts[3] = {10, 20, 40};
@@ -1150,6 +1155,7 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
     applySettings(h, best);
     out->tile = best.tile;
     out->sweep_block = best.K;
+    out->sweep_block_mom = best.Km;
     for (int q = 0; q < 4; ++q) out->n_iter[q] = best.nIter[q];
     out->us_per_iter = bestUs;
     out->decades_per_s = bestRate;

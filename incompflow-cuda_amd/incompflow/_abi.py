@@ -161,7 +161,7 @@ class AdjointReport(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile", C.c_int32), ("sweep_block", C.c_int32), ("n_iter", C.c_int32 * 4),
                 ("us_per_iter", C.c_double), ("decades_per_s", C.c_double), ("trials", C.c_int32),
-                ("pad", C.c_int32)]
+                ("sweep_block_mom", C.c_int32)]
 
 
 def default_config():

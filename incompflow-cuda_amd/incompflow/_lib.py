@@ -205,16 +205,18 @@ class Solver:
         return out.reshape(self.cfg.shape(field)) if shaped else out
 
     def autotune(self, what=_abi.TUNE_LAUNCH | _abi.TUNE_SWEEPS, trial_iters=0):
-        """Pick tile, sweep_block and sweeps per outer iteration by measurement (ifl_autotune);
+        """Pick tile, sweep_block(_mom) and sweeps per outer iteration by measurement (ifl_autotune);
         the choice is applied to this solver and mirrored into self.cfg."""
         r = _abi.TuneResult()
         _check(lib().ifl_autotune(self.h, what, trial_iters, C.byref(r)))
         if self.cfg.dim == 2:
             self.cfg.tile = r.tile
         self.cfg.sweep_block = r.sweep_block
+        self.cfg.sweep_block_mom = r.sweep_block_mom
         for q in range(4):
             self.cfg.n_iter[q] = r.n_iter[q]
-        return {"tile": r.tile, "sweep_block": r.sweep_block, "n_iter": list(r.n_iter),
+        return {"tile": r.tile, "sweep_block": r.sweep_block, "sweep_block_mom": r.sweep_block_mom,
+                "n_iter": list(r.n_iter),
                 "us_per_iter": r.us_per_iter, "decades_per_s": r.decades_per_s, "trials": r.trials}
 
 

@@ -50,3 +50,27 @@ def test_dropin_solver_example(tmp_path):
     k = lines.index("pressure 1 480 float")
     p_vtk = np.array(lines[k + 1].split(), dtype=np.float64)
     np.testing.assert_allclose(p_vtk, s.get(_abi.FIELD_P).ravel(), rtol=1e-5, atol=1e-6)   # 6 digits
+
+
+def test_dropin_solver_with_energy(tmp_path):
+    """IFL_SOLVE_ENERGY turns the temperature equation on behind the unchanged Solver surface; the
+    config.h's TempBCs (north 293 K, south 373 K, east/west adiabatic in the example case) drive it."""
+    cfg_text = open(os.path.join(EX, "cavity", "config.h")).read()
+    cfg_text = re.sub(r"constexpr int nx = \d+;", "constexpr int nx = 24;", cfg_text)
+    cfg_text = re.sub(r"constexpr int ny = \d+;", "constexpr int ny = 20;", cfg_text)
+    cfg_text = re.sub(r"constexpr int numOuterIter = \d+;", "constexpr int numOuterIter = 1500;", cfg_text)
+    (tmp_path / "config.h").write_text(cfg_text)
+    shutil.copy(os.path.join(EX, "cavity", "types.h"), tmp_path / "types.h")
+    exe = tmp_path / "example_energy"
+    subprocess.run(["g++", "-O2", "-std=c++20", "-DIFL_SOLVE_ENERGY", f"-I{tmp_path}",
+                    f"-I{os.path.join(ROOT, 'include')}", "-o", str(exe), os.path.join(EX, "main.cpp"),
+                    f"-L{LIBDIR}", "-lincompflow", f"-Wl,-rpath,{LIBDIR}"], check=True)
+    out = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = (tmp_path / "output" / "test1.vtk").read_text().split("\n")
+    k = lines.index("temperature 1 480 float")
+    T = np.array(lines[k + 1].split(), dtype=np.float64).reshape(20, 24)
+    assert T.min() > 290.0 and T.max() < 376.0
+    rows = T.mean(axis=1)
+    assert rows[0] > rows[-1] + 40.0              # hot south wall, cold north wall
+    assert 300.0 < rows[10] < 366.0
