@@ -125,6 +125,67 @@ __device__ __forceinline__ void tempBcRow(const CaseParams &P, int f, float (&a)
     bcRowCoefKind(given, f, a);
 }
 
+// The seven-point neighbourhood of a cell (C, E, W, N, S, T, B): offsets to the neighbours, 0
+// (the cell itself) where the domain ends, so that every neighbour load is issued unconditionally
+// and the missing ones are replaced by the caller.  With no branch around a load, the loads of a
+// cell are all in flight together instead of one dependent round trip each.
+struct Nbr {
+    int off[7];
+    bool has[7];
+};
+template <int DIM>
+__device__ __forceinline__ Nbr nbrOf(const CaseParams &P, int i, int j, int k) {
+    const int pl = P.nx * P.ny;
+    Nbr n;
+    n.has[0] = true;
+    n.has[1] = i != P.nx - 1;
+    n.has[2] = i != 0;
+    n.has[3] = j != P.ny - 1;
+    n.has[4] = j != 0;
+    n.has[5] = DIM == 3 && k != P.nz - 1;
+    n.has[6] = DIM == 3 && k != 0;
+    const int d[7] = {0, 1, -1, P.nx, -P.nx, pl, -pl};
+#pragma unroll
+    for (int s = 0; s < 7; ++s) n.off[s] = n.has[s] ? d[s] : 0;
+    return n;
+}
+template <int DIM>
+__device__ __forceinline__ void gather7(const float *f, int c, const Nbr &n, float (&x)[7]) {
+#pragma unroll
+    for (int s = 0; s < 1 + 2 * DIM; ++s) x[s] = f[c + n.off[s]];
+}
+// momResidual on a gathered neighbourhood (the same terms in the same order).
+template <int DIM>
+__device__ __forceinline__ double residual7(const float (&a)[7], float b, const float (&x)[7], const Nbr &n) {
+    double r = (double)b;
+    r -= (double)a[sC] * (double)x[0];
+    r -= (double)a[sE] * (double)(n.has[1] ? x[1] : 0.0f);
+    r -= (double)a[sW] * (double)(n.has[2] ? x[2] : 0.0f);
+    r -= (double)a[sN] * (double)(n.has[3] ? x[3] : 0.0f);
+    r -= (double)a[sS] * (double)(n.has[4] ? x[4] : 0.0f);
+    if (DIM == 3) {
+        r -= (double)a[sT] * (double)(n.has[5] ? x[5] : 0.0f);
+        r -= (double)a[sB] * (double)(n.has[6] ? x[6] : 0.0f);
+    }
+    return r;
+}
+// faceBC (kernels.cuh:87-93) with the face's profile value already loaded.
+__device__ __forceinline__ float faceBCv(const CaseParams &P, int f, float prof, float adjacent) {
+    const int t = P.bct[f];
+    if (t == IFL_WALL) return 0.0f;
+    if (t == IFL_INLET) return P.prof[f] != nullptr ? prof : P.bcv[f][f >> 1];
+    return adjacent;
+}
+// momSrc on a gathered pressure neighbourhood: direction d, position idx of n cells, lo/hi
+// neighbour slots.
+__device__ __forceinline__ float momSrc7(const CaseParams &P, int d, int idx, int n, const float (&p)[7]) {
+    const int lo = 2 * d + 1, hi = 2 * d;
+    const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
+    const float p_L = (idx == 0) ? (ghostCopy(P, lo) ? p[0] : 0.0f) : p[2 + 2 * d];
+    const float p_R = (idx == n - 1) ? (ghostCopy(P, hi) ? p[0] : 0.0f) : p[1 + 2 * d];
+    return 0.5 * (p_L - p_R) * area;
+}
+
 template <int DIM, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap M) {
     Blk blk;
@@ -135,43 +196,83 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
     forCells(P, blk, [&](int i, int j, int k, int c) {
         const int nx = P.nx, ny = P.ny, nz = P.nz;
-        applyFaceBCs<DIM>(P, i, j, k, c);
+        const Nbr nb = nbrOf<DIM>(P, i, j, k);
+        // ---- every load of the cell first ----
+        float xu[7], xv[7], xw[7] = {0, 0, 0, 0, 0, 0, 0}, Pn[7];
+        gather7<DIM>(P.u, c, nb, xu);
+        gather7<DIM>(P.v, c, nb, xv);
+        if (DIM == 3) gather7<DIM>(P.w, c, nb, xw);
+        gather7<DIM>(P.p, c, nb, Pn);
+        const float fE = P.uf[ufi(P, i + 1, j, k)], fW = P.uf[ufi(P, i, j, k)];
+        const float fN = P.vf[vfi(P, i, j + 1, k)], fS = P.vf[vfi(P, i, j, k)];
+        const float fT = DIM == 3 ? P.wf[wfi(P, i, j, k + 1)] : 0.0f, fB = DIM == 3 ? P.wf[wfi(P, i, j, k)] : 0.0f;
+        // per-face profile values (read from u where a face has none; then unused)
+        const int qf[6] = {j + ny * k, j + ny * k, i + nx * k, i + nx * k, i + nx * j, i + nx * j};
+        float prv[6];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) prv[f] = (P.prof[f] != nullptr ? P.prof[f] : P.u)[qf[f]];
+        // ---- face BCs of the boundary faces (Solver.cu:165), kernels.cuh:47-95 ----
+        const float adj[6] = {xu[0], xu[0], xv[0], xv[0], xw[0], xw[0]};
+        float fb[6];
+#pragma unroll
+        for (int f = 0; f < 2 * DIM; ++f) fb[f] = faceBCv(P, f, prv[f], adj[f]);
+        if (i == 0) P.uf[ufi(P, 0, j, k)] = fb[W];
+        if (i == nx - 1) P.uf[ufi(P, nx, j, k)] = fb[E];
+        if (j == 0) P.vf[vfi(P, i, 0, k)] = fb[S];
+        if (j == ny - 1) P.vf[vfi(P, i, ny, k)] = fb[N];
+        if (DIM == 3) {
+            if (k == 0) P.wf[wfi(P, i, j, 0)] = fb[B];
+            if (k == nz - 1) P.wf[wfi(P, i, j, nz)] = fb[T];
+        }
         // face velocities (boundary faces hold the values just applied)
-        const float ue = (i == nx - 1) ? faceBC(P, E, j + ny * k, P.u[c]) : P.uf[ufi(P, i + 1, j, k)];
-        const float uw = (i == 0) ? faceBC(P, W, j + ny * k, P.u[c]) : P.uf[ufi(P, i, j, k)];
-        const float vn = (j == ny - 1) ? faceBC(P, N, i + nx * k, P.v[c]) : P.vf[vfi(P, i, j + 1, k)];
-        const float vs = (j == 0) ? faceBC(P, S, i + nx * k, P.v[c]) : P.vf[vfi(P, i, j, k)];
-        const float wt = DIM == 3 ? ((k == nz - 1) ? faceBC(P, T, i + nx * j, P.w[c]) : P.wf[wfi(P, i, j, k + 1)]) : 0.0f;
-        const float wb = DIM == 3 ? ((k == 0) ? faceBC(P, B, i + nx * j, P.w[c]) : P.wf[wfi(P, i, j, k)]) : 0.0f;
+        const float ue = (i == nx - 1) ? fb[E] : fE;
+        const float uw = (i == 0) ? fb[W] : fW;
+        const float vn = (j == ny - 1) ? fb[N] : fN;
+        const float vs = (j == 0) ? fb[S] : fS;
+        const float wt = DIM == 3 ? ((k == nz - 1) ? fb[T] : fT) : 0.0f;
+        const float wb = DIM == 3 ? ((k == 0) ? fb[B] : fB) : 0.0f;
         float a[7];
         momLinkCoef<DIM>(P, ue, uw, vn, vs, wt, wb, a);
         float b[3];
-        b[0] = momSrc(P, 0, i, nx, c, 1);
-        b[1] = momSrc(P, 1, j, ny, c, nx);
-        b[2] = DIM == 3 ? momSrc(P, 2, k, nz, c, nx * ny) : 0.0f;
-        // BC rows in launch order E, W, N, S, T, B (Solver.cu:583-608)
-        if (i == nx - 1) bcRow<DIM>(P, E, j + ny * k, a, b);
-        if (i == 0) bcRow<DIM>(P, W, j + ny * k, a, b);
-        if (j == ny - 1) bcRow<DIM>(P, N, i + nx * k, a, b);
-        if (j == 0) bcRow<DIM>(P, S, i + nx * k, a, b);
+        b[0] = momSrc7(P, 0, i, nx, Pn);
+        b[1] = momSrc7(P, 1, j, ny, Pn);
+        b[2] = DIM == 3 ? momSrc7(P, 2, k, nz, Pn) : 0.0f;
+        // BC rows in launch order E, W, N, S, T, B (Solver.cu:583-608), kernels.cuh:304-331
+        auto row = [&](int f) {
+            const float aEx = a[bcExSlot(f)];
+            if (ghostCopy(P, f)) {
+#pragma unroll
+                for (int comp = 0; comp < DIM; ++comp) {
+                    const float val = (comp == (f >> 1) && P.prof[f] != nullptr) ? prv[f] : P.bcv[f][comp];
+                    b[comp] -= (float)8 / 3 * aEx * val;
+                }
+            }
+            bcRowCoef(P, f, a);
+        };
+        if (i == nx - 1) row(E);
+        if (i == 0) row(W);
+        if (j == ny - 1) row(N);
+        if (j == 0) row(S);
         if (DIM == 3) {
-            if (k == nz - 1) bcRow<DIM>(P, T, i + nx * j, a, b);
-            if (k == 0) bcRow<DIM>(P, B, i + nx * j, a, b);
+            if (k == nz - 1) row(T);
+            if (k == 0) row(B);
         }
 #pragma unroll
         for (int s = 0; s < 1 + 2 * DIM; ++s) P.mcoef[c + P.N * s] = a[s];
         P.bu[c] = b[0];
         P.bv[c] = b[1];
         if (DIM == 3) P.bw[c] = b[2];
-        const double r0 = momResidual<DIM>(P, P.u, a, b[0], i, j, k, c);
-        const double r1 = momResidual<DIM>(P, P.v, a, b[1], i, j, k, c);
+        const double r0 = residual7<DIM>(a, b[0], xu, nb);
+        const double r1 = residual7<DIM>(a, b[1], xv, nb);
         rs[0] += r0 * r0;
         rs[1] += r1 * r1;
         if (DIM == 3) {
-            const double r2 = momResidual<DIM>(P, P.w, a, b[2], i, j, k, c);
+            const double r2 = residual7<DIM>(a, b[2], xw, nb);
             rs[2] += r2 * r2;
         }
         if (P.energy) {      // the temperature equation on the same face velocities
+            float Tn[7];
+            gather7<DIM>(P.temp, c, nb, Tn);
             float t[7];
             linkCoef<DIM>(P, P.gam, ue, uw, vn, vs, wt, wb, t);
             float bt = 0.0f;
@@ -186,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
 #pragma unroll
             for (int s = 0; s < 1 + 2 * DIM; ++s) P.tcoef[c + P.N * s] = t[s];
             P.bt[c] = bt;
-            const double r3 = momResidual<DIM>(P, P.temp, t, bt, i, j, k, c);
+            const double r3 = residual7<DIM>(t, bt, Tn, nb);
             rs[3] += r3 * r3;
         }
     });
