@@ -374,29 +374,41 @@ __global__ __launch_bounds__(kBlock) void k_copy_back(CaseParams P0, BatchMap M,
 }
 
 // ---------------- Rhie-Chow, kernels.cuh:336-500, then face BCs (Solver.cu:187) ----------------
-__device__ __forceinline__ float rhieChow(const CaseParams &P, int d, int c, int idx, int n, int st,
-                                          const float *vel, const float *aCp) {
+// The face between cells L = c - st and R = c of direction d (idx = R's index along d, 1..n-1),
+// from the loaded pressures p_LL, p_L, p_R, p_RR (p_LL read at c when idx == 1, p_RR at c when
+// idx == n - 1: the ghost rules of kernels.cuh:481-495 replace them), velocities and diagonals.
+__device__ __forceinline__ float rhieChow(const CaseParams &P, int d, int idx, int n, float u_L, float u_R,
+                                          float aC_L, float aC_R, float p_LLr, float p_L, float p_R, float p_RRr) {
     const int lo = 2 * d + 1, hi = 2 * d;
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
-    const int LId = c - st, RId = c;
-    const float aC_L = aCp[LId];
-    const float aC_R = aCp[RId];
-    const float u_L = vel[LId];
-    const float u_R = vel[RId];
-    const float p_L = P.p[LId], p_R = P.p[RId];
-    float p_LL, p_RR;
-    if (idx == 1) {
-        p_LL = ghostCopy(P, lo) ? p_L : 0.0f;
-        p_RR = P.p[RId + st];
-    } else if (idx == n - 1) {
-        p_RR = ghostCopy(P, hi) ? p_R : 0.0f;
-        p_LL = P.p[LId - st];
-    } else {
-        p_LL = P.p[LId - st];
-        p_RR = P.p[RId + st];
-    }
+    const float p_LL = (idx == 1) ? (ghostCopy(P, lo) ? p_L : 0.0f) : p_LLr;
+    const float p_RR = (idx == n - 1) ? (ghostCopy(P, hi) ? p_R : 0.0f) : p_RRr;
     return 0.5 * (u_L + u_R) + (p_R - p_LL) * area / (4 * aC_L) + (p_RR - p_L) * area / (4 * aC_R)
            + 0.5 * (1 / aC_L + 1 / aC_R) * (p_L - p_R) * area;
+}
+
+// Profile values of the faces of cell (i,j,k) (read from u where a face has none; then unused).
+template <int DIM>
+__device__ __forceinline__ void loadProfiles(const CaseParams &P, int i, int j, int k, float (&prv)[6]) {
+    const int nx = P.nx, ny = P.ny;
+    const int qf[6] = {j + ny * k, j + ny * k, i + nx * k, i + nx * k, i + nx * j, i + nx * j};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) prv[f] = f < 2 * DIM ? (P.prof[f] != nullptr ? P.prof[f] : P.u)[qf[f]] : 0.0f;
+}
+
+// applyFaceBCs with the cell's velocities and face profile values already loaded.
+template <int DIM>
+__device__ __forceinline__ void storeFaceBCs(const CaseParams &P, int i, int j, int k, float uc, float vc, float wc,
+                                             const float (&prv)[6]) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    if (i == 0) P.uf[ufi(P, 0, j, k)] = faceBCv(P, W, prv[W], uc);
+    if (i == nx - 1) P.uf[ufi(P, nx, j, k)] = faceBCv(P, E, prv[E], uc);
+    if (j == 0) P.vf[vfi(P, i, 0, k)] = faceBCv(P, S, prv[S], vc);
+    if (j == ny - 1) P.vf[vfi(P, i, ny, k)] = faceBCv(P, N, prv[N], vc);
+    if (DIM == 3) {
+        if (k == 0) P.wf[wfi(P, i, j, 0)] = faceBCv(P, B, prv[B], wc);
+        if (k == nz - 1) P.wf[wfi(P, i, j, nz)] = faceBCv(P, T, prv[T], wc);
+    }
 }
 
 template <int DIM, bool BATCH>
@@ -407,10 +419,31 @@ __global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P0, BatchMap M,
     runFinish<BATCH>(P, M, blk, fin);
     const float *aC = P.mcoef;   // plane sC
     forCells(P, blk, [&](int i, int j, int k, int c) {
-        if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, c, i, P.nx, 1, P.u, aC);
-        if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, c, j, P.ny, P.nx, P.v, aC);
-        if (DIM == 3 && k >= 1) P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, c, k, P.nz, P.nx * P.ny, P.w, aC);
-        applyFaceBCs<DIM>(P, i, j, k, c);
+        const int n3[3] = {P.nx, P.ny, P.nz}, id3[3] = {i, j, k};
+        const int st3[3] = {1, P.nx, P.nx * P.ny};
+        const float *vel3[3] = {P.u, P.v, P.w};
+        // every load first: for each direction the cells c-2st, c-st, c, c+st (at c where missing)
+        float pLL[3], pL[3], pR[3], pRR[3], uL[3], uR[3], aL[3], aR[3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int st = st3[d], idx = id3[d], n = n3[d];
+            const int cL = idx >= 1 ? c - st : c, cLL = idx >= 2 ? c - 2 * st : c, cRR = idx <= n - 2 ? c + st : c;
+            pLL[d] = P.p[cLL];
+            pL[d] = P.p[cL];
+            pR[d] = P.p[c];
+            pRR[d] = P.p[cRR];
+            uL[d] = vel3[d][cL];
+            uR[d] = vel3[d][c];
+            aL[d] = aC[cL];
+            aR[d] = aC[c];
+        }
+        float prv[6];
+        loadProfiles<DIM>(P, i, j, k, prv);
+        if (i >= 1) P.uf[ufi(P, i, j, k)] = rhieChow(P, 0, i, P.nx, uL[0], uR[0], aL[0], aR[0], pLL[0], pL[0], pR[0], pRR[0]);
+        if (j >= 1) P.vf[vfi(P, i, j, k)] = rhieChow(P, 1, j, P.ny, uL[1], uR[1], aL[1], aR[1], pLL[1], pL[1], pR[1], pRR[1]);
+        if (DIM == 3 && k >= 1)
+            P.wf[wfi(P, i, j, k)] = rhieChow(P, 2, k, P.nz, uL[2], uR[2], aL[2], aR[2], pLL[2], pL[2], pR[2], pRR[2]);
+        storeFaceBCs<DIM>(P, i, j, k, uR[0], uR[1], DIM == 3 ? uR[2] : 0.0f, prv);
     });
 }
 
@@ -422,22 +455,27 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchM
     if (caseDone(P)) return;
     double s[1] = {0.0};
     forCells(P, blk, [&](int i, int j, int k, int c) {
-        const int N = P.N, pl = P.nx * P.ny;
-        const float *uC = P.mcoef;
+        const int N = P.N;
         const float rho = P.rho, Ax = P.Ax, Ay = P.Ay, Az = P.Az;
+        const Nbr nb = nbrOf<DIM>(P, i, j, k);
+        float uC[7];
+        gather7<DIM>(P.mcoef, c, nb, uC);          // the momentum diagonal (plane sC)
+        const float ufW = P.uf[ufi(P, i, j, k)], ufE = P.uf[ufi(P, i + 1, j, k)];
+        const float vfS = P.vf[vfi(P, i, j, k)], vfN = P.vf[vfi(P, i, j + 1, k)];
+        const float wfB = DIM == 3 ? P.wf[wfi(P, i, j, k)] : 0.0f, wfT = DIM == 3 ? P.wf[wfi(P, i, j, k + 1)] : 0.0f;
         float aE = 0.0f, aW = 0.0f, aN = 0.0f, aS = 0.0f, aT = 0.0f, aB = 0.0f;
-        if (i < P.nx - 1) aE = -0.5 * rho * Ax * Ax * (1 / uC[c] + 1 / uC[c + 1]);
-        if (i > 0) aW = -0.5 * rho * Ax * Ax * (1 / uC[c] + 1 / uC[c - 1]);
-        if (j < P.ny - 1) aN = -0.5 * rho * Ay * Ay * (1 / uC[c] + 1 / uC[c + P.nx]);
-        if (j > 0) aS = -0.5 * rho * Ay * Ay * (1 / uC[c] + 1 / uC[c - P.nx]);
+        if (nb.has[1]) aE = -0.5 * rho * Ax * Ax * (1 / uC[0] + 1 / uC[1]);
+        if (nb.has[2]) aW = -0.5 * rho * Ax * Ax * (1 / uC[0] + 1 / uC[2]);
+        if (nb.has[3]) aN = -0.5 * rho * Ay * Ay * (1 / uC[0] + 1 / uC[3]);
+        if (nb.has[4]) aS = -0.5 * rho * Ay * Ay * (1 / uC[0] + 1 / uC[4]);
         float aCc = -(aE + aW + aN + aS);
         if (DIM == 3) {
-            if (k < P.nz - 1) aT = -0.5 * rho * Az * Az * (1 / uC[c] + 1 / uC[c + pl]);
-            if (k > 0) aB = -0.5 * rho * Az * Az * (1 / uC[c] + 1 / uC[c - pl]);
+            if (nb.has[5]) aT = -0.5 * rho * Az * Az * (1 / uC[0] + 1 / uC[5]);
+            if (nb.has[6]) aB = -0.5 * rho * Az * Az * (1 / uC[0] + 1 / uC[6]);
             aCc += -(aT + aB);
         }
         float *pcf = P.pcoef;
-        P.rinv[c] = 1 / uC[c];
+        P.rinv[c] = 1 / uC[0];
         pcf[c] = aCc;
         pcf[c + N * sE] = aE;
         pcf[c + N * sW] = aW;
@@ -447,9 +485,8 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchM
             pcf[c + N * sT] = aT;
             pcf[c + N * sB] = aB;
         }
-        float b = rho * (Ax * (P.uf[ufi(P, i, j, k)] - P.uf[ufi(P, i + 1, j, k)])
-                         + Ay * (P.vf[vfi(P, i, j, k)] - P.vf[vfi(P, i, j + 1, k)]));
-        if (DIM == 3) b += rho * Az * (P.wf[wfi(P, i, j, k)] - P.wf[wfi(P, i, j, k + 1)]);
+        float b = rho * (Ax * (ufW - ufE) + Ay * (vfS - vfN));
+        if (DIM == 3) b += rho * Az * (wfB - wfT);
         P.bp[c] = b;
         P.pc[c] = 0.0f;                       // Solver.cu:193
         P.alt[3][c] = 0.0f;                   // the partner buffer too (odd launch counts start there)
@@ -466,10 +503,13 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_defect(CaseParams P0, BatchMap
     if (caseDone(P)) return;
     runFinish<BATCH>(P, M, blk, fin);
     forCells(P, blk, [&](int i, int j, int k, int c) {
-        float a[7];
+        const Nbr nb = nbrOf<DIM>(P, i, j, k);
+        float a[7], x[7];
 #pragma unroll
         for (int s = 0; s < 7; ++s) a[s] = (s < 1 + 2 * DIM) ? P.pcoef[c + P.N * s] : 0.0f;
-        P.tmp[c] = (float)momResidual<DIM>(P, P.pc, a, P.bp[c], i, j, k, c);
+        gather7<DIM>(P.pc, c, nb, x);
+        const float b = P.bp[c];
+        P.tmp[c] = (float)residual7<DIM>(a, b, x, nb);
         P.tmp2[c] = 0.0f;
         P.alt[4][c] = 0.0f;
     });
@@ -484,28 +524,19 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_accumulate(CaseParams P0, Batc
 }
 
 // ---------------- updateField, kernels.cuh:583-843 ----------------
-__device__ __forceinline__ float velCorr(const CaseParams &P, int d, int idx, int n, int c, int st) {
+// updateInteriorVelKernel / updateBCVelKernel (:583-748) on the loaded p' neighbourhood: direction
+// d, position idx of n cells.
+__device__ __forceinline__ float velCorr7(const CaseParams &P, int d, int idx, int n, const float (&pc)[7], float aC_C) {
     const int lo = 2 * d + 1, hi = 2 * d;
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
-    float pc_L, pc_R;
-    if (idx == 0) {
-        pc_L = ghostCopy(P, lo) ? P.pc[c] : 0.0f;
-        pc_R = P.pc[c + st];
-    } else if (idx == n - 1) {
-        pc_R = ghostCopy(P, hi) ? P.pc[c] : 0.0f;
-        pc_L = P.pc[c - st];
-    } else {
-        pc_L = P.pc[c - st];
-        pc_R = P.pc[c + st];
-    }
-    const float aC_C = P.mcoef[c];
+    const float pc_L = (idx == 0) ? (ghostCopy(P, lo) ? pc[0] : 0.0f) : pc[2 + 2 * d];
+    const float pc_R = (idx == n - 1) ? (ghostCopy(P, hi) ? pc[0] : 0.0f) : pc[1 + 2 * d];
     return P.relax[d] * 0.5 * (pc_L - pc_R) * area / aC_C;
 }
-__device__ __forceinline__ float faceCorr(const CaseParams &P, int d, int c, int st) {
+// updateFaceVelKernel (:750-812) for the face between the cell's lower neighbour along d and the cell.
+__device__ __forceinline__ float faceCorr7(const CaseParams &P, int d, float aC_L, float aC_R, float pc_L, float pc_R) {
     const float area = d == 0 ? P.Ax : d == 1 ? P.Ay : P.Az;
-    const float aC_L = P.mcoef[c - st];
-    const float aC_R = P.mcoef[c];
-    return P.relax[d] * 0.5 * (1 / aC_L + 1 / aC_R) * (P.pc[c - st] - P.pc[c]) * area;
+    return P.relax[d] * 0.5 * (1 / aC_L + 1 / aC_R) * (pc_L - pc_R) * area;
 }
 
 template <int DIM, bool BATCH>
@@ -516,42 +547,49 @@ __global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M, Fi
     runFinish<BATCH>(P, M, blk, fin);
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     forCells(P, blk, [&](int i, int j, int k, int c) {
-        const int pl = P.nx * P.ny;
-        const float dp = P.relax[3] * P.pc[c];          // kernels.cuh:825
-        P.p[c] += dp;
+        const Nbr nb = nbrOf<DIM>(P, i, j, k);
+        // every load first
+        float pc[7], aC[7];
+        gather7<DIM>(P.pc, c, nb, pc);
+        gather7<DIM>(P.mcoef, c, nb, aC);
+        const float p0 = P.p[c], u0 = P.u[c], v0 = P.v[c], w0 = DIM == 3 ? P.w[c] : 0.0f;
+        const float fu = P.uf[ufi(P, i, j, k)], fv = P.vf[vfi(P, i, j, k)];
+        const float fw = DIM == 3 ? P.wf[wfi(P, i, j, k)] : 0.0f;
+        const float dp = P.relax[3] * pc[0];            // kernels.cuh:825
+        P.p[c] = p0 + dp;
         s[0] += (double)dp * dp;
         // corrected velocities also go to the smoothers' partner buffers, which a momentum call
         // with an odd launch count reads first
-        const float du = velCorr(P, 0, i, P.nx, c, 1);
-        const float un = P.u[c] + du;
+        const float du = velCorr7(P, 0, i, P.nx, pc, aC[0]);
+        const float un = u0 + du;
         P.u[c] = un;
         P.alt[0][c] = un;
         s[1] += (double)du * du;
-        const float dv = velCorr(P, 1, j, P.ny, c, P.nx);
-        const float vn = P.v[c] + dv;
+        const float dv = velCorr7(P, 1, j, P.ny, pc, aC[0]);
+        const float vn = v0 + dv;
         P.v[c] = vn;
         P.alt[1][c] = vn;
         s[2] += (double)dv * dv;
         if (DIM == 3) {
-            const float dw = velCorr(P, 2, k, P.nz, c, pl);
-            const float wn = P.w[c] + dw;
+            const float dw = velCorr7(P, 2, k, P.nz, pc, aC[0]);
+            const float wn = w0 + dw;
             P.w[c] = wn;
             P.alt[2][c] = wn;
             s[3] += (double)dw * dw;
         }
         if (i >= 1) {
-            const float d = faceCorr(P, 0, c, 1);
-            P.uf[ufi(P, i, j, k)] += d;
+            const float d = faceCorr7(P, 0, aC[2], aC[0], pc[2], pc[0]);
+            P.uf[ufi(P, i, j, k)] = fu + d;
             s[4] += (double)d * d;
         }
         if (j >= 1) {
-            const float d = faceCorr(P, 1, c, P.nx);
-            P.vf[vfi(P, i, j, k)] += d;
+            const float d = faceCorr7(P, 1, aC[4], aC[0], pc[4], pc[0]);
+            P.vf[vfi(P, i, j, k)] = fv + d;
             s[5] += (double)d * d;
         }
         if (DIM == 3 && k >= 1) {
-            const float d = faceCorr(P, 2, c, pl);
-            P.wf[wfi(P, i, j, k)] += d;
+            const float d = faceCorr7(P, 2, aC[6], aC[0], pc[6], pc[0]);
+            P.wf[wfi(P, i, j, k)] = fw + d;
             s[6] += (double)d * d;
         }
     });
