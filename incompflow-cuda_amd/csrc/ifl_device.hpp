@@ -113,24 +113,64 @@ __device__ __forceinline__ bool ghostCopy(const CaseParams &P, int f) {
 // std::min(x, 0.0f) with its exact selection rule ((0 < x) ? 0 : x).
 __device__ __forceinline__ float min0(float x) { return (0.0f < x) ? 0.0f : x; }
 
-// ---------------- FP64 reductions (wave64 butterflies, fixed order) ----------------
-__device__ __forceinline__ double waveSum(double x) {
+// ---------------- FP64 reductions (fixed order) ----------------
+// acc + d * d in FP64 as one fused operation: the square of a float is exact in FP64, so this
+// equals the separate multiply and add bit for bit.
+__device__ __forceinline__ double sqAdd(double acc, float d) {
+    const double x = (double)d;
+    return __builtin_fma(x, x, acc);
+}
+// A double moved across lanes by a DPP pattern (both 32-bit halves); every pattern used here is a
+// permutation inside each row of 16 lanes, so every lane reads an active lane of its own row.
+template <int CTRL>
+__device__ __forceinline__ double dppMove(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double laneValue(double x, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// K wave64 sums at once, all lanes active: within each row of 16 lanes, pairs (quad_perm 1,0,3,2),
+// quads (quad_perm 2,3,0,1), half rows (row_half_mirror) and rows (row_mirror) are added, each
+// step commutative so every lane of a row holds the same row total; then (r0 + r1) + (r2 + r3)
+// over the four rows.  The K chains are interleaved step by step, so their latencies overlap.
+template <int K>
+__device__ __forceinline__ void waveSums(double (&x)[K]) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
-    return x;
+    for (int s = 0; s < K; ++s) x[s] += dppMove<0xB1>(x[s]);
+#pragma unroll
+    for (int s = 0; s < K; ++s) x[s] += dppMove<0x4E>(x[s]);
+#pragma unroll
+    for (int s = 0; s < K; ++s) x[s] += dppMove<0x141>(x[s]);
+#pragma unroll
+    for (int s = 0; s < K; ++s) x[s] += dppMove<0x140>(x[s]);
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+        x[s] = (laneValue(x[s], 0) + laneValue(x[s], 16)) + (laneValue(x[s], 32) + laneValue(x[s], 48));
 }
 
 // Reduce K per-thread values over the workgroup of NW waves (all threads must call); every
 // thread returns the K totals.
 template <int K, int NW = kWaves>
 __device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {
+    static_assert(K <= 64, "one value per lane");
     __shared__ double lds[NW][K];
     __shared__ double bc[K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double w[K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-        double w = waveSum(v[s]);
-        if (lane == 0) lds[wave][s] = w;
+    for (int s = 0; s < K; ++s) w[s] = v[s];
+    waveSums<K>(w);
+    if (lane < K) {
+        double mine = w[0];
+#pragma unroll
+        for (int s = 1; s < K; ++s) mine = lane == s ? w[s] : mine;
+        lds[wave][lane] = mine;
     }
     __syncthreads();
     if (threadIdx.x < K) {

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
             if (owned) {
 #pragma unroll
                 for (int jj = 0; jj < K; ++jj)
-                    if (jj == j) acc[jj][q] += (double)d * (double)d;
+                    if (jj == j) acc[jj][q] = sqAdd(acc[jj][q], d);
             }
         }
     };
@@ -446,8 +446,7 @@ __global__ __launch_bounds__(NT) void k_rb_pc2d(CaseParams P0, BatchMap M, Sweep
                         const float d = rlx * (newPhi - phi_C);
                         lds[oPHI + own] = phi_C + d;
                         const bool owned = lx >= H && lx < H + TX && ly >= H && ly < H + TY;
-                        const double dd = owned ? (double)d : 0.0;
-                        acc[p >> 1] += dd * dd;
+                        acc[p >> 1] = sqAdd(acc[p >> 1], owned ? d : 0.0f);
                     }
                 }
             }

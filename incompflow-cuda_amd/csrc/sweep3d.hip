@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
             newPhi /= aC;
             const float d = erlx[e] * (newPhi - phi_C);
             wc[ly][lx] = phi_C + d;
-            if (owned) acc[e] += (double)d * (double)d;
+            if (owned) acc[e] = sqAdd(acc[e], d);
         }
     };
     auto redPlane = [&](int z, const Ops (&o)[NRT]) {
