@@ -9,15 +9,8 @@
 // order).  Reads come from one buffer and writes go to the other (ping-pong), so no workgroup
 // reads a value another workgroup has already updated.
 //
-//   * k_rb_tile2d: the momentum equations.  Coefficients are recomputed from the four face
-//     velocities and the boundary rows (the functions of the assembly kernel, ifl_stencil.hpp),
-//     one set for u and v, which are swept together in one launch since they share the matrix.
-//     The temperature equation (COEF_TEMP) runs through the same kernel with its own diffusion
-//     coefficient and boundary rows.
-//   * k_rb_pc2d: the p' system, with the coefficients towards +x and +y of the stored p' matrix
-//     (the coefficient towards -x of a cell is the +x one of its west neighbour, bit for bit).
-// Every global load of the tile is issued first; the stop decision, which sums the previous
-// launch's partials, is taken after the relax phases, with those loads in flight meanwhile.
+// One kernel, k_rb_sweep2d, serves the momentum equations (u and v together: they share the
+// matrix), the temperature equation and the p' system.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
@@ -26,30 +19,83 @@
 
 namespace ifl {
 
-// Colour-split LDS layout: cell (lx, ly) of a staged region has colour (lx + ly) & 1 (tile origins
-// and halos are even, so this is the global red-black colour) and sits at [colour][ly][lx >> 1].
-// The same-colour cells of a half sweep are then contiguous, and so are their neighbours.
+// Cells a launch relaxes (those within 2K-1 of the tile), per colour, ordered by ring (ring 0 is
+// the tile, ring r the cells r outside it), row-major within a ring; an entry is ly * 256 + lx in
+// staged coordinates.  Every ring-r rectangle has even sides, so each colour holds half of it:
+// half sweep p relaxes the first upTo(2K-1-p) cells of its colour's list.
+template <int TX, int TY, int K>
+struct RingOrder {
+    static constexpr int H = 2 * K, R0 = 2 * K - 1;
+    static constexpr int HX = TX + 2 * H, HY = TY + 2 * H;
+    static_assert(HX <= 256 && HY <= 256, "cells are packed as ly * 256 + lx");
+    static_assert(TX % 2 == 0 && TY % 2 == 0, "even tile sides");
+    static constexpr int upTo(int r) { return (TX + 2 * r) * (TY + 2 * r) / 2; }
+    static constexpr int NPC = upTo(R0);
+    struct Table {
+        unsigned short cell[2][NPC];
+        bool ok;
+    };
+    static constexpr Table make() {
+        Table t{};
+        int n[2] = {0, 0};
+        t.ok = true;
+        for (int r = 0; r <= R0; ++r) {
+            for (int ly = H - r; ly < H + TY + r; ++ly)
+                for (int lx = H - r; lx < H + TX + r; ++lx) {
+                    const int dx = lx < H ? H - lx : (lx >= H + TX ? lx - (H + TX - 1) : 0);
+                    const int dy = ly < H ? H - ly : (ly >= H + TY ? ly - (H + TY - 1) : 0);
+                    if ((dx > dy ? dx : dy) != r) continue;
+                    const int c = (lx + ly) & 1;
+                    t.cell[c][n[c]++] = (unsigned short)(ly * 256 + lx);
+                }
+            t.ok = t.ok && n[0] == upTo(r) && n[1] == upTo(r);
+        }
+        return t;
+    }
+    static constexpr Table table = make();
+    static_assert(table.ok, "each ring holds half of its cells in each colour");
+};
+
+// K red-black sweeps of NEQ equations sharing one matrix (u and v; or T; or the p' system).
+//   * The old fields are staged with a 2K halo in LDS, colour split: cell (lx, ly) has colour
+//     (lx + ly) & 1 (tile origins and halos are even, so this is the global red-black colour) and
+//     sits at [colour][ly][lx >> 1], so a cell's neighbours are the other colour's plane at
+//     +-(row) and at two adjacent entries of its own row.
+//   * A thread owns fixed cells of each colour (entries tid, tid + NT, ... of RingOrder's lists).
+//     Their matrix rows (four links, the centre coefficient and the source), computed or read
+//     once per launch, stay in registers through all 2K half sweeps; a half sweep touches LDS
+//     only for the field.  Momentum and temperature rows come from the four face velocities and
+//     the boundary rows (the functions of the assembly kernel, ifl_stencil.hpp); the p' rows are
+//     the stored coefficients towards +x and +y (the coefficient towards -x of a cell is the +x
+//     one of its west neighbour, bit for bit).
+// Every global load of the tile is issued first; the stop decision, which sums the previous
+// launch's partials, is taken after the relax phases, with those loads in flight meanwhile.
 template <int TX, int TY, int K, int NEQ, int KIND, bool BATCH, int NT>
-__global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1, int launch) {
+__global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1, int launch) {
+    using RO = RingOrder<TX, TY, K>;
+    constexpr bool PC = KIND == COEF_PCORR;
+    static_assert(!PC || NEQ == 1, "the p' system is one equation");
     constexpr int H = 2 * K;                                   // field halo
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
-    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0, SX2 = SX / 2;   // cells ever updated (sources)
-    // the momentum and temperature matrices are rebuilt from the face velocities
-    constexpr bool MOM = KIND == COEF_MOMENTUM || KIND == COEF_TEMP;
-    static_assert(MOM, "the p' system has its own kernel (k_rb_pc2d)");
-    constexpr int UX = SX + 1, UY = SY;                        // uf faces of the updated cells
-    constexpr int VX = SX, VY = SY + 1;                        // vf faces of the updated cells
+    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;          // cells ever relaxed
+    constexpr int UX = SX + 1, UY = SY;                        // uf faces of the relaxed cells
+    constexpr int VX = SX, VY = SY + 1;                        // vf faces of the relaxed cells
     constexpr int NST = (HX * HY + NT - 1) / NT;
     constexpr int NSR = (SX * SY + NT - 1) / NT;
-    constexpr int NUT = MOM ? (UX * UY + NT - 1) / NT : 1;
-    constexpr int NVT = MOM ? (VX * VY + NT - 1) / NT : 1;
-    constexpr int NPF = 4;                                     // prefetched partials per thread and sweep
-    static_assert(HX % 2 == 0 && SX % 2 == 0, "colour split needs even widths");
-    __shared__ float sphi[NEQ][2][HY][HX2];
-    __shared__ float ssrc[NEQ][2][SY][SX2];
-    __shared__ float suf[MOM ? UY : 1][MOM ? UX : 1];
-    __shared__ float svf[MOM ? VY : 1][MOM ? VX : 1];
+    constexpr int NUT = PC ? 1 : (UX * UY + NT - 1) / NT;
+    constexpr int NVT = PC ? 1 : (VX * VY + NT - 1) / NT;
+    constexpr int NPF = NT >= 1024 ? 1 : 4;                    // prefetched partials per thread and sweep
+    constexpr int NSL = (RO::NPC + NT - 1) / NT;               // cells per thread and colour
+    constexpr int CO = HY * HX2;                               // colour stride of a field plane
+    constexpr int PL = 2 * CO;                                 // field plane stride
+    constexpr int CS = (HY - 2) * HX2;                         // colour stride of a source plane (rows 1..HY-2)
+    constexpr int oSRC = NEQ * PL;                             // source planes
+    constexpr int oEX = NEQ * PL + NEQ * 2 * CS, oEY = oEX + PL;   // p' links towards +x, +y
+    constexpr int oUF = oEX, oVF = oUF + UY * UX;              // face velocities
+    constexpr int NLDS = oEX + (PC ? 2 * PL : UY * UX + VY * VX);
+    static_assert(HX % 2 == 0, "colour split needs an even width");
+    __shared__ float lds[NLDS];
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_TILE, blk);
     const int done = caseDone(P);          // checked after the loads are issued (they are in bounds)
@@ -74,81 +120,92 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
     const int tid = threadIdx.x;
 
     // ---- every global load of the tile first ----
-    float rphi[NEQ][NST], rsrc[NEQ][NSR];
+    // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
+    // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
+    const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
+    float rphi[NEQ][NST], rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[NEQ][NSR];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
-        const int e = tid + m * NT;
-        const int ly = e / HX, lx = e - ly * HX;
+        const int i = tid + m * NT;
+        const int ly = i / HX, lx = i - ly * HX;
         const int gx = x0 - H + lx, gy = y0 - H + ly;
-        const bool ok = e < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool ok = i < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         const int g = gy * nx + gx;
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) rphi[q][m] = ok ? in[q][g] : 0.0f;
+        if constexpr (PC) {
+            rex[m] = ok ? cfE[g] : 0.0f;
+            rey[m] = ok ? cfN[g] : 0.0f;
+        }
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
-        const int e = tid + m * NT;
-        const int ly = e / SX, lx = e - ly * SX;
+        const int i = tid + m * NT;
+        const int ly = i / SX, lx = i - ly * SX;
         const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
-        const bool ok = e < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool ok = i < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) rsrc[q][m] = ok ? src[q][gy * nx + gx] : 0.0f;
     }
     float ruf[NUT], rvf[NVT];
-    if constexpr (MOM) {
+    if constexpr (!PC) {
 #pragma unroll
         for (int m = 0; m < NUT; ++m) {
-            const int e = tid + m * NT;
-            const int ly = e / UX, lx = e - ly * UX;
+            const int i = tid + m * NT;
+            const int ly = i / UX, lx = i - ly * UX;
             const int fx = x0 - R0 + lx, gy = y0 - R0 + ly;
-            const bool in = e < UX * UY && fx >= 0 && fx <= nx && gy >= 0 && gy < ny;
-            ruf[m] = in ? P.uf[ufi(P, fx, gy, 0)] : 0.0f;
+            const bool ok = i < UX * UY && fx >= 0 && fx <= nx && gy >= 0 && gy < ny;
+            ruf[m] = ok ? P.uf[ufi(P, fx, gy, 0)] : 0.0f;
         }
 #pragma unroll
         for (int m = 0; m < NVT; ++m) {
-            const int e = tid + m * NT;
-            const int ly = e / VX, lx = e - ly * VX;
+            const int i = tid + m * NT;
+            const int ly = i / VX, lx = i - ly * VX;
             const int gx = x0 - R0 + lx, fy = y0 - R0 + ly;
-            const bool in = e < VX * VY && gx >= 0 && gx < nx && fy >= 0 && fy <= ny;
-            rvf[m] = in ? P.vf[vfi(P, gx, fy, 0)] : 0.0f;
+            const bool ok = i < VX * VY && gx >= 0 && gx < nx && fy >= 0 && fy <= ny;
+            rvf[m] = ok ? P.vf[vfi(P, gx, fy, 0)] : 0.0f;
         }
     }
-    // ---- stage to LDS (colour split) ----
+    // ---- stage to LDS ----
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
-        const int e = tid + m * NT;
-        if (e < HX * HY) {
-            const int ly = e / HX, lx = e - ly * HX;
-            const int c = (lx + ly) & 1;
+        const int i = tid + m * NT;
+        if (i < HX * HY) {
+            const int ly = i / HX, lx = i - ly * HX;
+            const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) sphi[q][c][ly][lx >> 1] = rphi[q][m];
+            for (int q = 0; q < NEQ; ++q) lds[q * PL + a] = rphi[q][m];
+            if constexpr (PC) {
+                lds[oEX + a] = rex[m];
+                lds[oEY + a] = rey[m];
+            }
         }
     }
 #pragma unroll
     for (int m = 0; m < NSR; ++m) {
-        const int e = tid + m * NT;
-        if (e < SX * SY) {
-            const int ly = e / SX, lx = e - ly * SX;
-            const int c = (lx + ly) & 1;
+        const int i = tid + m * NT;
+        if (i < SX * SY) {
+            const int sy = i / SX, sx = i - sy * SX;
+            const int lx = sx + 1, ly = sy + 1;            // field coordinates (R0 = H - 1)
+            const int a = ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1);
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) ssrc[q][c][ly][lx >> 1] = rsrc[q][m];
+            for (int q = 0; q < NEQ; ++q) lds[oSRC + q * 2 * CS + a] = rsrc[q][m];
         }
     }
-    if constexpr (MOM) {
+    if constexpr (!PC) {
 #pragma unroll
         for (int m = 0; m < NUT; ++m) {
-            const int e = tid + m * NT;
-            if (e < UX * UY) suf[e / UX][e % UX] = ruf[m];
+            const int i = tid + m * NT;
+            if (i < UX * UY) lds[oUF + i] = ruf[m];
         }
 #pragma unroll
         for (int m = 0; m < NVT; ++m) {
-            const int e = tid + m * NT;
-            if (e < VX * VY) svf[e / VX][e % VX] = rvf[m];
+            const int i = tid + m * NT;
+            if (i < VX * VY) lds[oVF + i] = rvf[m];
         }
     }
-    // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*256
-    // records are loaded now (after the tile's registers are free) and added after the relax
-    // phases, so their latency overlaps them ----
+    // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*NT
+    // records are loaded now and added after the relax phases, so their latency overlaps them ----
     const int nprev = blk.n;
     PrevState prev[NEQ];
 #pragma unroll
@@ -169,64 +226,105 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
     if (done) return;                      // a finished case: nothing is written
     __syncthreads();
 
-    double acc[K][2];
+    // ---- this thread's cells: LDS positions and matrix rows, in registers ----
+    // own: the cell's entry in its colour plane (-1: none, or outside the grid); wn: its west
+    // neighbour's entry in the other colour's plane (the east neighbour follows it).
+    int own[2][NSL], wn[2][NSL];
+    float cE[2][NSL], cW[2][NSL], cN[2][NSL], cS[2][NSL], cC[2][NSL], cB[NEQ][2][NSL];
 #pragma unroll
-    for (int j = 0; j < K; ++j) acc[j][0] = acc[j][1] = 0.0;
-
-    // relax the cell (gx, gy) of colour `col` in half sweep p (sweep p/2 of this launch)
-    auto relax = [&](int gx, int gy, int col, int p) {
-        const int lx = gx - x0 + H, ly = gy - y0 + H;     // field position
-        const int ix = lx >> 1, ixE = (lx + 1) >> 1, ixW = (lx - 1) >> 1;
-        const int oc = col ^ 1;
-        float a[7];
-        {
-            const int sx = gx - x0 + R0, sy = gy - y0 + R0;
-            const float uw = suf[sy][sx], ue = suf[sy][sx + 1];
-            const float vs = svf[sy][sx], vn = svf[sy + 1][sx];
-            if constexpr (KIND == COEF_TEMP) {
-                linkCoef<2>(P, P.gam, ue, uw, vn, vs, 0.0f, 0.0f, a);
-                tempBcRowsCoef<2>(P, gx, gy, 0, a);
-            } else {
-                momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
-                momBcRowsCoef<2>(P, gx, gy, 0, a);
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int s = 0; s < NSL; ++s) {
+            const int i = s * NT + tid;
+            own[c][s] = -1;
+            wn[c][s] = 0;
+            cE[c][s] = cW[c][s] = cN[c][s] = cS[c][s] = 0.0f;
+            cC[c][s] = 1.0f;
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) cB[q][c][s] = 0.0f;
+            if ((s + 1) * NT <= RO::NPC || i < RO::NPC) {
+                const int e = RO::table.cell[c][i];
+                const int lx = e & 255, ly = e >> 8;
+                const int gx = x0 - H + lx, gy = y0 - H + ly;
+                if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+                    const int o = c * CO + ly * HX2 + (lx >> 1);
+                    const int ot = (1 - c) * CO + ly * HX2 + (lx >> 1);
+                    const int w = ot + (lx & 1) - 1;
+                    own[c][s] = o;
+                    wn[c][s] = w;
+#pragma unroll
+                    for (int q = 0; q < NEQ; ++q) cB[q][c][s] = lds[oSRC + q * 2 * CS + o - c * (CO - CS) - HX2];
+                    if constexpr (PC) {
+                        const float aE = lds[oEX + o], aW = lds[oEX + w];
+                        const float aN = lds[oEY + o], aS = lds[oEY + ot - HX2];
+                        cE[c][s] = aE;
+                        cW[c][s] = aW;
+                        cN[c][s] = aN;
+                        cS[c][s] = aS;
+                        cC[c][s] = -(aE + aW + aN + aS);
+                    } else {
+                        const int sx = lx - 1, sy = ly - 1;    // face coordinates (R0 = H - 1)
+                        const float uw = lds[oUF + sy * UX + sx], ue = lds[oUF + sy * UX + sx + 1];
+                        const float vs = lds[oVF + sy * VX + sx], vn = lds[oVF + (sy + 1) * VX + sx];
+                        float a[7];
+                        if constexpr (KIND == COEF_TEMP) {
+                            linkCoef<2>(P, P.gam, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                            tempBcRowsCoef<2>(P, gx, gy, 0, a);
+                        } else {
+                            momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                            momBcRowsCoef<2>(P, gx, gy, 0, a);
+                        }
+                        cE[c][s] = a[sE];
+                        cW[c][s] = a[sW];
+                        cN[c][s] = a[sN];
+                        cS[c][s] = a[sS];
+                        cC[c][s] = a[sC];
+                    }
+                }
             }
         }
-        const bool owned = gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY;
-        const int j = p >> 1;
-        const int sxi = (lx - 1) >> 1, syi = ly - 1;       // source position (same colour)
-#pragma unroll
-        for (int q = 0; q < NEQ; ++q) {
-            if (2 * eqs[q].k <= p) continue;
-            const float phi_C = sphi[q][col][ly][ix];
-            float newPhi = ssrc[q][col][syi][sxi] - a[sE] * sphi[q][oc][ly][ixE] - a[sW] * sphi[q][oc][ly][ixW]
-                           - a[sN] * sphi[q][oc][ly + 1][ix] - a[sS] * sphi[q][oc][ly - 1][ix];
-            newPhi /= a[sC];
-            const float d = rlx[q] * (newPhi - phi_C);
-            sphi[q][col][ly][ix] = phi_C + d;
-            if (owned) {
-#pragma unroll
-                for (int jj = 0; jj < K; ++jj)
-                    if (jj == j) acc[jj][q] = sqAdd(acc[jj][q], d);
-            }
-        }
-    };
 
+    // ---- the 2K half sweeps ----
+    double acc[K][NEQ];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q) acc[j][q] = 0.0;
     const int kmax = eqs[0].k > eqs[NEQ - 1].k ? eqs[0].k : eqs[NEQ - 1].k;
     const int nph = 2 * kmax;
+    auto phase = [&](auto pcst) {
+        constexpr int p = decltype(pcst)::value;
+        constexpr int col = p & 1, ring = 2 * K - 1 - p;
+        constexpr int cnt = RO::upTo(ring), core = RO::upTo(0);
+        constexpr int dO = (1 - 2 * col) * CO;              // own entry -> same position, other colour
 #pragma unroll
-    for (int p = 0; p < 2 * K; ++p) {
-        if (p >= nph) break;
-        const int color = p & 1, ring = 2 * K - 1 - p;
-        const int xs = x0 - ring, ys = y0 - ring;
-        const int half = (TX + 2 * ring) / 2, rows = TY + 2 * ring;
-        for (int e = tid; e < rows * half; e += NT) {
-            const int r = e / half, kk = e - r * half;
-            const int gy = ys + r;
-            const int gx = xs + 2 * kk + ((xs + gy + color) & 1);
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) relax(gx, gy, color, p);
+        for (int s = 0; s < NSL; ++s) {
+            if (s * NT < cnt) {          // constant after unrolling
+                const int i = s * NT + tid;
+                const int o = own[col][s];
+                if (o >= 0 && ((s + 1) * NT <= cnt || i < cnt)) {
+                    const int w = wn[col][s];
+                    const bool isCore = (s + 1) * NT <= core || i < core;
+#pragma unroll
+                    for (int q = 0; q < NEQ; ++q) {
+                        if (2 * eqs[q].k <= p) continue;
+                        float *f = lds + q * PL;
+                        const float phi_C = f[o];
+                        float newPhi = cB[q][col][s] - cE[col][s] * f[w + 1] - cW[col][s] * f[w]
+                                       - cN[col][s] * f[o + dO + HX2] - cS[col][s] * f[o + dO - HX2];
+                        newPhi /= cC[col][s];
+                        const float d = rlx[q] * (newPhi - phi_C);
+                        f[o] = phi_C + d;
+                        if (isCore) acc[p >> 1][q] = sqAdd(acc[p >> 1][q], d);
+                    }
+                }
+            }
         }
         __syncthreads();
-    }
+    };
+    [&]<int... Ps>(std::integer_sequence<int, Ps...>) {
+        ((Ps < nph ? (phase(std::integral_constant<int, Ps>{}), 0) : 0), ...);
+    }(std::make_integer_sequence<int, 2 * K>{});
 
     // ---- stop decision (launchRuns) from the prefetched and remaining partials ----
     bool act[2] = {true, true};
@@ -236,14 +334,14 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
         for (int q = 0; q < NEQ; ++q)
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
-                double s = 0.0;
+                double sacc = 0.0;
                 if (launch > 0 && kk < eqs[q].kPrev) {
                     const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
 #pragma unroll
-                    for (int f = 0; f < NPF; ++f) s += pf[q][kk][f];
-                    for (int bidx = tid + NPF * NT; bidx < nprev; bidx += NT) s += part[bidx];
+                    for (int f = 0; f < NPF; ++f) sacc += pf[q][kk][f];
+                    for (int bidx = tid + NPF * NT; bidx < nprev; bidx += NT) sacc += part[bidx];
                 }
-                t[q * K + kk] = s;
+                t[q * K + kk] = sacc;
             }
         if (launch > 0) {
             double tot[NEQ * K];
@@ -263,15 +361,15 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
 
     // ---- write the tile: computed values, or the input unchanged for a stopped equation (keeps
     // the ping-pong parity) ----
-    for (int e = tid; e < TX * TY; e += NT) {
-        const int r = e / TX, xx = e - r * TX;
+    for (int i = tid; i < TX * TY; i += NT) {
+        const int r = i / TX, xx = i - r * TX;
         const int gx = x0 + xx, gy = y0 + r;
         if (gx < nx && gy < ny) {
             const int g = gy * nx + gx;
             const int ly = r + H, lx = xx + H;
 #pragma unroll
             for (int q = 0; q < NEQ; ++q) {
-                const float val = act[q] ? sphi[q][(lx + ly) & 1][ly][lx >> 1] : in[q][g];
+                const float val = act[q] ? lds[q * PL + ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1)] : in[q][g];
                 out[q][g] = val;
                 if (eqs[q].accum) P.pc[g] += val;       // the accumulation of k_pcorr_accumulate
             }
@@ -293,227 +391,15 @@ __global__ __launch_bounds__(NT) void k_rb_tile2d(CaseParams P0, BatchMap M, Swe
     }
 }
 
-// ---------------- the p' system: a kernel of its own ----------------
-// Same launch contract as k_rb_tile2d with one equation of kind COEF_PCORR, organised so that
-// the relax loop is mostly arithmetic:
-//   * one flat LDS block holds four planes with a common colour-split layout [colour][ly][ix]
-//     over the field region: the field, the coefficients towards +x and +y, and the source
-//     (rows 1..HY-2 only), so every operand of a cell is a compile-time offset from one of two
-//     addresses (its own colour, the other colour);
-//   * in half sweep p (compile-time) a thread owns one column ix of its colour and marches down
-//     pairs of rows; the lx parity of a colour alternates row by row, so within a pair the E/W
-//     offsets are constants;
-//   * tiles whose ring lies inside the grid skip the per-cell grid test.
-template <int TX, int TY, int K, bool BATCH, int NT>
-__global__ __launch_bounds__(NT) void k_rb_pc2d(CaseParams P0, BatchMap M, SweepSel e0, int launch) {
-    constexpr int H = 2 * K;
-    constexpr int R0 = 2 * K - 1;
-    constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
-    constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;
-    constexpr int NST = (HX * HY + NT - 1) / NT;
-    constexpr int NSR = (SX * SY + NT - 1) / NT;
-    constexpr int NPF = 4;
-    constexpr int CO = HY * HX2;            // colour stride of the field and coefficient planes
-    constexpr int PL = 2 * CO;              // their plane stride
-    constexpr int CS = (HY - 2) * HX2;      // colour stride of the source plane (rows 1..HY-2)
-    constexpr int oPHI = 0, oEX = PL, oEY = 2 * PL, oSRC = 3 * PL;
-    static_assert(HX % 2 == 0, "colour split needs an even width");
-    __shared__ float lds[3 * PL + 2 * CS];
-    Blk blk;
-    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_TILE, blk);
-    const int done = caseDone(P);
-    const int nx = P.nx, ny = P.ny;
-    const int tilesX = (nx + TX - 1) / TX;
-    const int by = blk.flat / tilesX;
-    const int bx = blk.flat - by * tilesX;
-    const int x0 = bx * TX, y0 = by * TY;
-    const SweepSel e = e0;
-    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
-    const bool oddL = ((launch + e.flip) & 1) != 0;
-    const float *in = oddL ? alt : fld;
-    float *out = oddL ? fld : alt;
-    const float *src = slotSource(P, e.slot);
-    const float rlx = P.relax[e.q];
-    const int tid = threadIdx.x;
-
-    // ---- global loads ----
-    // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
-    // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
-    const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
-    float rphi[NST], rex[NST], rey[NST], rsrc[NSR];
-#pragma unroll
-    for (int m = 0; m < NST; ++m) {
-        const int i = tid + m * NT;
-        const int ly = i / HX, lx = i - ly * HX;
-        const int gx = x0 - H + lx, gy = y0 - H + ly;
-        const bool ok = i < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const int g = gy * nx + gx;
-        rphi[m] = ok ? in[g] : 0.0f;
-        rex[m] = ok ? cfE[g] : 0.0f;
-        rey[m] = ok ? cfN[g] : 0.0f;
-    }
-#pragma unroll
-    for (int m = 0; m < NSR; ++m) {
-        const int i = tid + m * NT;
-        const int ly = i / SX, lx = i - ly * SX;
-        const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
-        const bool ok = i < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        rsrc[m] = ok ? src[gy * nx + gx] : 0.0f;
-    }
-    if (done) return;
-
-    // ---- stage: field, coefficients, source at field coordinates ----
-#pragma unroll
-    for (int m = 0; m < NST; ++m) {
-        const int i = tid + m * NT;
-        if (i < HX * HY) {
-            const int ly = i / HX, lx = i - ly * HX;
-            const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
-            lds[oPHI + a] = rphi[m];
-            lds[oEX + a] = rex[m];
-            lds[oEY + a] = rey[m];
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < NSR; ++m) {
-        const int i = tid + m * NT;
-        if (i < SX * SY) {
-            const int sy = i / SX, sx = i - sy * SX;
-            const int lx = sx + 1, ly = sy + 1;
-            lds[oSRC + ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1)] = rsrc[m];
-        }
-    }
-    // partials of the previous launch (stop decision), in flight during the relax phases
-    const int nprev = blk.n;
-    const PrevState prev = loadPrev(P, e.slot, launch);
-    double pf[K][NPF];
-#pragma unroll
-    for (int kk = 0; kk < K; ++kk) {
-        const bool need = launch > 0 && kk < e.kPrev;
-        const double *part = sweepPart(P, e.slot, launch - 1, kk);
-#pragma unroll
-        for (int f = 0; f < NPF; ++f) {
-            const int bidx = tid + f * NT;
-            pf[kk][f] = (need && bidx < nprev) ? part[bidx] : 0.0;
-        }
-    }
-    __syncthreads();
-    // ---- the 2K half sweeps ----
-    double acc[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) acc[j] = 0.0;
-    const int nph = 2 * e.k;
-    auto phase = [&](auto pcst) {
-        constexpr int p = decltype(pcst)::value;
-        constexpr int col = p & 1, ring = 2 * K - 1 - p;
-        constexpr int lyS = H - ring, rows = TY + 2 * ring;
-        constexpr int lxS = H - ring, lxE = H + TX + ring;        // [lxS, lxE)
-        constexpr int ixLo = lxS >> 1, ixHi = (lxE - 1) >> 1;
-        constexpr int ncol = ixHi - ixLo + 1;
-        constexpr int G = NT / ncol;                          // row-pair groups
-        constexpr int npair = (rows + 1) / 2;
-        constexpr int par0 = (lyS + col) & 1;                     // lx parity of the colour in row lyS
-        static_assert(G >= 1, "tile too wide for one workgroup");
-        // Lanes at the ends of a column range hold a cell one ring beyond this half sweep's region
-        // and update it too: later half sweeps read only cells within their own ring plus one, and
-        // it is not written out, so its value is never used.  Cells outside the grid are read as
-        // neighbours (with coefficient 0) and must keep their staged 0: boundary tiles skip them.
-        const bool interior = x0 - ring - 1 >= 0 && x0 + TX + ring + 1 <= nx && y0 - ring - 1 >= 0 && y0 + TY + ring + 1 <= ny;
-        const int ci = tid % ncol, g = tid / ncol;
-        if (g < G) {
-            const int ix = ixLo + ci;
-            for (int m = g; m < npair; m += G) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int ly = lyS + 2 * m + h;
-                    const int par = h == 0 ? par0 : 1 - par0;
-                    const int lx = 2 * ix + par;
-                    const int gx = x0 - H + lx, gy = y0 - H + ly;
-                    if (interior || (gx >= 0 && gx < nx && gy >= 0 && gy < ny)) {
-                        const int own = col * CO + ly * HX2 + ix;
-                        const int oth = (1 - col) * CO + ly * HX2 + ix;
-                        const float aE = lds[oEX + own];
-                        const float aW = lds[oEX + oth + par - 1];
-                        const float aN = lds[oEY + own];
-                        const float aS = lds[oEY + oth - HX2];
-                        const float aC = -(aE + aW + aN + aS);
-                        const float phi_C = lds[oPHI + own];
-                        // source plane: own - col * (CO - CS) - HX2 (a constant offset)
-                        float newPhi = lds[oSRC + own - col * (CO - CS) - HX2] - aE * lds[oPHI + oth + par]
-                                       - aW * lds[oPHI + oth + par - 1]
-                                       - aN * lds[oPHI + oth + HX2] - aS * lds[oPHI + oth - HX2];
-                        newPhi /= aC;
-                        const float d = rlx * (newPhi - phi_C);
-                        lds[oPHI + own] = phi_C + d;
-                        const bool owned = lx >= H && lx < H + TX && ly >= H && ly < H + TY;
-                        acc[p >> 1] = sqAdd(acc[p >> 1], owned ? d : 0.0f);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    };
-    [&]<int... Ps>(std::integer_sequence<int, Ps...>) {
-        ((Ps < nph ? (phase(std::integral_constant<int, Ps>{}), 0) : 0), ...);
-    }(std::make_integer_sequence<int, 2 * K>{});
-
-    // ---- stop decision ----
-    bool act = true;
-    {
-        double t[K];
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-            double sacc = 0.0;
-            if (launch > 0 && kk < e.kPrev) {
-                const double *part = sweepPart(P, e.slot, launch - 1, kk);
-#pragma unroll
-                for (int f = 0; f < NPF; ++f) sacc += pf[kk][f];
-                for (int bidx = tid + NPF * NT; bidx < nprev; bidx += NT) sacc += part[bidx];
-            }
-            t[kk] = sacc;
-        }
-        if (launch > 0) {
-            double tot[K];
-            blockSum<K, NT / 64>(t, tot);
-            act = e.k > 0 && launchDecide<K>(P, blk, e.slot, launch, e.kPrev, prev, tot, P.tol[e.q]);
-        } else {
-            act = e.k > 0;
-        }
-    }
-    // ---- write the tile (or copy the input through for a stopped call) ----
-    for (int i = tid; i < TX * TY; i += NT) {
-        const int r = i / TX, xx = i - r * TX;
-        const int gx = x0 + xx, gy = y0 + r;
-        if (gx < nx && gy < ny) {
-            const int g = gy * nx + gx;
-            const int ly = r + H, lx = xx + H;
-            const float val = act ? lds[oPHI + ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1)] : in[g];
-            out[g] = val;
-            if (e.accum) P.pc[g] += val;
-        }
-    }
-    // ---- this launch's sweep norms ----
-    {
-        double tot[K];
-        blockSum<K, NT / 64>(acc, tot);
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (tid == j && act && j < e.k) sweepPart(P, e.slot, launch, j)[blk.flat] = tot[j];
-    }
-}
-
 template <int TX, int TY, int K, bool BATCH, int NT = kBlock>
 static void launchTile(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch) {
     const CaseParams &P = *L.P;
     const dim3 grid = L.batch ? dim3((unsigned)L.total[MAP_TILE])
                               : dim3((unsigned)(((P.nx + TX - 1) / TX) * ((P.ny + TY - 1) / TY)));
-    if (kind == COEF_PCORR) {
-        k_rb_pc2d<TX, TY, K, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, launch);
-        return;
-    }
-    if (kind == COEF_TEMP) k_rb_tile2d<TX, TY, K, 1, COEF_TEMP, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
-    else if (neq == 2) k_rb_tile2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
-    else k_rb_tile2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    if (kind == COEF_PCORR) k_rb_sweep2d<TX, TY, K, 1, COEF_PCORR, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    else if (kind == COEF_TEMP) k_rb_sweep2d<TX, TY, K, 1, COEF_TEMP, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    else if (neq == 2) k_rb_sweep2d<TX, TY, K, 2, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
+    else k_rb_sweep2d<TX, TY, K, 1, COEF_MOMENTUM, BATCH, NT><<<grid, NT, 0, L.s>>>(P, L.M, e0, e1, launch);
 }
 
 template <int K, bool BATCH>
