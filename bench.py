@@ -62,7 +62,7 @@ def sweep_bytes_per_cell(dim):
 def batch_leg(args, incompflow, _abi):
     cfgs = _abi.uneven_batch_configs(tol_residual=args.batch_target, max_iter=args.batch_max_iter)
     for c in cfgs:
-        c.sweep_block = args.sweep_block
+        c.sweep_block = args.sweep_block_batch
     cells = sum(c.nx * c.ny for c in cfgs)
     log(f"[bench] batch: {len(cfgs)} cases, {cells} cells, to scaled residual {args.batch_target:g}")
     b = incompflow.Batch(cfgs)
@@ -172,7 +172,7 @@ def adjoint_leg(args, incompflow, _abi):
     c.stop_rule = _abi.STOP_SCALED
     c.tol_residual = args.adjoint_forward_target
     c.num_outer_iter = 200000
-    c.sweep_block = args.sweep_block
+    c.sweep_block = args.sweep_block_batch
     prof = np.array(_abi.adjoint_inlet_profile(c))
     log(f"[bench] adjoint channel 1024x256: forward to {c.tol_residual:g}")
     s = incompflow.Solver(c)
@@ -286,9 +286,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=600)
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--sweep-block", type=int, default=4)
-    ap.add_argument("--sweep-block-mom", type=int, default=4,
+    ap.add_argument("--sweep-block", type=int, default=5,
+                    help="p' sweeps per launch on the 2D cavity")
+    ap.add_argument("--sweep-block-mom", type=int, default=5,
                     help="momentum/temperature sweeps per launch on the 2D cavity (batch and adjoint legs keep 2)")
+    ap.add_argument("--sweep-block-batch", type=int, default=4,
+                    help="p' sweeps per launch of the batch and adjoint legs (small tiles: a smaller halo pays)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")

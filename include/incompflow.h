@@ -129,7 +129,7 @@ typedef struct ifl_config {
     int32_t check_every;         /* outer iterations between host reads of the stop flag (0 = auto) */
     int32_t tile;                /* 2D sweep tile: 0 auto, 1 64x16, 2 64x8, 3 32x8, 4 32x4,
                                     5 128x32 (1024-thread workgroups) */
-    int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2 or 4) of the p' system: a
+    int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2, 4 or 5) of the p' system: a
                                     stop test met inside a block takes effect at the block's end
                                     (1 = the reference's per-sweep exit) */
     /* Energy (temperature) transport.  The reference declares the thermal properties and the
@@ -148,8 +148,8 @@ typedef struct ifl_config {
     int32_t temp_bc_type[6];     /* config.h:46-53 TempBCs::type[face]: IFL_TEMP_DIRICHLET / NEUMANN */
     double temp_bc_val[6];       /* config.h:55-62 TempBCs::val[face] (Dirichlet values) */
     int32_t sweep_block_mom;     /* the same for the momentum and temperature equations: 0 = the
-                                    smaller of sweep_block and 2; 1, 2 or 4.  Four pays on large
-                                    grids with 128x32 tiles; on small tiles the 8-cell halo costs more */
+                                    smaller of sweep_block and 2; 1, 2, 4 or 5.  Four or five pay on
+                                    large grids with 128x32 tiles; on small tiles the halo costs more */
     int32_t reserved[3];
 } ifl_config;
 

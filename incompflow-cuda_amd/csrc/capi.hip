@@ -89,10 +89,10 @@ int validate(const ifl_config &c) {
     if (c.stop_rule < 0 || c.stop_rule > 3) return fail(IFL_ERR_ARG, "bad stop_rule");
     if (c.p_refine < 0 || c.p_refine > 16) return fail(IFL_ERR_ARG, "p_refine out of range");
     if (c.tile < 0 || c.tile > ifl::TILE_128x32) return fail(IFL_ERR_ARG, "tile must be 0..5");
-    if (c.sweep_block < 0 || c.sweep_block > 4 || c.sweep_block == 3)
-        return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2 or 4");
-    if (c.sweep_block_mom < 0 || c.sweep_block_mom > 4 || c.sweep_block_mom == 3)
-        return fail(IFL_ERR_ARG, "sweep_block_mom must be 0, 1, 2 or 4");
+    if (c.sweep_block < 0 || c.sweep_block > 5 || c.sweep_block == 3)
+        return fail(IFL_ERR_ARG, "sweep_block must be 0, 1, 2, 4 or 5");
+    if (c.sweep_block_mom < 0 || c.sweep_block_mom > 5 || c.sweep_block_mom == 3)
+        return fail(IFL_ERR_ARG, "sweep_block_mom must be 0, 1, 2, 4 or 5");
     if (c.energy != 0 && c.energy != 1) return fail(IFL_ERR_ARG, "energy must be 0 or 1");
     if (c.energy) {
         if (c.n_iter_t < 0 || c.n_iter_t > kMaxSweeps) return fail(IFL_ERR_ARG, "n_iter_t out of range");
@@ -1109,9 +1109,9 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         bestUs = 1e300;
         Settings x = best;
         for (int tile = ifl::TILE_64x16; tile <= ifl::TILE_128x32; ++tile)
-            for (int K = 1; K <= 4; K *= 2)
-                for (int Km : {0, 4}) {          // momentum: the default rule, or four per launch
-                    if (Km == 4 && K < 4) continue;
+            for (int K : {1, 2, 4, 5})
+                for (int Km : {0, 4, 5}) {       // momentum: the default rule, or as many as p'
+                    if (Km != 0 && Km != K) continue;
                     x.tile = tile;
                     x.K = K;
                     x.Km = Km;

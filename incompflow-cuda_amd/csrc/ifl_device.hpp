@@ -307,7 +307,7 @@ __device__ __forceinline__ const float *slotCoef(const CaseParams &P, int slot) 
 }
 
 // Sweep partials: [slot][launch parity][sweep in the launch, < kMaxBlock][partStride].
-constexpr int kMaxBlock = 4;    // most red-black sweeps per launch
+constexpr int kMaxBlock = 5;    // most red-black sweeps per launch
 __device__ __forceinline__ double *sweepPart(const CaseParams &P, int slot, int launch, int kk) {
     return P.spart + (size_t)((slot * 2 + (launch & 1)) * kMaxBlock + kk) * P.partStride;
 }

@@ -1,4 +1,4 @@
-// sweep2d.hip - K (1, 2 or 4) full red-black Gauss-Seidel sweeps per launch for 2D grids.
+// sweep2d.hip - K (1, 2, 4 or 5) full red-black Gauss-Seidel sweeps per launch for 2D grids.
 //
 // A workgroup owns a TX x TY tile.  It stages the old field on the tile plus a 2K-cell halo in
 // LDS and runs the 2K half sweeps R1 B1 (R2 B2 ...) on shrinking regions: half sweep p updates
@@ -79,12 +79,8 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     constexpr int R0 = 2 * K - 1;                              // ring of the first half sweep
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
     constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;          // cells ever relaxed
-    constexpr int UX = SX + 1, UY = SY;                        // uf faces of the relaxed cells
-    constexpr int VX = SX, VY = SY + 1;                        // vf faces of the relaxed cells
     constexpr int NST = (HX * HY + NT - 1) / NT;
     constexpr int NSR = (SX * SY + NT - 1) / NT;
-    constexpr int NUT = PC ? 1 : (UX * UY + NT - 1) / NT;
-    constexpr int NVT = PC ? 1 : (VX * VY + NT - 1) / NT;
     constexpr int NPF = NT >= 1024 ? 1 : 4;                    // prefetched partials per thread and sweep
     constexpr int NSL = (RO::NPC + NT - 1) / NT;               // cells per thread and colour
     constexpr int CO = HY * HX2;                               // colour stride of a field plane
@@ -92,8 +88,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     constexpr int CS = (HY - 2) * HX2;                         // colour stride of a source plane (rows 1..HY-2)
     constexpr int oSRC = NEQ * PL;                             // source planes
     constexpr int oEX = NEQ * PL + NEQ * 2 * CS, oEY = oEX + PL;   // p' links towards +x, +y
-    constexpr int oUF = oEX, oVF = oUF + UY * UX;              // face velocities
-    constexpr int NLDS = oEX + (PC ? 2 * PL : UY * UX + VY * VX);
+    constexpr int NLDS = oEX + (PC ? 2 * PL : 0);
     static_assert(HX % 2 == 0, "colour split needs an even width");
     __shared__ float lds[NLDS];
     Blk blk;
@@ -147,25 +142,6 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
 #pragma unroll
         for (int q = 0; q < NEQ; ++q) rsrc[q][m] = ok ? src[q][gy * nx + gx] : 0.0f;
     }
-    float ruf[NUT], rvf[NVT];
-    if constexpr (!PC) {
-#pragma unroll
-        for (int m = 0; m < NUT; ++m) {
-            const int i = tid + m * NT;
-            const int ly = i / UX, lx = i - ly * UX;
-            const int fx = x0 - R0 + lx, gy = y0 - R0 + ly;
-            const bool ok = i < UX * UY && fx >= 0 && fx <= nx && gy >= 0 && gy < ny;
-            ruf[m] = ok ? P.uf[ufi(P, fx, gy, 0)] : 0.0f;
-        }
-#pragma unroll
-        for (int m = 0; m < NVT; ++m) {
-            const int i = tid + m * NT;
-            const int ly = i / VX, lx = i - ly * VX;
-            const int gx = x0 - R0 + lx, fy = y0 - R0 + ly;
-            const bool ok = i < VX * VY && gx >= 0 && gx < nx && fy >= 0 && fy <= ny;
-            rvf[m] = ok ? P.vf[vfi(P, gx, fy, 0)] : 0.0f;
-        }
-    }
     // ---- stage to LDS ----
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
@@ -192,17 +168,33 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             for (int q = 0; q < NEQ; ++q) lds[oSRC + q * 2 * CS + a] = rsrc[q][m];
         }
     }
+    // this thread's cells (entries tid, tid + NT, ... of each colour's list; 0xffff: none) and, for
+    // the momentum and temperature rows, their four face velocities (in the grid for every cell
+    // of the grid, so read unconditionally at a clamped cell), loaded once the staging registers
+    // are free
+    int cell[2][NSL];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int sl = 0; sl < NSL; ++sl) {
+            const int i = sl * NT + tid;
+            cell[c][sl] = ((sl + 1) * NT <= RO::NPC || i < RO::NPC) ? RO::table.cell[c][i] : 0xffff;
+        }
+    float fUW[PC ? 1 : 2][NSL], fUE[PC ? 1 : 2][NSL], fVS[PC ? 1 : 2][NSL], fVN[PC ? 1 : 2][NSL];
     if constexpr (!PC) {
 #pragma unroll
-        for (int m = 0; m < NUT; ++m) {
-            const int i = tid + m * NT;
-            if (i < UX * UY) lds[oUF + i] = ruf[m];
-        }
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int m = 0; m < NVT; ++m) {
-            const int i = tid + m * NT;
-            if (i < VX * VY) lds[oVF + i] = rvf[m];
-        }
+            for (int sl = 0; sl < NSL; ++sl) {
+                const int e = cell[c][sl];
+                const int gx = x0 - H + (e & 255), gy = y0 - H + (e >> 8);
+                const bool ok = e != 0xffff && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                const int cx = ok ? gx : 0, cy = ok ? gy : 0;
+                fUW[c][sl] = P.uf[ufi(P, cx, cy, 0)];
+                fUE[c][sl] = P.uf[ufi(P, cx + 1, cy, 0)];
+                fVS[c][sl] = P.vf[vfi(P, cx, cy, 0)];
+                fVN[c][sl] = P.vf[vfi(P, cx, cy + 1, 0)];
+            }
     }
     // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*NT
     // records are loaded now and added after the relax phases, so their latency overlaps them ----
@@ -243,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
 #pragma unroll
             for (int q = 0; q < NEQ; ++q) cB[q][c][s] = 0.0f;
             if ((s + 1) * NT <= RO::NPC || i < RO::NPC) {
-                const int e = RO::table.cell[c][i];
+                const int e = cell[c][s];
                 const int lx = e & 255, ly = e >> 8;
                 const int gx = x0 - H + lx, gy = y0 - H + ly;
                 if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
@@ -263,9 +255,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                         cS[c][s] = aS;
                         cC[c][s] = -(aE + aW + aN + aS);
                     } else {
-                        const int sx = lx - 1, sy = ly - 1;    // face coordinates (R0 = H - 1)
-                        const float uw = lds[oUF + sy * UX + sx], ue = lds[oUF + sy * UX + sx + 1];
-                        const float vs = lds[oVF + sy * VX + sx], vn = lds[oVF + (sy + 1) * VX + sx];
+                        const float uw = fUW[c][s], ue = fUE[c][s], vs = fVS[c][s], vn = fVN[c][s];
                         float a[7];
                         if constexpr (KIND == COEF_TEMP) {
                             linkCoef<2>(P, P.gam, ue, uw, vn, vs, 0.0f, 0.0f, a);
@@ -432,11 +422,13 @@ int default_tile(int nx, int ny) {
 
 void launch_rb_sweep2d(const Lc &L, int kind, const SweepSel &e0, const SweepSel &e1, int neq, int launch, int block) {
     if (L.batch) {
-        if (block == 4) launchK<4, true>(L, kind, e0, e1, neq, launch);
+        if (block == 5) launchK<5, true>(L, kind, e0, e1, neq, launch);
+        else if (block == 4) launchK<4, true>(L, kind, e0, e1, neq, launch);
         else if (block >= 2) launchK<2, true>(L, kind, e0, e1, neq, launch);
         else launchK<1, true>(L, kind, e0, e1, neq, launch);
     } else {
-        if (block == 4) launchK<4, false>(L, kind, e0, e1, neq, launch);
+        if (block == 5) launchK<5, false>(L, kind, e0, e1, neq, launch);
+        else if (block == 4) launchK<4, false>(L, kind, e0, e1, neq, launch);
         else if (block >= 2) launchK<2, false>(L, kind, e0, e1, neq, launch);
         else launchK<1, false>(L, kind, e0, e1, neq, launch);
     }

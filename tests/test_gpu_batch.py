@@ -109,7 +109,7 @@ def test_batch_jacobi():
     _check_batch(cs, 10, "jacobi")
 
 
-@pytest.mark.parametrize("sweep_block,p_refine", [(2, 0), (1, 1), (2, 1), (4, 1)])
+@pytest.mark.parametrize("sweep_block,p_refine", [(2, 0), (1, 1), (2, 1), (4, 1), (5, 1)])
 def test_batch_sweep_block_and_refine(sweep_block, p_refine):
     cs = _mixed_2d(2)
     for c in cs:

@@ -90,7 +90,8 @@ def test_energy_outer_iterations_bitwise(name, nT):
     assert [r.sweeps_t for r in rg["history"]] == [r.sweeps_t for r in ro["history"]]
 
 
-@pytest.mark.parametrize("tile,block,mblock", [(1, 2, 0), (3, 1, 0), (4, 4, 4), (5, 2, 0), (5, 4, 4), (1, 4, 1)])
+@pytest.mark.parametrize("tile,block,mblock", [(1, 2, 0), (3, 1, 0), (4, 4, 4), (5, 2, 0), (5, 4, 4), (1, 4, 1),
+                                               (5, 5, 5), (2, 5, 0)])
 def test_energy_2d_tiles_and_sweep_blocks(tile, block, mblock):
     cfg = energy(cavity(150, 70), n=9)
     cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = tile, block, mblock

@@ -235,13 +235,13 @@ def test_odd_and_unequal_sweep_counts():
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
-@pytest.mark.parametrize("block", [2, 4])
+@pytest.mark.parametrize("block", [2, 4, 5])
 @pytest.mark.parametrize("tile", [_abi.TILE_64x16, _abi.TILE_32x4, _abi.TILE_128x32])
 @pytest.mark.parametrize("name", ["channel2d", "cavity2d"])
 def test_two_sweeps_per_launch(tile, name, block):
-    """sweep_block=2 and 4 (two or four red-black sweeps per launch over a 4- or 8-cell halo; the
-    momentum equations keep 2) vs the oracle's blocked stop rule, with odd and unequal sweep counts
-    and the p' refinement."""
+    """sweep_block=2, 4 and 5 (two, four or five red-black sweeps per launch over a 4-, 8- or
+    10-cell halo; the momentum equations keep 2) vs the oracle's blocked stop rule, with odd and
+    unequal sweep counts and the p' refinement."""
     cfg = CASES[name]()
     cfg.sweep_block = block
     cfg.tile = tile
@@ -259,9 +259,9 @@ def test_two_sweeps_per_launch(tile, name, block):
 
 @pytest.mark.parametrize("tile,shape", [(_abi.TILE_64x16, (150, 70)), (_abi.TILE_32x8, (40, 30)),
                                         (_abi.TILE_128x32, (400, 128))])
-@pytest.mark.parametrize("mom_block", [1, 4])
+@pytest.mark.parametrize("mom_block", [1, 4, 5])
 def test_momentum_sweep_block(tile, shape, mom_block):
-    """sweep_block_mom (momentum sweeps per launch) 1 or 4 with four p' sweeps per launch: the
+    """sweep_block_mom (momentum sweeps per launch) 1, 4 or 5 with four p' sweeps per launch: the
     blocked stop rule of the oracle, odd and unequal sweep counts, interior and boundary tiles."""
     cfg = cavity(*shape)
     cfg.sweep_block, cfg.sweep_block_mom, cfg.tile = 4, mom_block, tile
@@ -274,7 +274,7 @@ def test_momentum_sweep_block(tile, shape, mom_block):
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
-@pytest.mark.parametrize("block", [2, 4])
+@pytest.mark.parametrize("block", [2, 4, 5])
 @pytest.mark.parametrize("tile,shape", [(_abi.TILE_64x16, (256, 96)), (_abi.TILE_128x32, (400, 128))])
 def test_sweep_blocks_on_interior_tiles_with_refinement(block, tile, shape):
     """A grid with interior tiles (their ring lies inside the grid, the unchecked fast path),
