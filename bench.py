@@ -354,6 +354,11 @@ def main():
     log(f"[bench] {ms_per_step:.4f} ms/step (device {r['elapsed_ms'] / args.steps:.4f} ms), "
         f"{sweeps / args.steps:.1f} sweeps/step")
 
+    # the whole outer iteration against the byte model of the reference data flow: 38 (2D) / 51 (3D)
+    # words per cell for the non-sweep stages plus 8 / 10 words per cell and executed sweep
+    words = (38 + 8 * sweeps / args.steps) if cfg.dim == 2 else (51 + 10 * sweeps / args.steps)
+    iter_model_bytes = 4.0 * words * cells
+
     # ---- roofline of the dominant kernel (one red-black sweep of the p' system) ----
     us, per = s.time_kernel(_abi.KERNEL_SWEEP, reps=args.kernel_reps)
     kb = max(1, args.sweep_block) if cfg.dim == 2 else 1   # sweeps per launch of the p' kernel
@@ -369,8 +374,8 @@ def main():
     traffic_source = None
     tile = (args.tile or _abi.default_tile(cfg.nx, cfg.ny)) if cfg.dim == 2 else 0
     tname = {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16"}.get(tile)
-    tfile = os.path.join(ROOT, "profiles", f"k_rb_pc2d_{tname}_cavity1024_k4_traffic.json")
-    if cfg.dim == 2 and n == 1024 and kb == 4 and tname and os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", f"k_rb_sweep2d_pc_{tname}_cavity1024_k{kb}_traffic.json")
+    if cfg.dim == 2 and n == 1024 and tname and os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh)["traffic_bytes_per_launch"]
         traffic_source = os.path.relpath(tfile, ROOT)
@@ -468,10 +473,15 @@ def main():
                             _abi.TILE_32x8: "32x8", _abi.TILE_32x4: "32x4"}.get(tile, "3d")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
-                     "kernel": ("k_rb_pc2d (fused red-black sweeps of p')" if cfg.dim == 2 else
+                     "kernel": ("k_rb_sweep2d<COEF_PCORR> (fused red-black sweeps of p')" if cfg.dim == 2 else
                                 "k_rb_zmarch3d (one fused red-black sweep of p')"), "avg_launch_us": us,
                      "alg_bytes_per_launch": alg_bytes, "own_bytes_per_launch": own_bytes,
-                     "own_gbs": own_gbs},
+                     "own_gbs": own_gbs,
+                     "note": ("algorithmic bytes count 32 B (2D) / 40 B (3D) per cell and sweep, the traffic of a "
+                              "sweep that streams its stored matrix; one launch runs %d fused sweeps on one staged "
+                              "copy of the field, source and coefficients, so frac can exceed 1; 'traffic' and "
+                              "'own_gbs' are the bytes the launch really moves" % kb),
+                     "iteration_model_frac": iter_model_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)},
         "kernel_us": other,
         "time_to_residual_s": ttr,
         "residual_target": cfg.tol_residual,
