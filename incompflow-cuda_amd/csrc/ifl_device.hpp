@@ -371,6 +371,50 @@ __device__ __forceinline__ bool launchDecide(const CaseParams &P, const Blk &b, 
     return !stopped;
 }
 
+// launchDecide for wave 0 of a workgroup (all its lanes): lane kk computes sweep kk's norm, so the
+// K square roots and divisions run side by side.  The values and the decision are launchDecide's
+// bit for bit: each lane takes the running maximum over the sweeps up to its own in the loop's
+// order (fmax is exact), and the call stops if any sweep's test is met.
+template <int K>
+__device__ __forceinline__ bool launchDecideWave(const CaseParams &P, const Blk &b, int slot, int L, int kPrev,
+                                                 const PrevState &pv, const double (&T)[K], double tol) {
+    if (L == 0) return true;
+    static_assert(K <= 64, "one sweep per lane");
+    const int lane = threadIdx.x & 63;
+    double mx = pv.mx;
+    bool stopped = pv.stoppedBefore;
+    double last = 0.0;
+    int ran = pv.before;
+    if (!pv.stoppedBefore) {
+        double tk = T[0];
+#pragma unroll
+        for (int kk = 1; kk < K; ++kk) tk = lane == kk ? T[kk] : tk;
+        const double norm = sqrt(tk / (double)P.N);
+        double m = pv.mx;
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const double nk = laneValue(norm, kk);
+            if (kk < kPrev && kk <= lane) m = fmax(nk, m);
+        }
+        const bool hit = lane < kPrev && norm / (m + 1e-20) < tol;
+        stopped = __builtin_amdgcn_ballot_w64(hit) != 0;
+        if (kPrev > 0) mx = laneValue(m, kPrev - 1);
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) last = kk == kPrev - 1 ? T[kk] : last;
+        ran = pv.before + kPrev;
+    }
+    if (leader(b)) {
+        SmoothState &c = P.st->sm[slot];
+        const int q = (L - 1) & 1;
+        c.stopped[q] = stopped ? 1 : 0;
+        c.maxNorm[q] = mx;
+        c.swAt[q] = ran;
+        if (!pv.stoppedBefore) c.lastSum = last;
+        c.sweeps = ran;
+    }
+    return !stopped;
+}
+
 // Smoother calls whose closing step (k_smooth_finish) workgroup 0 of another kernel performs.
 struct Finish {
     int n = 0;

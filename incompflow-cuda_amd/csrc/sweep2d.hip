@@ -336,13 +336,22 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         if (launch > 0) {
             double tot[NEQ * K];
             blockSum<NEQ * K, NT / 64>(t, tot);
+            // the decision's FP64 square roots and divisions run in wave 0 only, one sweep per lane
+            // (the leader, thread 0, publishes the state), and reach the other waves through LDS
+            __shared__ int sAct[2];
+            if (tid < 64) {
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) {
-                double tq[K];
+                for (int q = 0; q < NEQ; ++q) {
+                    double tq[K];
 #pragma unroll
-                for (int kk = 0; kk < K; ++kk) tq[kk] = tot[q * K + kk];
-                act[q] = eqs[q].k > 0 && launchDecide<K>(P, blk, eqs[q].slot, launch, eqs[q].kPrev, prev[q], tq, P.tol[eqs[q].q]);
+                    for (int kk = 0; kk < K; ++kk) tq[kk] = tot[q * K + kk];
+                    const bool a = eqs[q].k > 0 && launchDecideWave<K>(P, blk, eqs[q].slot, launch, eqs[q].kPrev, prev[q], tq, P.tol[eqs[q].q]);
+                    if (tid == 0) sAct[q] = a ? 1 : 0;
+                }
             }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) act[q] = sAct[q] != 0;
         } else {
 #pragma unroll
             for (int q = 0; q < NEQ; ++q) act[q] = eqs[q].k > 0;
