@@ -6,25 +6,50 @@
 
 namespace ifl {
 
+// The case constants of the link coefficients: density, face areas and the diffusion conductances
+// mu*A/d of each direction.  A kernel reads them once (as the same float expressions, so the same
+// bits); read through P after every store to global memory, the conductances' divisions would be
+// recomputed per cell.
+struct LinkConsts {
+    float rho, Ax, Ay, Az, gx, gy, gz;
+};
+template <int DIM>
+__device__ __forceinline__ LinkConsts linkConsts(const CaseParams &P, float mu) {
+    LinkConsts k;
+    k.rho = P.rho;
+    k.Ax = P.Ax;
+    k.Ay = P.Ay;
+    k.Az = P.Az;
+    k.gx = mu * P.Ax / P.dx;
+    k.gy = mu * P.Ay / P.dy;
+    k.gz = DIM == 3 ? mu * P.Az / P.dz : 0.0f;
+    return k;
+}
+
 // Link coefficients of a transported quantity with diffusion coefficient mu from the six face
 // velocities: calcMomLinkCoefKernel (kernels.cuh:116-131; T3 fixed: aT/aB in their own planes)
 // with mu = P.mu for momentum and mu = P.gam (k / c_p) for temperature.
 template <int DIM>
-__device__ __forceinline__ void linkCoef(const CaseParams &P, float mu, float ue, float uw, float vn, float vs,
-                                         float wt, float wb, float (&a)[7]) {
-    const float rho = P.rho;
-    a[sE] = (rho * min0(ue) * P.Ax) - mu * P.Ax / P.dx;
-    a[sW] = (rho * min0(-uw) * P.Ax) - mu * P.Ax / P.dx;
-    a[sN] = (rho * min0(vn) * P.Ay) - mu * P.Ay / P.dy;
-    a[sS] = (rho * min0(-vs) * P.Ay) - mu * P.Ay / P.dy;
+__device__ __forceinline__ void linkCoef(const LinkConsts &k, float ue, float uw, float vn, float vs, float wt,
+                                         float wb, float (&a)[7]) {
+    const float rho = k.rho;
+    a[sE] = (rho * min0(ue) * k.Ax) - k.gx;
+    a[sW] = (rho * min0(-uw) * k.Ax) - k.gx;
+    a[sN] = (rho * min0(vn) * k.Ay) - k.gy;
+    a[sS] = (rho * min0(-vs) * k.Ay) - k.gy;
     a[sC] = -(a[sE] + a[sW] + a[sN] + a[sS]);
     a[sT] = 0.0f;
     a[sB] = 0.0f;
     if (DIM == 3) {
-        a[sT] = (rho * min0(wt) * P.Az) - mu * P.Az / P.dz;
-        a[sB] = (rho * min0(-wb) * P.Az) - mu * P.Az / P.dz;
+        a[sT] = (rho * min0(wt) * k.Az) - k.gz;
+        a[sB] = (rho * min0(-wb) * k.Az) - k.gz;
         a[sC] += -(a[sT] + a[sB]);
     }
+}
+template <int DIM>
+__device__ __forceinline__ void linkCoef(const CaseParams &P, float mu, float ue, float uw, float vn, float vs,
+                                         float wt, float wb, float (&a)[7]) {
+    linkCoef<DIM>(linkConsts<DIM>(P, mu), ue, uw, vn, vs, wt, wb, a);
 }
 
 template <int DIM>

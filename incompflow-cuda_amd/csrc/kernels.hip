@@ -194,6 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
     if (leader(blk))
         for (int q = 0; q < 5; ++q) P.st->sweepsIter[q] = 0;   // a new outer iteration starts
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    const LinkConsts lkU = linkConsts<DIM>(P, P.mu), lkT = linkConsts<DIM>(P, P.gam);
     forCells(P, blk, [&](int i, int j, int k, int c) {
         const int nx = P.nx, ny = P.ny, nz = P.nz;
         const Nbr nb = nbrOf<DIM>(P, i, j, k);
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
         const float wt = DIM == 3 ? ((k == nz - 1) ? fb[T] : fT) : 0.0f;
         const float wb = DIM == 3 ? ((k == 0) ? fb[B] : fB) : 0.0f;
         float a[7];
-        momLinkCoef<DIM>(P, ue, uw, vn, vs, wt, wb, a);
+        linkCoef<DIM>(lkU, ue, uw, vn, vs, wt, wb, a);
         float b[3];
         b[0] = momSrc7(P, 0, i, nx, Pn);
         b[1] = momSrc7(P, 1, j, ny, Pn);
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_assemble(CaseParams P0, BatchMap
             float Tn[7];
             gather7<DIM>(P.temp, c, nb, Tn);
             float t[7];
-            linkCoef<DIM>(P, P.gam, ue, uw, vn, vs, wt, wb, t);
+            linkCoef<DIM>(lkT, ue, uw, vn, vs, wt, wb, t);
             float bt = 0.0f;
             if (i == nx - 1) tempBcRow<DIM>(P, E, t, bt);
             if (i == 0) tempBcRow<DIM>(P, W, t, bt);

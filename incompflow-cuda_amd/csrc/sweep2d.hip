@@ -222,6 +222,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     // own: the cell's entry in its colour plane (-1: none, or outside the grid); wn: its west
     // neighbour's entry in the other colour's plane (the east neighbour follows it).
     int own[2][NSL], wn[2][NSL];
+    const LinkConsts lk = linkConsts<2>(P, KIND == COEF_TEMP ? P.gam : P.mu);
     float cE[2][NSL], cW[2][NSL], cN[2][NSL], cS[2][NSL], cC[2][NSL], cB[NEQ][2][NSL];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -258,10 +259,10 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                         const float uw = fUW[c][s], ue = fUE[c][s], vs = fVS[c][s], vn = fVN[c][s];
                         float a[7];
                         if constexpr (KIND == COEF_TEMP) {
-                            linkCoef<2>(P, P.gam, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                            linkCoef<2>(lk, ue, uw, vn, vs, 0.0f, 0.0f, a);
                             tempBcRowsCoef<2>(P, gx, gy, 0, a);
                         } else {
-                            momLinkCoef<2>(P, ue, uw, vn, vs, 0.0f, 0.0f, a);
+                            linkCoef<2>(lk, ue, uw, vn, vs, 0.0f, 0.0f, a);
                             momBcRowsCoef<2>(P, gx, gy, 0, a);
                         }
                         cE[c][s] = a[sE];
