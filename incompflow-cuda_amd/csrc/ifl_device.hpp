@@ -135,10 +135,21 @@ __device__ __forceinline__ double laneValue(double x, int lane) {
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
-// K wave64 sums at once, all lanes active: within each row of 16 lanes, pairs (quad_perm 1,0,3,2),
-// quads (quad_perm 2,3,0,1), half rows (row_half_mirror) and rows (row_mirror) are added, each
-// step commutative so every lane of a row holds the same row total; then (r0 + r1) + (r2 + r3)
-// over the four rows.  The K chains are interleaved step by step, so their latencies overlap.
+// A double from the previous row's lane 15 (CTRL row_bcast:15) or from lane 31 (row_bcast:31) in
+// the rows of ROWMASK, 0 in the others.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dppBcast(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// K wave64 sums at once, all lanes active, the totals landing in lane 63: within each row of 16
+// lanes, pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror) and
+// rows (row_mirror) are added, each step commutative so every lane of a row holds the same row
+// total R; then row_bcast:15 adds R0 to row 1 and R2 to row 3, and row_bcast:31 adds row 1's
+// R1 + R0 to row 3: lane 63 holds (R3 + R2) + (R1 + R0).  The K chains are interleaved step by
+// step, so their latencies overlap.
 template <int K>
 __device__ __forceinline__ void waveSums(double (&x)[K]) {
 #pragma unroll
@@ -150,34 +161,53 @@ __device__ __forceinline__ void waveSums(double (&x)[K]) {
 #pragma unroll
     for (int s = 0; s < K; ++s) x[s] += dppMove<0x140>(x[s]);
 #pragma unroll
-    for (int s = 0; s < K; ++s)
-        x[s] = (laneValue(x[s], 0) + laneValue(x[s], 16)) + (laneValue(x[s], 32) + laneValue(x[s], 48));
+    for (int s = 0; s < K; ++s) x[s] += dppBcast<0x142, 0xA>(x[s]);
+#pragma unroll
+    for (int s = 0; s < K; ++s) x[s] += dppBcast<0x143, 0xC>(x[s]);
 }
 
-// Reduce K per-thread values over the workgroup of NW waves (all threads must call); every
-// thread returns the K totals.
+// Reduce K (<= 64) per-thread values over the workgroup of NW waves (all threads must call); the
+// lanes of wave 0 return the K totals (the other waves' out is unspecified).  Each wave's totals go to
+// LDS from lane 63; wave 0 adds the waves in order.  A second call of the same instantiation may
+// follow only after a barrier that wave 0 reaches after reading the totals of the first.
 template <int K, int NW = kWaves>
-__device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {
+__device__ __forceinline__ void blockSumWave0(const double (&v)[K], double (&out)[K]) {
     static_assert(K <= 64, "one value per lane");
     __shared__ double lds[NW][K];
-    __shared__ double bc[K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double w[K];
 #pragma unroll
     for (int s = 0; s < K; ++s) w[s] = v[s];
     waveSums<K>(w);
-    if (lane < K) {
-        double mine = w[0];
+    if (lane == 63)
 #pragma unroll
-        for (int s = 1; s < K; ++s) mine = lane == s ? w[s] : mine;
-        lds[wave][lane] = mine;
-    }
+        for (int s = 0; s < K; ++s) lds[wave][s] = w[s];
     __syncthreads();
-    if (threadIdx.x < K) {
-        double r = lds[0][threadIdx.x];
+    if (wave == 0) {
+        // lane s adds the waves' totals of value s in order; every lane then reads the K sums
+        double r = 0.0;
+        if (lane < K) {
+            r = lds[0][lane];
 #pragma unroll
-        for (int q = 1; q < NW; ++q) r += lds[q][threadIdx.x];
-        bc[threadIdx.x] = r;
+            for (int q = 1; q < NW; ++q) r += lds[q][lane];
+        }
+#pragma unroll
+        for (int s = 0; s < K; ++s) out[s] = laneValue(r, s);
+    }
+}
+
+// The same with every thread returning the K totals.
+template <int K, int NW = kWaves>
+__device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {
+    static_assert(K <= 64, "one value per lane");
+    __shared__ double bc[K];
+    double t[K];
+    blockSumWave0<K, NW>(v, t);
+    if (threadIdx.x < K) {
+        double mine = t[0];
+#pragma unroll
+        for (int s = 1; s < K; ++s) mine = (int)threadIdx.x == s ? t[s] : mine;
+        bc[threadIdx.x] = mine;
     }
     __syncthreads();
 #pragma unroll
@@ -189,8 +219,10 @@ __device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K])
 template <int K>
 __device__ __forceinline__ void storeBlockSums(const double (&v)[K], double *out) {
     double tot[K];
-    blockSum<K>(v, tot);
-    if (threadIdx.x < K) out[threadIdx.x] = tot[threadIdx.x];
+    blockSumWave0<K>(v, tot);
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+        if ((int)threadIdx.x == s) out[s] = tot[s];
 }
 
 // Totals of n partial records of K doubles (record b at part[b*K]); thread t adds records

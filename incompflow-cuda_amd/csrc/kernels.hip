@@ -708,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P0, BatchMap
             for (int s = 0; s < 12; ++s) acc[s] += r[q][s];
     }
     double tot[12];
-    blockSum<12>(acc, tot);
+    blockSumWave0<12>(acc, tot);
     if (threadIdx.x != 0) return;
     CaseState &S = *P.st;
     if (dm)

@@ -114,6 +114,17 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     }
     const int tid = threadIdx.x;
 
+    // ---- this thread's cells (entries tid, tid + NT, ... of each colour's list; 0xffff: none),
+    // looked up first: the face velocity loads and the row setup wait for them ----
+    int cell[2][NSL];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int sl = 0; sl < NSL; ++sl) {
+            const int i = sl * NT + tid;
+            cell[c][sl] = ((sl + 1) * NT <= RO::NPC || i < RO::NPC) ? RO::table.cell[c][i] : 0xffff;
+        }
+
     // ---- every global load of the tile first ----
     // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
     // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
@@ -168,18 +179,9 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             for (int q = 0; q < NEQ; ++q) lds[oSRC + q * 2 * CS + a] = rsrc[q][m];
         }
     }
-    // this thread's cells (entries tid, tid + NT, ... of each colour's list; 0xffff: none) and, for
-    // the momentum and temperature rows, their four face velocities (in the grid for every cell
-    // of the grid, so read unconditionally at a clamped cell), loaded once the staging registers
-    // are free
-    int cell[2][NSL];
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int sl = 0; sl < NSL; ++sl) {
-            const int i = sl * NT + tid;
-            cell[c][sl] = ((sl + 1) * NT <= RO::NPC || i < RO::NPC) ? RO::table.cell[c][i] : 0xffff;
-        }
+    // for the momentum and temperature rows, the four face velocities of this thread's cells (in
+    // the grid for every cell of the grid, so read unconditionally at a clamped cell), loaded once
+    // the staging registers are free
     float fUW[PC ? 1 : 2][NSL], fUE[PC ? 1 : 2][NSL], fVS[PC ? 1 : 2][NSL], fVN[PC ? 1 : 2][NSL];
     if constexpr (!PC) {
 #pragma unroll
@@ -202,17 +204,18 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     PrevState prev[NEQ];
 #pragma unroll
     for (int q = 0; q < NEQ; ++q) prev[q] = loadPrev(P, eqs[q].slot, launch);
+    // (read unconditionally, at record 0 where there is none: a conditional load would be sunk to
+    // its use after the relax phases; launch 0 reads the other parity's records, also in bounds)
     double pf[NEQ][K][NPF];
 #pragma unroll
     for (int q = 0; q < NEQ; ++q)
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            const bool need = launch > 0 && kk < eqs[q].kPrev;
             const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
 #pragma unroll
             for (int f = 0; f < NPF; ++f) {
                 const int bidx = tid + f * NT;
-                pf[q][kk][f] = (need && bidx < nprev) ? part[bidx] : 0.0;
+                pf[q][kk][f] = part[bidx < nprev ? bidx : 0];
             }
         }
     if (done) return;                      // a finished case: nothing is written
@@ -329,14 +332,14 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 if (launch > 0 && kk < eqs[q].kPrev) {
                     const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
 #pragma unroll
-                    for (int f = 0; f < NPF; ++f) sacc += pf[q][kk][f];
+                    for (int f = 0; f < NPF; ++f) sacc += (tid + f * NT < nprev) ? pf[q][kk][f] : 0.0;
                     for (int bidx = tid + NPF * NT; bidx < nprev; bidx += NT) sacc += part[bidx];
                 }
                 t[q * K + kk] = sacc;
             }
         if (launch > 0) {
             double tot[NEQ * K];
-            blockSum<NEQ * K, NT / 64>(t, tot);
+            blockSumWave0<NEQ * K, NT / 64>(t, tot);
             // the decision's FP64 square roots and divisions run in wave 0 only, one sweep per lane
             // (the leader, thread 0, publishes the state), and reach the other waves through LDS
             __shared__ int sAct[2];
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         for (int q = 0; q < NEQ; ++q)
 #pragma unroll
             for (int j = 0; j < K; ++j) v[q * K + j] = acc[j][q];
-        blockSum<NEQ * K, NT / 64>(v, tot);
+        blockSumWave0<NEQ * K, NT / 64>(v, tot);      // the writing threads are in wave 0
 #pragma unroll
         for (int q = 0; q < NEQ; ++q)
 #pragma unroll
