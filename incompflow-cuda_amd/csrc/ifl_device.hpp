@@ -144,14 +144,12 @@ __device__ __forceinline__ double dppBcast(double x) {
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWMASK, 0xF, false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
-// K wave64 sums at once, all lanes active, the totals landing in lane 63: within each row of 16
-// lanes, pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror) and
-// rows (row_mirror) are added, each step commutative so every lane of a row holds the same row
-// total R; then row_bcast:15 adds R0 to row 1 and R2 to row 3, and row_bcast:31 adds row 1's
-// R1 + R0 to row 3: lane 63 holds (R3 + R2) + (R1 + R0).  The K chains are interleaved step by
-// step, so their latencies overlap.
+// K sums over each row of 16 lanes at once, all lanes active: pairs (quad_perm 1,0,3,2), quads
+// (quad_perm 2,3,0,1), half rows (row_half_mirror) and rows (row_mirror) are added, each step
+// commutative, so every lane of a row ends with the same row total.  The K chains are
+// interleaved step by step, so their latencies overlap.
 template <int K>
-__device__ __forceinline__ void waveSums(double (&x)[K]) {
+__device__ __forceinline__ void rowSums(double (&x)[K]) {
 #pragma unroll
     for (int s = 0; s < K; ++s) x[s] += dppMove<0xB1>(x[s]);
 #pragma unroll
@@ -160,39 +158,47 @@ __device__ __forceinline__ void waveSums(double (&x)[K]) {
     for (int s = 0; s < K; ++s) x[s] += dppMove<0x141>(x[s]);
 #pragma unroll
     for (int s = 0; s < K; ++s) x[s] += dppMove<0x140>(x[s]);
+}
+// K wave64 sums, the totals landing in lane 63: row sums, then row_bcast:15 adds R0 to row 1 and
+// R2 to row 3, and row_bcast:31 adds row 1's R1 + R0 to row 3: lane 63 holds (R3 + R2) + (R1 + R0).
+template <int K>
+__device__ __forceinline__ void waveSums(double (&x)[K]) {
+    rowSums<K>(x);
 #pragma unroll
     for (int s = 0; s < K; ++s) x[s] += dppBcast<0x142, 0xA>(x[s]);
 #pragma unroll
     for (int s = 0; s < K; ++s) x[s] += dppBcast<0x143, 0xC>(x[s]);
 }
 
-// Reduce K (<= 64) per-thread values over the workgroup of NW waves (all threads must call); the
-// lanes of wave 0 return the K totals (the other waves' out is unspecified).  Each wave's totals go to
-// LDS from lane 63; wave 0 adds the waves in order.  A second call of the same instantiation may
-// follow only after a barrier that wave 0 reaches after reading the totals of the first.
+// Reduce K (<= 64) per-thread values over the workgroup of NW (<= 16) waves (all threads must
+// call); the lanes of wave 0 return the K totals (the other waves' out is unspecified).  Each
+// wave adds its rows of 16 lanes and stores the four row totals; wave 0 then loads row total l
+// into lane l and adds its 64 lanes.  Waves from `active` on hold only zeros and skip their part
+// (a sum of non-negative terms does not change when zeros are left out).  A second call of the
+// same instantiation may follow only after a barrier that wave 0 reaches after reading the first.
 template <int K, int NW = kWaves>
-__device__ __forceinline__ void blockSumWave0(const double (&v)[K], double (&out)[K]) {
-    static_assert(K <= 64, "one value per lane");
-    __shared__ double lds[NW][K];
+__device__ __forceinline__ void blockSumWave0(const double (&v)[K], double (&out)[K], int active = NW) {
+    static_assert(K <= 64 && NW <= 16, "one value per lane, 64 row totals");
+    __shared__ double lds[4 * NW][K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double w[K];
+    if (wave < active) {
+        double w[K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) w[s] = v[s];
-    waveSums<K>(w);
-    if (lane == 63)
+        for (int s = 0; s < K; ++s) w[s] = v[s];
+        rowSums<K>(w);
+        if ((lane & 15) == 15)
 #pragma unroll
-        for (int s = 0; s < K; ++s) lds[wave][s] = w[s];
+            for (int s = 0; s < K; ++s) lds[4 * wave + (lane >> 4)][s] = w[s];
+    }
     __syncthreads();
     if (wave == 0) {
-        // lane s adds the waves' totals of value s in order; every lane then reads the K sums
-        double r = 0.0;
-        if (lane < K) {
-            r = lds[0][lane];
+        double r[K];
+        const bool has = lane < 4 * active;
 #pragma unroll
-            for (int q = 1; q < NW; ++q) r += lds[q][lane];
-        }
+        for (int s = 0; s < K; ++s) r[s] = has ? lds[lane][s] : 0.0;
+        waveSums<K>(r);
 #pragma unroll
-        for (int s = 0; s < K; ++s) out[s] = laneValue(r, s);
+        for (int s = 0; s < K; ++s) out[s] = laneValue(r[s], 63);
     }
 }
 

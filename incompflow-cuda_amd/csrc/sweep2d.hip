@@ -339,7 +339,9 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             }
         if (launch > 0) {
             double tot[NEQ * K];
-            blockSumWave0<NEQ * K, NT / 64>(t, tot);
+            // only the waves that hold records (threads below nprev, or all past NPF * NT) add
+            const int active = nprev > NT ? NT / 64 : (nprev + 63) / 64;
+            blockSumWave0<NEQ * K, NT / 64>(t, tot, active);
             // the decision's FP64 square roots and divisions run in wave 0 only, one sweep per lane
             // (the leader, thread 0, publishes the state), and reach the other waves through LDS
             __shared__ int sAct[2];
