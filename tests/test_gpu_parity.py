@@ -274,6 +274,25 @@ def test_momentum_sweep_block(tile, shape, mom_block):
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
+@pytest.mark.parametrize("tile,shape,block", [(_abi.TILE_128x32, (4224, 1024), 5), (_abi.TILE_64x16, (2112, 1024), 4)])
+def test_many_tiles_stop_decisions(tile, shape, block):
+    """More tiles than one pass of prefetched partial records: 1056 tiles of 128x32 (1024-thread
+    workgroups) or 2112 of 64x16 (256 threads), so every wave of a sweep's stop decision holds
+    records and, with 64x16, the loop over the records past the prefetched ones runs; loose sweep
+    tolerances make stop tests trigger inside blocks (the v sweeps stop early)."""
+    cfg = cavity(*shape)
+    cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = tile, block, block
+    cfg.n_iter[0], cfg.n_iter[1], cfg.n_iter[3] = 7, 10, 12
+    cfg.tol[0], cfg.tol[1], cfg.tol[3] = 2e-2, 2e-2, 0.25
+    _log(f"[many tiles] tile={tile} grid={shape} block={block}")
+    g, o = Solver(cfg), Oracle(cfg)
+    rg, ro = g.solve(3), o.solve(3)
+    _compare_all(g, o, cfg, f"many tiles {tile}")
+    sweeps = [tuple(r.sweeps) for r in rg["history"]]
+    assert sweeps == [tuple(r.sweeps) for r in ro["history"]]
+    assert any(sw[0] < cfg.n_iter[0] or sw[1] < cfg.n_iter[1] or sw[3] < cfg.n_iter[3] for sw in sweeps), sweeps
+
+
 @pytest.mark.parametrize("block", [2, 4, 5])
 @pytest.mark.parametrize("tile,shape", [(_abi.TILE_64x16, (256, 96)), (_abi.TILE_128x32, (400, 128))])
 def test_sweep_blocks_on_interior_tiles_with_refinement(block, tile, shape):
