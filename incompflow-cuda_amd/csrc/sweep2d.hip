@@ -19,6 +19,16 @@
 
 namespace ifl {
 
+// Raw buffer access (gfx9 descriptor: stride 0, dword 3 = 0x00020000): loads at a byte offset at
+// or past num_records return 0.
+constexpr int kNoOffset = (int)0x80000000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const float *p, int nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, nbytes, 0x00020000);
+}
+__device__ __forceinline__ float bufferLoad(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 // Cells a launch relaxes (those within 2K-1 of the tile), per colour, ordered by ring (ring 0 is
 // the tile, ring r the cells r outside it), row-major within a ring; an entry is ly * 256 + lx in
 // staged coordinates.  Every ring-r rectangle has even sides, so each colour holds half of it:
@@ -128,20 +138,32 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     // ---- every global load of the tile first ----
     // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
     // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
+    // Raw buffer loads: one descriptor per array (N floats) and a 32-bit byte offset per cell (a
+    // case holds at most 2^28 cells, so 4 N < 2^31); a cell outside the grid gets an offset past
+    // the array, which the hardware reads as 0, so no branch surrounds a load.
     const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
+    const int nbytes = P.N * 4;
+    __amdgpu_buffer_rsrc_t bIn[NEQ], bSrc[NEQ];
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q) {
+        bIn[q] = bufferOf(in[q], nbytes);
+        bSrc[q] = bufferOf(src[q], nbytes);
+    }
+    const __amdgpu_buffer_rsrc_t bE = bufferOf(cfE, nbytes), bN = bufferOf(cfN, nbytes);
     float rphi[NEQ][NST], rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[NEQ][NSR];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int i = tid + m * NT;
         const int ly = i / HX, lx = i - ly * HX;
         const int gx = x0 - H + lx, gy = y0 - H + ly;
-        const bool ok = i < HX * HY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const int g = gy * nx + gx;
+        // one unsigned compare per axis, no short circuit: no branch around the offset either
+        const bool ok = (i < HX * HY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+        const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) rphi[q][m] = ok ? in[q][g] : 0.0f;
+        for (int q = 0; q < NEQ; ++q) rphi[q][m] = bufferLoad(bIn[q], off);
         if constexpr (PC) {
-            rex[m] = ok ? cfE[g] : 0.0f;
-            rey[m] = ok ? cfN[g] : 0.0f;
+            rex[m] = bufferLoad(bE, off);
+            rey[m] = bufferLoad(bN, off);
         }
     }
 #pragma unroll
@@ -149,9 +171,10 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         const int i = tid + m * NT;
         const int ly = i / SX, lx = i - ly * SX;
         const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
-        const bool ok = i < SX * SY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const bool ok = (i < SX * SY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+        const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) rsrc[q][m] = ok ? src[q][gy * nx + gx] : 0.0f;
+        for (int q = 0; q < NEQ; ++q) rsrc[q][m] = bufferLoad(bSrc[q], off);
     }
     // ---- stage to LDS ----
 #pragma unroll
