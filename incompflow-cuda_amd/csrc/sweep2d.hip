@@ -207,18 +207,20 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     // the staging registers are free
     float fUW[PC ? 1 : 2][NSL], fUE[PC ? 1 : 2][NSL], fVS[PC ? 1 : 2][NSL], fVN[PC ? 1 : 2][NSL];
     if constexpr (!PC) {
+        const __amdgpu_buffer_rsrc_t bUf = bufferOf(P.uf, 4 * (nx + 1) * ny), bVf = bufferOf(P.vf, 4 * nx * (ny + 1));
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int sl = 0; sl < NSL; ++sl) {
                 const int e = cell[c][sl];
                 const int gx = x0 - H + (e & 255), gy = y0 - H + (e >> 8);
-                const bool ok = e != 0xffff && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                const bool ok = (e != 0xffff) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
                 const int cx = ok ? gx : 0, cy = ok ? gy : 0;
-                fUW[c][sl] = P.uf[ufi(P, cx, cy, 0)];
-                fUE[c][sl] = P.uf[ufi(P, cx + 1, cy, 0)];
-                fVS[c][sl] = P.vf[vfi(P, cx, cy, 0)];
-                fVN[c][sl] = P.vf[vfi(P, cx, cy + 1, 0)];
+                const int ou = 4 * ufi(P, cx, cy, 0), ov = 4 * vfi(P, cx, cy, 0);
+                fUW[c][sl] = bufferLoad(bUf, ou);
+                fUE[c][sl] = bufferLoad(bUf, ou + 4);
+                fVS[c][sl] = bufferLoad(bVf, ov);
+                fVN[c][sl] = bufferLoad(bVf, ov + 4 * nx);
             }
     }
     // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*NT
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 const int e = cell[c][s];
                 const int lx = e & 255, ly = e >> 8;
                 const int gx = x0 - H + lx, gy = y0 - H + ly;
-                if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+                if (((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny)) {
                     const int o = c * CO + ly * HX2 + (lx >> 1);
                     const int ot = (1 - c) * CO + ly * HX2 + (lx >> 1);
                     const int w = ot + (lx & 1) - 1;
