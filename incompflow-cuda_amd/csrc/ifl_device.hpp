@@ -113,6 +113,17 @@ __device__ __forceinline__ bool ghostCopy(const CaseParams &P, int f) {
 // std::min(x, 0.0f) with its exact selection rule ((0 < x) ? 0 : x).
 __device__ __forceinline__ float min0(float x) { return (0.0f < x) ? 0.0f : x; }
 
+// Raw buffer access (gfx9 descriptor: stride 0, dword 3 = 0x00020000): loads at a byte offset at
+// or past num_records return 0.
+constexpr int kNoOffset = (int)0x80000000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const float *p, int nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, nbytes, 0x00020000);
+}
+__device__ __forceinline__ float bufferLoad(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+// (A case holds at most 2^28 cells, so the byte offsets of its fields fit in 31 bits.)
+
 // ---------------- FP64 reductions (fixed order) ----------------
 // acc + d * d in FP64 as one fused operation: the square of a float is exact in FP64, so this
 // equals the separate multiply and add bit for bit.

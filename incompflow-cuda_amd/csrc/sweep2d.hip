@@ -19,16 +19,6 @@
 
 namespace ifl {
 
-// Raw buffer access (gfx9 descriptor: stride 0, dword 3 = 0x00020000): loads at a byte offset at
-// or past num_records return 0.
-constexpr int kNoOffset = (int)0x80000000;
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const float *p, int nbytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, nbytes, 0x00020000);
-}
-__device__ __forceinline__ float bufferLoad(__amdgpu_buffer_rsrc_t r, int off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-
 // Cells a launch relaxes (those within 2K-1 of the tile), per colour, ordered by ring (ring 0 is
 // the tile, ring r the cells r outside it), row-major within a ring; an entry is ly * 256 + lx in
 // staged coordinates.  Every ring-r rectangle has even sides, so each colour holds half of it:
