@@ -86,9 +86,12 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     constexpr int CO = HY * HX2;                               // colour stride of a field plane
     constexpr int PL = 2 * CO;                                 // field plane stride
     constexpr int CS = (HY - 2) * HX2;                         // colour stride of a source plane (rows 1..HY-2)
-    constexpr int oSRC = NEQ * PL;                             // source planes
-    constexpr int oEX = NEQ * PL + NEQ * 2 * CS, oEY = oEX + PL;   // p' links towards +x, +y
-    constexpr int NLDS = oEX + (PC ? 2 * PL : 0);
+    constexpr int oSRC = NEQ * PL;                             // p' source plane
+    constexpr int oEX = NEQ * PL + 2 * CS, oEY = oEX + PL;     // p' links towards +x, +y
+    constexpr int UX = SX + 1, UY = SY, VX = SX, VY = SY + 1;  // face velocities of the relaxed cells
+    constexpr int NUT = PC ? 1 : (UX * UY + NT - 1) / NT, NVT = PC ? 1 : (VX * VY + NT - 1) / NT;
+    constexpr int oUF = NEQ * PL, oVF = oUF + UY * UX;         // momentum / temperature face planes
+    constexpr int NLDS = PC ? oEX + 2 * PL : oVF + VY * VX;
     static_assert(HX % 2 == 0, "colour split needs an even width");
     __shared__ float lds[NLDS];
     Blk blk;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         bSrc[q] = bufferOf(src[q], nbytes);
     }
     const __amdgpu_buffer_rsrc_t bE = bufferOf(cfE, nbytes), bN = bufferOf(cfN, nbytes);
-    float rphi[NEQ][NST], rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[NEQ][NSR];
+    float rphi[NEQ][NST], rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[PC ? 1 : 1][PC ? NSR : 1];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int i = tid + m * NT;
@@ -156,15 +159,47 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             rey[m] = bufferLoad(bN, off);
         }
     }
+    float ruf[NUT], rvf[NVT], csrc[PC ? 1 : NEQ][2][PC ? 1 : NSL];
+    if constexpr (PC) {
 #pragma unroll
-    for (int m = 0; m < NSR; ++m) {
-        const int i = tid + m * NT;
-        const int ly = i / SX, lx = i - ly * SX;
-        const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
-        const bool ok = (i < SX * SY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
-        const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
+        for (int m = 0; m < NSR; ++m) {
+            const int i = tid + m * NT;
+            const int ly = i / SX, lx = i - ly * SX;
+            const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
+            const bool ok = (i < SX * SY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+            rsrc[0][m] = bufferLoad(bSrc[0], ok ? 4 * (gy * nx + gx) : kNoOffset);
+        }
+    } else {
+        // face velocities over the relaxed cells, coalesced, for LDS; each cell's sources straight
+        // into registers (off-grid faces and cells read as 0)
+        const __amdgpu_buffer_rsrc_t bUf = bufferOf(P.uf, 4 * (nx + 1) * ny), bVf = bufferOf(P.vf, 4 * nx * (ny + 1));
 #pragma unroll
-        for (int q = 0; q < NEQ; ++q) rsrc[q][m] = bufferLoad(bSrc[q], off);
+        for (int m = 0; m < NUT; ++m) {
+            const int i = tid + m * NT;
+            const int ly = i / UX, lx = i - ly * UX;
+            const int fx = x0 - R0 + lx, gy = y0 - R0 + ly;
+            const bool ok = (i < UX * UY) & ((unsigned)fx <= (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+            ruf[m] = bufferLoad(bUf, ok ? 4 * ufi(P, fx, gy, 0) : kNoOffset);
+        }
+#pragma unroll
+        for (int m = 0; m < NVT; ++m) {
+            const int i = tid + m * NT;
+            const int ly = i / VX, lx = i - ly * VX;
+            const int gx = x0 - R0 + lx, fy = y0 - R0 + ly;
+            const bool ok = (i < VX * VY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)fy <= (unsigned)ny);
+            rvf[m] = bufferLoad(bVf, ok ? 4 * vfi(P, gx, fy, 0) : kNoOffset);
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int sl = 0; sl < NSL; ++sl) {
+                const int e = cell[c][sl];
+                const int gx = x0 - H + (e & 255), gy = y0 - H + (e >> 8);
+                const bool ok = (e != 0xffff) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+                const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
+#pragma unroll
+                for (int q = 0; q < NEQ; ++q) csrc[q][c][sl] = bufferLoad(bSrc[q], off);
+            }
     }
     // ---- stage to LDS ----
 #pragma unroll
@@ -182,36 +217,27 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         }
     }
 #pragma unroll
-    for (int m = 0; m < NSR; ++m) {
+    for (int m = 0; m < (PC ? NSR : 0); ++m) {
         const int i = tid + m * NT;
         if (i < SX * SY) {
             const int sy = i / SX, sx = i - sy * SX;
             const int lx = sx + 1, ly = sy + 1;            // field coordinates (R0 = H - 1)
             const int a = ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1);
-#pragma unroll
-            for (int q = 0; q < NEQ; ++q) lds[oSRC + q * 2 * CS + a] = rsrc[q][m];
+            lds[oSRC + a] = rsrc[0][m];
         }
     }
-    // for the momentum and temperature rows, the four face velocities of this thread's cells (in
-    // the grid for every cell of the grid, so read unconditionally at a clamped cell), loaded once
-    // the staging registers are free
-    float fUW[PC ? 1 : 2][NSL], fUE[PC ? 1 : 2][NSL], fVS[PC ? 1 : 2][NSL], fVN[PC ? 1 : 2][NSL];
+    // the face velocities of the momentum and temperature rows
     if constexpr (!PC) {
-        const __amdgpu_buffer_rsrc_t bUf = bufferOf(P.uf, 4 * (nx + 1) * ny), bVf = bufferOf(P.vf, 4 * nx * (ny + 1));
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int m = 0; m < NUT; ++m) {
+            const int i = tid + m * NT;
+            if (i < UX * UY) lds[oUF + i] = ruf[m];
+        }
 #pragma unroll
-            for (int sl = 0; sl < NSL; ++sl) {
-                const int e = cell[c][sl];
-                const int gx = x0 - H + (e & 255), gy = y0 - H + (e >> 8);
-                const bool ok = (e != 0xffff) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
-                const int cx = ok ? gx : 0, cy = ok ? gy : 0;
-                const int ou = 4 * ufi(P, cx, cy, 0), ov = 4 * vfi(P, cx, cy, 0);
-                fUW[c][sl] = bufferLoad(bUf, ou);
-                fUE[c][sl] = bufferLoad(bUf, ou + 4);
-                fVS[c][sl] = bufferLoad(bVf, ov);
-                fVN[c][sl] = bufferLoad(bVf, ov + 4 * nx);
-            }
+        for (int m = 0; m < NVT; ++m) {
+            const int i = tid + m * NT;
+            if (i < VX * VY) lds[oVF + i] = rvf[m];
+        }
     }
     // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*NT
     // records are loaded now and added after the relax phases, so their latency overlaps them ----
@@ -263,9 +289,8 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                     const int w = ot + (lx & 1) - 1;
                     own[c][s] = o;
                     wn[c][s] = w;
-#pragma unroll
-                    for (int q = 0; q < NEQ; ++q) cB[q][c][s] = lds[oSRC + q * 2 * CS + o - c * (CO - CS) - HX2];
                     if constexpr (PC) {
+                        cB[0][c][s] = lds[oSRC + o - c * (CO - CS) - HX2];
                         const float aE = lds[oEX + o], aW = lds[oEX + w];
                         const float aN = lds[oEY + o], aS = lds[oEY + ot - HX2];
                         cE[c][s] = aE;
@@ -274,7 +299,11 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                         cS[c][s] = aS;
                         cC[c][s] = -(aE + aW + aN + aS);
                     } else {
-                        const float uw = fUW[c][s], ue = fUE[c][s], vs = fVS[c][s], vn = fVN[c][s];
+#pragma unroll
+                        for (int q = 0; q < NEQ; ++q) cB[q][c][s] = csrc[q][c][s];
+                        const int sx = lx - 1, sy = ly - 1;    // face coordinates (R0 = H - 1)
+                        const float uw = lds[oUF + sy * UX + sx], ue = lds[oUF + sy * UX + sx + 1];
+                        const float vs = lds[oVF + sy * VX + sx], vn = lds[oVF + (sy + 1) * VX + sx];
                         float a[7];
                         if constexpr (KIND == COEF_TEMP) {
                             linkCoef<2>(lk, ue, uw, vn, vs, 0.0f, 0.0f, a);
