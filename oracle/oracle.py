@@ -45,6 +45,7 @@ def lib():
         L.orc_config_size.restype = C.c_size_t
         L.orc_record_size.restype = C.c_size_t
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_fault.argtypes = [C.c_int]
         dp = C.POINTER(C.c_double)
         L.orc_adj_setup.argtypes = [vp, C.c_int]
         L.orc_adj_sizes.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
@@ -69,6 +70,32 @@ def set_threads(n):
 
 def get_threads():
     return lib().orc_get_threads()
+
+
+# Oracle-only field ids: the v and w copies of the momentum matrix (IFL_FIELD_MCOEF is u's).
+FIELD_VCOEF = 100
+FIELD_WCOEF = 101
+
+# Planted mistakes (oracle/simple_oracle.hpp, FAULT_*): set only inside `planted_fault`.
+FAULT_DEFECT_SIGN = 1
+FAULT_DEFECT_NO_WEST = 2
+FAULT_REFINE_SUBTRACT = 3
+FAULT_INITFACE_RIGHT = 4
+FAULT_V_MATRIX_ROW = 5
+
+
+class planted_fault:
+    """Context manager: run the oracle with one planted mistake, then restore it."""
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        lib().orc_set_fault(self.code)
+
+    def __exit__(self, *exc):
+        lib().orc_set_fault(0)
+        return False
 
 
 class OracleError(RuntimeError):
@@ -106,7 +133,8 @@ class Oracle:
         self._check(lib().orc_get_field(self.h, field, out.ctypes.data_as(C.POINTER(C.c_double)), n))
         if not self.fp64:
             out = out.astype(np.float32)
-        return out.reshape(self.cfg.shape(field)) if shaped else out
+        shape = self.cfg.shape(_abi.FIELD_MCOEF if field in (FIELD_VCOEF, FIELD_WCOEF) else field)
+        return out.reshape(shape) if shaped else out
 
     def set(self, field, arr):
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())

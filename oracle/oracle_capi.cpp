@@ -144,6 +144,10 @@ extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
 
+// Planted mistakes (oracle::FAULT_*) for the tests that show a pin fails when the oracle is wrong.
+void orc_set_fault(int code) { oracle::g_fault = code; }
+int orc_get_fault(void) { return oracle::g_fault; }
+
 void orc_set_threads(int n) {
     if (n > 0) omp_set_num_threads(n);
 }

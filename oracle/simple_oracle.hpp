@@ -46,6 +46,22 @@ enum { wall = 0, inlet = 1, outlet = 2 };                            // constant
 
 inline int normal_dir(int face) { return face / 2; }  // E/W -> x, N/S -> y, T/B -> z
 
+// Planted mistakes for the tests that show a pin is sensitive (0 = none; test use only).  Each
+// code breaks exactly one stage the way a plausible transcription slip would.
+enum {
+    FAULT_NONE = 0,
+    FAULT_DEFECT_SIGN = 1,       // defect: the per-cell output is A phi - b instead of b - A phi
+    FAULT_DEFECT_NO_WEST = 2,    // defect: the west link term is dropped
+    FAULT_REFINE_SUBTRACT = 3,   // refinement: p' -= delta instead of p' += delta
+    FAULT_INITFACE_RIGHT = 4,    // initFaceVel: the face takes the right cell's value, not the mean
+    FAULT_V_MATRIX_ROW = 5       // v matrix: the wall rows skip the centre update (breaks I1)
+};
+inline int g_fault = FAULT_NONE;
+
+// The oracle keeps the reference's three momentum matrices (Solver.h:28-30, copied at
+// Solver.cu:167-171); IFL_FIELD_MCOEF is the u matrix, these ids read the v and w copies.
+enum { ORC_FIELD_VCOEF = 100, ORC_FIELD_WCOEF = 101 };
+
 template <class Real>
 class Simple {
 public:
@@ -184,7 +200,9 @@ public:
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 0; k < nz; ++k)
             for (int j = 0; j < ny; ++j) {
-                for (int i = 1; i < nx; ++i) uf[ufi(i, j, k)] = (u[cid(i - 1, j, k)] + u[cid(i, j, k)]) / 2;
+                for (int i = 1; i < nx; ++i)
+                    uf[ufi(i, j, k)] = g_fault == FAULT_INITFACE_RIGHT ? u[cid(i, j, k)]
+                                                                       : (u[cid(i - 1, j, k)] + u[cid(i, j, k)]) / 2;
                 if (j >= 1)
                     for (int i = 0; i < nx; ++i) vf[vfi(i, j, k)] = (v[cid(i, j - 1, k)] + v[cid(i, j, k)]) / 2;
                 if (dim == 3 && k >= 1)
@@ -295,7 +313,7 @@ public:
             Real *coef = coefs[comp];
             if (bct[f] == wall || bct[f] == inlet) {     // :304-320
                 Real phib = bcval(f, comp, q);
-                coef[iC] -= 2 * coef[iEx];
+                if (!(g_fault == FAULT_V_MATRIX_ROW && comp == 1)) coef[iC] -= 2 * coef[iEx];
                 coef[iIn] += coef[iEx] / 3;
                 srcs[comp][c] -= (Real)8 / 3 * coef[iEx] * phib;
                 coef[iEx] = 0;
@@ -493,14 +511,15 @@ public:
                 double rr = (double)src[c];
                 rr -= (double)coef[slot(c, aC)] * (double)fld[c];
                 rr -= (double)coef[slot(c, aE)] * (double)((i != nx - 1) ? fld[c + 1] : (Real)0);
-                rr -= (double)coef[slot(c, aW)] * (double)((i != 0) ? fld[c - 1] : (Real)0);
+                if (g_fault != FAULT_DEFECT_NO_WEST)
+                    rr -= (double)coef[slot(c, aW)] * (double)((i != 0) ? fld[c - 1] : (Real)0);
                 rr -= (double)coef[slot(c, aN)] * (double)((j != ny - 1) ? fld[c + nx] : (Real)0);
                 rr -= (double)coef[slot(c, aS)] * (double)((j != 0) ? fld[c - nx] : (Real)0);
                 if (dim == 3) {
                     rr -= (double)coef[slot(c, aT)] * (double)((k != nz - 1) ? fld[c + pl] : (Real)0);
                     rr -= (double)coef[slot(c, aB)] * (double)((k != 0) ? fld[c - pl] : (Real)0);
                 }
-                if (out) out[c] = (Real)rr;
+                if (out) out[c] = (Real)(g_fault == FAULT_DEFECT_SIGN ? -rr : rr);
                 s += rr * rr;
             }
             part[r] = s;
@@ -610,7 +629,10 @@ public:
                 std::fill(delta.begin(), delta.end(), (Real)0);
                 sw += iterate(delta.data(), pcoef.data(), rr.data(), nIter[3], relax[3], tol[3]);
                 lastSum = lastSweepSum;
-                for (int64_t c = 0; c < Nc; ++c) pc[c] += delta[c];
+                if (g_fault == FAULT_REFINE_SUBTRACT)
+                    for (int64_t c = 0; c < Nc; ++c) pc[c] -= delta[c];
+                else
+                    for (int64_t c = 0; c < Nc; ++c) pc[c] += delta[c];
             }
         }
         lastSweeps = sw;
@@ -833,6 +855,8 @@ public:
             case IFL_FIELD_BP: return &bp;
             case IFL_FIELD_TCOEF: return energy ? &tcoef : nullptr;
             case IFL_FIELD_BT: return energy ? &bt : nullptr;
+            case ORC_FIELD_VCOEF: return &vcoef;
+            case ORC_FIELD_WCOEF: return &wcoef;
             default: return nullptr;
         }
     }

@@ -164,15 +164,25 @@ def test_torch_binding_on_the_device():
     c, prof = channel_case(16, 8, noise=1e-2)
     c.stop_rule = _abi.STOP_NONE
     c.num_outer_iter = 5000
-    be = HipBackend(c, adjoint_tol=1e-6, adjoint_max_iter=20000)
+    be = HipBackend(c, adjoint_tol=1e-6, adjoint_max_iter=20000, require_converged=False)
+    runs = []
+    solve = be.solve
+    be.solve = lambda pr, m: runs.append(solve(pr, m)) or runs[-1]
     p = torch.tensor(prof, dtype=torch.float64, requires_grad=True)
     mu = torch.tensor(c.viscosity, dtype=torch.float64, requires_grad=True)
     J = steady_objective(p, mu, be)
     J.backward()
-    rep = be.adjoint_report
+    rep = runs[-1].adjoint_report
     assert np.array_equal(p.grad.numpy(), rep["dJ_dprofile"])
     assert mu.grad.item() == rep["dJ_dmu"]
     # the objective the binding returns is the FP64 oracle's to FP32 accuracy
     J64 = pressure_drop(converged(c, prof), c)
     assert abs(J.item() - J64) <= 1e-4 * abs(J64)
-    be.solver.close()
+    runs[-1].close()
+    # a solve that runs out of iterations without meeting its stop rule is refused by default
+    c2 = c.copy()
+    c2.stop_rule = _abi.STOP_SCALED
+    c2.tol_residual = 1e-12
+    c2.num_outer_iter = 50
+    with pytest.raises(RuntimeError, match="did not meet its stop rule"):
+        HipBackend(c2).solve(prof, c.viscosity)

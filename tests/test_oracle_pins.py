@@ -474,3 +474,194 @@ def test_sweep_block_only_delays_the_stop():
     c4.sweep_block_mom = 4
     h4 = Oracle(c4).solve(1)["history"][0]
     assert h4.sweeps[1] == 4
+
+
+# ---------------------------------------------------------------- FP64 defect, momentum residual,
+# refinement, initFaceVel and invariant I1 (independent checks; each shown to catch a planted slip)
+from oracle.oracle import (FAULT_DEFECT_NO_WEST, FAULT_DEFECT_SIGN, FAULT_INITFACE_RIGHT,  # noqa: E402
+                           FAULT_REFINE_SUBTRACT, FAULT_V_MATRIX_ROW, FIELD_VCOEF, FIELD_WCOEF,
+                           planted_fault)
+from helpers import load_state, random_state  # noqa: E402
+
+
+def _assembled(cfg, seed, fp64=True):
+    """Oracle at a random state with the momentum equations assembled (face BCs, matrix, sources)."""
+    o = Oracle(cfg, fp64=fp64)
+    load_state(o, random_state(cfg, seed))
+    o.stage(A.STAGE_MOM_ASSEMBLE)
+    return o
+
+
+def _resid_numpy(cfg, a, b, phi):
+    return np.linalg.norm(b.ravel() - _dense(cfg, a) @ phi.ravel())
+
+
+def _momentum_residual_matches(cfg, seed):
+    o = _assembled(cfg, seed)
+    out = o.stage(A.STAGE_MOM_RESIDUAL)
+    a = o.get(A.FIELD_MCOEF)
+    ok = True
+    comps = ((A.FIELD_U, A.FIELD_BU, A.FIELD_MCOEF), (A.FIELD_V, A.FIELD_BV, FIELD_VCOEF),
+             (A.FIELD_W, A.FIELD_BW, FIELD_WCOEF))[:cfg.dim]
+    for q, (fphi, fb, fa) in enumerate(comps):
+        ref = _resid_numpy(cfg, o.get(fa), o.get(fb), o.get(fphi))
+        ok &= abs(out[q] - ref) <= 1e-12 * ref
+    return ok, o, a
+
+
+@pytest.mark.parametrize("mk", [lambda: channel(9, 7), lambda: cavity(8, 10), lambda: duct3d(6, 5, 4)])
+def test_momentum_residual_is_b_minus_A_phi(mk):
+    """res_u, res_v (res_w) = ||b - A phi|| with A the assembled matrix as a dense numpy product."""
+    cfg = mk()
+    ok, _, _ = _momentum_residual_matches(cfg, 21)
+    assert ok
+    with planted_fault(FAULT_DEFECT_NO_WEST):
+        bad, _, _ = _momentum_residual_matches(cfg, 21)
+    assert not bad
+
+
+def test_outer_iteration_records_the_residuals_of_its_own_equations():
+    """The record of one outer iteration holds ||b - A phi|| of the momentum equations it assembled
+    (before the sweeps) and ||b'|| of the continuity equation after Rhie-Chow, with b' the
+    discrete divergence rho * sum(inflow - outflow) * area of the Rhie-Chow face velocities."""
+    cfg = channel(10, 6)
+    st = random_state(cfg, 4)
+    o = Oracle(cfg, fp64=True)
+    load_state(o, st)
+    o.set_iter(7)
+    rec = o.solve(1)["history"][0]
+    s = Oracle(cfg, fp64=True)
+    load_state(s, st)
+    s.stage(A.STAGE_MOM_ASSEMBLE)
+    for fphi, fb, fa, val in ((A.FIELD_U, A.FIELD_BU, A.FIELD_MCOEF, rec.res_u),
+                              (A.FIELD_V, A.FIELD_BV, FIELD_VCOEF, rec.res_v)):
+        ref = _resid_numpy(cfg, s.get(fa), s.get(fb), s.get(fphi))
+        assert val == pytest.approx(ref, rel=1e-12)
+    s.stage(A.STAGE_SMOOTH_U)
+    s.stage(A.STAGE_SMOOTH_V)
+    s.stage(A.STAGE_RHIE_CHOW)
+    uf, vf = s.get(A.FIELD_UF)[0], s.get(A.FIELD_VF)[0]
+    dx, dy, dz, Ax, Ay, Az = geom(cfg)
+    div = cfg.density * (Ax * (uf[:, :-1] - uf[:, 1:]) + Ay * (vf[:-1, :] - vf[1:, :]))
+    out = s.stage(A.STAGE_PCORR_ASSEMBLE)
+    np.testing.assert_allclose(s.get(A.FIELD_BP)[0], div, rtol=1e-12, atol=1e-15 * np.abs(div).max())
+    assert rec.res_cont == pytest.approx(np.linalg.norm(div), rel=1e-12)
+    assert out[0] == pytest.approx(np.linalg.norm(div) ** 2, rel=1e-12)
+
+
+def _pcorr_system(cfg, seed):
+    o = _assembled(cfg, seed)
+    o.stage(A.STAGE_RHIE_CHOW)
+    o.stage(A.STAGE_PCORR_ASSEMBLE)
+    return o.get(A.FIELD_PCOEF), o.get(A.FIELD_BP)
+
+
+def _p_solve(cfg, a, b, p_refine):
+    c = cfg.copy()
+    c.p_refine = p_refine
+    o = Oracle(c, fp64=True)
+    o.set(A.FIELD_PCOEF, a)
+    o.set(A.FIELD_BP, b)
+    o.set(A.FIELD_PCORR, np.zeros(cfg.field_size(A.FIELD_PCORR)))
+    out = o.stage(A.STAGE_SMOOTH_P)
+    return o.get(A.FIELD_PCORR), out
+
+
+def _refinement_is_one_defect_solve(cfg, a, b):
+    """p2 (one refinement cycle) == p1 + delta, delta = the same smoother from zero on the numpy
+    defect b - A p1; returns (identity holds, ||b - A p1||, ||b - A p2||)."""
+    M = _dense(cfg, a)
+    p1, o1 = _p_solve(cfg, a, b, 0)
+    p2, o2 = _p_solve(cfg, a, b, 1)
+    d = b.ravel() - M @ p1.ravel()
+    delta, od = _p_solve(cfg, a, d.reshape(b.shape), 0)
+    same = np.allclose(p2, p1 + delta, rtol=1e-12, atol=1e-13 * np.abs(p2).max())
+    assert o2[0] == o1[0] + od[0]          # sweeps of both solves are counted
+    return same, np.linalg.norm(d), np.linalg.norm(b.ravel() - M @ p2.ravel())
+
+
+@pytest.mark.parametrize("mk", [lambda: cavity(12, 10), lambda: channel(12, 8), lambda: duct3d(6, 5, 4)])
+def test_refinement_cycle_adds_the_smoothed_defect(mk):
+    cfg = mk()
+    cfg.tol[3] = 0.0              # every call runs its full n_iter sweeps
+    a, b = _pcorr_system(cfg, 13)
+    same, r1, r2 = _refinement_is_one_defect_solve(cfg, a, b)
+    assert same and r2 < r1
+    for fault in (FAULT_DEFECT_SIGN, FAULT_REFINE_SUBTRACT):
+        with planted_fault(fault):
+            p2, _ = _p_solve(cfg, a, b, 1)
+        p1, _ = _p_solve(cfg, a, b, 0)
+        delta, _ = _p_solve(cfg, a, (b.ravel() - _dense(cfg, a) @ p1.ravel()).reshape(b.shape), 0)
+        assert not np.allclose(p2, p1 + delta, rtol=1e-6, atol=0)
+        # a wrong-signed step moves p' away from the solution
+        assert np.linalg.norm(b.ravel() - _dense(cfg, a) @ p2.ravel()) > r1
+
+
+@pytest.mark.parametrize("mk", [lambda: channel(9, 6), lambda: duct3d(6, 5, 4)])
+def test_init_face_velocity_is_the_cell_average(mk):
+    """initFaceVel (kernels.cuh:10-45): interior faces take (phi_L + phi_R) / 2; the boundary faces
+    are not written."""
+    cfg = mk()
+
+    def run():
+        o = Oracle(cfg, fp64=True)
+        st = random_state(cfg, 8)
+        load_state(o, st)
+        o.stage(A.STAGE_INIT_FACES)
+        return o, st
+
+    o, st = run()
+    u = st[A.FIELD_U].reshape(cfg.shape(A.FIELD_U)).astype(np.float64)
+    v = st[A.FIELD_V].reshape(cfg.shape(A.FIELD_V)).astype(np.float64)
+    uf, vf = o.get(A.FIELD_UF), o.get(A.FIELD_VF)
+    assert np.array_equal(uf[:, :, 1:-1], 0.5 * (u[:, :, :-1] + u[:, :, 1:]))
+    assert np.array_equal(vf[:, 1:-1, :], 0.5 * (v[:, :-1, :] + v[:, 1:, :]))
+    uf0 = st[A.FIELD_UF].reshape(cfg.shape(A.FIELD_UF))
+    vf0 = st[A.FIELD_VF].reshape(cfg.shape(A.FIELD_VF))
+    assert np.array_equal(uf[:, :, [0, -1]], uf0[:, :, [0, -1]])
+    assert np.array_equal(vf[:, [0, -1], :], vf0[:, [0, -1], :])
+    if cfg.dim == 3:
+        w = st[A.FIELD_W].reshape(cfg.shape(A.FIELD_W)).astype(np.float64)
+        wf = o.get(A.FIELD_WF)
+        assert np.array_equal(wf[1:-1], 0.5 * (w[:-1] + w[1:]))
+        wf0 = st[A.FIELD_WF].reshape(cfg.shape(A.FIELD_WF))
+        assert np.array_equal(wf[[0, -1]], wf0[[0, -1]])
+    with planted_fault(FAULT_INITFACE_RIGHT):
+        ob, _ = run()
+    assert not np.array_equal(ob.get(A.FIELD_UF)[:, :, 1:-1], 0.5 * (u[:, :, :-1] + u[:, :, 1:]))
+
+
+def _i1_holds(cfg, seed):
+    o = Oracle(cfg)
+    load_state(o, random_state(cfg, seed))
+    o.stage(A.STAGE_MOM_ASSEMBLE)
+    a = o.get(A.FIELD_MCOEF).view(np.uint32)
+    same = np.array_equal(o.get(FIELD_VCOEF).view(np.uint32), a)
+    if cfg.dim == 3:
+        same &= np.array_equal(o.get(FIELD_WCOEF).view(np.uint32), a)
+    return same, o
+
+
+def test_i1_momentum_matrices_identical_and_shared():
+    """Invariant I1 (Solver.cu:167-171 copies the u matrix before the BC rows, and the rows change
+    the matrix identically for every component): after assembly on a 3D duct with an inlet, an
+    outlet and a moving wall (so the boundary values differ per component) the u, v and w
+    matrices are bitwise equal, and sweeping v with the u matrix gives v's own result, which is
+    what lets the device store one matrix for all components."""
+    cfg = duct3d(7, 6, 5)
+    cfg.bc_val[A.TOP][1] = -0.3
+    cfg.bc_val[A.WEST][2] = 0.1
+    same, o = _i1_holds(cfg, 17)
+    assert same
+    st = {f: o.get(f) for f in (A.FIELD_MCOEF, A.FIELD_BV, A.FIELD_V)}
+    o.stage(A.STAGE_SMOOTH_V)
+    v_own = o.get(A.FIELD_V).copy()
+    o2 = Oracle(cfg)
+    o2.set(A.FIELD_MCOEF, st[A.FIELD_MCOEF])   # sets the u, v and w matrices to the u matrix
+    o2.set(A.FIELD_BV, st[A.FIELD_BV])
+    o2.set(A.FIELD_V, st[A.FIELD_V])
+    o2.stage(A.STAGE_SMOOTH_V)
+    assert np.array_equal(o2.get(A.FIELD_V).view(np.uint32), v_own.view(np.uint32))
+    with planted_fault(FAULT_V_MATRIX_ROW):
+        bad, _ = _i1_holds(cfg, 17)
+    assert not bad
