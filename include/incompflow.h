@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define IFL_ABI_VERSION 2
+#define IFL_ABI_VERSION 3
 
 /* ---- status codes ---- */
 enum {
@@ -35,7 +35,8 @@ enum {
     IFL_ERR_ARG = 1,        /* invalid argument / config */
     IFL_ERR_DEVICE = 2,     /* HIP runtime error */
     IFL_ERR_DIVERGED = 3,   /* non-finite residual detected on the device */
-    IFL_ERR_NOT_CONVERGED = 4, /* stop rule not met within num_outer_iter (only when requested) */
+    IFL_ERR_NOT_CONVERGED = 4, /* stop rule not met within the iterations allowed (only with
+                                  ifl_config.fail_if_not_converged = 1) */
     IFL_ERR_IO = 5,
     IFL_ERR_NOMEM = 6
 };
@@ -58,6 +59,12 @@ enum {
     IFL_SMOOTHER_RBGS = 0,   /* red-black Gauss-Seidel (deterministic form of GaussSeidelIterate) */
     IFL_SMOOTHER_JACOBI = 1, /* point Jacobi, pointJacobiIterate with the result copied back */
     IFL_SMOOTHER_LEX = 2     /* lexicographic Gauss-Seidel: CPU oracle only */
+};
+
+/* ---- p' solver ---- */
+enum {
+    IFL_PSOLVE_SWEEPS = 0,   /* the reference's relaxed smoother call (Solver.cu:196), optionally refined */
+    IFL_PSOLVE_MG = 1        /* aggregation-multigrid V-cycles of controlled strength (mg_* fields) */
 };
 
 /* ---- outer stop rule ---- */
@@ -150,6 +157,24 @@ typedef struct ifl_config {
     int32_t sweep_block_mom;     /* the same for the momentum and temperature equations: 0 = the
                                     smaller of sweep_block and 2; 1, 2, 4 or 5.  Four or five pay on
                                     large grids with 128x32 tiles; on small tiles the halo costs more */
+    /* The p' solve.  The reference relaxes p' with nIter_p Gauss-Seidel sweeps (Solver.cu:196) and
+     * notes that large cases converge slowly (README.md:416).  IFL_PSOLVE_MG replaces that call by
+     * mg_cycles V-cycles of an aggregation multigrid on the same matrix (kernels.cuh:502-540):
+     * coarse cells join 2x2(x2) fine cells, coarse couplings are the sums of the fine couplings
+     * across the coarse face, every level is smoothed by unrelaxed red-black Gauss-Seidel, the
+     * residual and its restriction are FP64, and the coarse correction is scaled by mg_over.  The
+     * operator adds, on outlet faces, the coupling to the ghost value p' = 0 that the correction
+     * step applies there (kernels.cuh:713-727).  A converged state is the same for every p' solver
+     * (b' = 0 gives p' = 0); the solver only changes how fast it is reached.  A near-exact p' solve
+     * makes this SIMPLE variant diverge, so the strength (mg_cycles, mg_over) is a tuning choice
+     * (ifl_autotune, IFL_TUNE_PSOLVE).  2D only; not with p_refine or in a batch. */
+    int32_t p_solver;            /* IFL_PSOLVE_SWEEPS (default) or IFL_PSOLVE_MG */
+    int32_t mg_cycles;           /* V-cycles per p' solve, 1..8 (0 = 1) */
+    int32_t mg_sweeps;           /* red-black sweeps before and after each coarse correction, 1..3 (0 = 2) */
+    int32_t mg_coarse_sweeps;    /* red-black sweeps on the coarsest level, 1..200 (0 = 20) */
+    double mg_over;              /* coarse-correction factor, (0, 2] (0 = 1.0) */
+    int32_t fail_if_not_converged; /* 1: ifl_solve and ifl_batch_solve return IFL_ERR_NOT_CONVERGED when
+                                      the stop rule was not met within the iterations allowed */
     int32_t reserved[3];
 } ifl_config;
 

@@ -6,7 +6,7 @@ Field-for-field correspondence with the reference's compile-time contract:
 """
 import ctypes as C
 
-IFL_ABI_VERSION = 2
+IFL_ABI_VERSION = 3
 
 # status codes
 IFL_OK, IFL_ERR_ARG, IFL_ERR_DEVICE, IFL_ERR_DIVERGED, IFL_ERR_NOT_CONVERGED, IFL_ERR_IO, IFL_ERR_NOMEM = range(7)
@@ -22,6 +22,7 @@ TEMP_DIRICHLET, TEMP_NEUMANN = range(2)
 
 SMOOTHER_RBGS, SMOOTHER_JACOBI, SMOOTHER_LEX = range(3)
 STOP_NONE, STOP_FAITHFUL, STOP_FIXED, STOP_SCALED = range(4)
+PSOLVE_SWEEPS, PSOLVE_MG = range(2)
 
 (FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_TEMP, FIELD_PCORR, FIELD_UF, FIELD_VF, FIELD_WF,
  FIELD_MCOEF, FIELD_PCOEF, FIELD_BU, FIELD_BV, FIELD_BW, FIELD_BP, FIELD_TCOEF, FIELD_BT) = range(17)
@@ -91,6 +92,12 @@ class Config(C.Structure):
         ("temp_bc_type", C.c_int32 * 6),
         ("temp_bc_val", C.c_double * 6),
         ("sweep_block_mom", C.c_int32),
+        ("p_solver", C.c_int32),
+        ("mg_cycles", C.c_int32),
+        ("mg_sweeps", C.c_int32),
+        ("mg_coarse_sweeps", C.c_int32),
+        ("mg_over", C.c_double),
+        ("fail_if_not_converged", C.c_int32),
         ("reserved", C.c_int32 * 3),
     ]
 

@@ -46,6 +46,12 @@ def lib():
         L.orc_record_size.restype = C.c_size_t
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_fault.argtypes = [C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.orc_mg_build.argtypes = [vp, ip, ip]
+        L.orc_mg_get.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_mg_set.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.orc_mg_cycles.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.orc_mg_smooth.argtypes = [vp, C.c_int, C.c_int]
         dp = C.POINTER(C.c_double)
         L.orc_adj_setup.argtypes = [vp, C.c_int]
         L.orc_adj_sizes.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
@@ -82,6 +88,8 @@ FAULT_DEFECT_NO_WEST = 2
 FAULT_REFINE_SUBTRACT = 3
 FAULT_INITFACE_RIGHT = 4
 FAULT_V_MATRIX_ROW = 5
+FAULT_MG_PROLONG_SIGN = 6
+FAULT_MG_COARSE_EDGE = 7
 
 
 class planted_fault:
@@ -163,6 +171,35 @@ class Oracle:
 
     def set_iter(self, it):
         lib().orc_set_iter(self.h, int(it))
+
+    # ---- p' multigrid levels (IFL_PSOLVE_MG) ----
+    MG_X, MG_B, MG_C, MG_E, MG_N, MG_T, MG_DG = range(7)
+
+    def mg_build(self):
+        """Levels of the current p' system; returns [(nx, ny, nz), ...] from level 0."""
+        n = C.c_int()
+        dims = (C.c_int * 48)()
+        self._check(lib().orc_mg_build(self.h, C.byref(n), dims))
+        self.mg_dims = [(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]) for l in range(n.value)]
+        return self.mg_dims
+
+    def mg_get(self, level, which):
+        nx, ny, nz = self.mg_dims[level]
+        out = np.empty(nx * ny * nz)
+        self._check(lib().orc_mg_get(self.h, level, which, self._dp(out), out.size))
+        return out.reshape(nz, ny, nx)
+
+    def mg_set(self, level, which, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+        self._check(lib().orc_mg_set(self.h, level, which, self._dp(a), a.size))
+
+    def mg_cycles(self, n):
+        r = C.c_double()
+        self._check(lib().orc_mg_cycles(self.h, int(n), C.byref(r)))
+        return r.value
+
+    def mg_smooth(self, level, n):
+        self._check(lib().orc_mg_smooth(self.h, level, int(n)))
 
     # ---- discrete adjoint (oracle/adjoint_oracle.hpp) ----
     # Flat layouts: state = [u, v, w, p (N each), uf, vf, wf]; parameters = [face profiles

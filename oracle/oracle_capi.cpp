@@ -332,6 +332,80 @@ int orc_adj_gradient(void *hv, double *tbar) {
     return IFL_OK;
 }
 
+// ---- p' multigrid internals (IFL_PSOLVE_MG), for the tests that pin it ----
+// Build the levels from the current p' system (the p' matrix, b' and the momentum diagonal)
+// with x = 0; *nlevels and dims[3 * l + {0,1,2}] receive the level count and sizes.
+int orc_mg_build(void *hv, int *nlevels, int *dims) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s) {
+        s->mgBuild();
+        *nlevels = (int)s->mg.size();
+        for (size_t l = 0; l < s->mg.size(); ++l) {
+            dims[3 * l] = s->mg[l].nx;
+            dims[3 * l + 1] = s->mg[l].ny;
+            dims[3 * l + 2] = s->mg[l].nz;
+        }
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+// which: 0 x, 1 b, 2 C, 3 E, 4 N, 5 T, 6 outlet term
+int orc_mg_get(void *hv, int level, int which, double *dst, size_t n) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s) {
+        if (level < 0 || level >= (int)s->mg.size()) { g_err = "bad level"; return IFL_ERR_ARG; }
+        auto &L = s->mg[level];
+        auto *v = which == 0 ? &L.x : which == 1 ? &L.b : which == 2 ? &L.C : which == 3 ? &L.E : which == 4 ? &L.N
+                 : which == 5 ? &L.T : &L.dg;
+        if (n != v->size()) { g_err = "size mismatch"; return IFL_ERR_ARG; }
+        for (size_t q = 0; q < n; ++q) dst[q] = (double)(*v)[q];
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+int orc_mg_set(void *hv, int level, int which, const double *src, size_t n) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s) {
+        using R = typename std::remove_reference<decltype(s->u[0])>::type;
+        if (level < 0 || level >= (int)s->mg.size()) { g_err = "bad level"; return IFL_ERR_ARG; }
+        auto &L = s->mg[level];
+        auto *v = which == 0 ? &L.x : which == 1 ? &L.b : nullptr;
+        if (!v || n != v->size()) { g_err = "only x and b of a level can be set; size must match"; return IFL_ERR_ARG; }
+        for (size_t q = 0; q < n; ++q) (*v)[q] = (R)src[q];
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+// n V-cycles from the current level-0 x; *res receives the FP64 norm of b - A x on level 0.
+int orc_mg_cycles(void *hv, int n, double *res) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s) {
+        if (s->mg.empty()) { g_err = "no levels: call orc_mg_build first"; return IFL_ERR_ARG; }
+        for (int q = 0; q < n; ++q) s->mgCycle(0);
+        auto &L = s->mg[0];
+        double t = 0;
+        for (int k = 0; k < L.nz; ++k)
+            for (int j = 0; j < L.ny; ++j)
+                for (int i = 0; i < L.nx; ++i) {
+                    const double r = s->mgResidual(L, i, j, k);
+                    t += r * r;
+                }
+        if (res) *res = std::sqrt(t);
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+// n smoothing sweeps of one level (a smoother pin)
+int orc_mg_smooth(void *hv, int level, int n) {
+    Handle *h = (Handle *)hv;
+    auto go = [&](auto *s) {
+        if (level < 0 || level >= (int)s->mg.size()) { g_err = "bad level"; return IFL_ERR_ARG; }
+        s->mgSmooth(s->mg[level], n);
+        return IFL_OK;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+
 size_t orc_config_size(void) { return sizeof(ifl_config); }
 size_t orc_record_size(void) { return sizeof(ifl_iter_record); }
 

@@ -54,7 +54,9 @@ enum {
     FAULT_DEFECT_NO_WEST = 2,    // defect: the west link term is dropped
     FAULT_REFINE_SUBTRACT = 3,   // refinement: p' -= delta instead of p' += delta
     FAULT_INITFACE_RIGHT = 4,    // initFaceVel: the face takes the right cell's value, not the mean
-    FAULT_V_MATRIX_ROW = 5       // v matrix: the wall rows skip the centre update (breaks I1)
+    FAULT_V_MATRIX_ROW = 5,      // v matrix: the wall rows skip the centre update (breaks I1)
+    FAULT_MG_PROLONG_SIGN = 6,   // multigrid: the coarse correction is subtracted
+    FAULT_MG_COARSE_EDGE = 7     // multigrid: a coarse x-coupling sums the first fine column, not the last
 };
 inline int g_fault = FAULT_NONE;
 
@@ -85,6 +87,10 @@ public:
     int sweepBlock = 1;      // sweeps per block: a met stop test ends the call at the block's end
     int sweepBlockMom = 1;   // the same for the momentum and temperature equations
     int numOuterIter = 1;
+    // p' solver (build addition): the reference's sweeps, or aggregation-multigrid V-cycles
+    int pSolver = IFL_PSOLVE_SWEEPS;
+    int mgCycles = 1, mgSweeps = 2, mgCoarse = 20;
+    Real mgOver = 1;
     // temperature (config.h:19-23, 45-63; declared but never solved by the reference): the
     // momentum discretisation with mu -> k / c_p and the TempBCs rows (build addition)
     bool energy = false;
@@ -149,6 +155,13 @@ public:
         sweepBlock = (blocked && c.sweep_block > 1) ? c.sweep_block : 1;
         sweepBlockMom = !blocked ? 1 : c.sweep_block_mom > 0 ? c.sweep_block_mom : std::min(sweepBlock, 2);
         numOuterIter = c.num_outer_iter;
+        pSolver = c.p_solver;
+        if (pSolver != IFL_PSOLVE_SWEEPS && pSolver != IFL_PSOLVE_MG) return fail("bad p_solver");
+        mgCycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
+        mgSweeps = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
+        mgCoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 20;
+        mgOver = (Real)(c.mg_over > 0 ? c.mg_over : 1.0);
+        if (pSolver == IFL_PSOLVE_MG && c.p_refine > 0) return fail("p_refine needs p_solver = IFL_PSOLVE_SWEEPS");
         energy = c.energy != 0;
         gam = (Real)c.conductivity / (Real)c.specific_heat;
         for (int f = 0; f < 6; ++f) {
@@ -615,11 +628,198 @@ public:
         return tot;
     }
 
+    // ---- aggregation multigrid for the p' system (build addition, IFL_PSOLVE_MG) ----
+    // Level 0 is the assembled p' matrix (kernels.cuh:502-540) plus, in cells next to an outlet
+    // face, the coupling rho A^2 / aC to the ghost value p' = 0 that the velocity correction uses
+    // there (kernels.cuh:713-727); level l+1 joins the 2x2(x2) blocks of level l (the last block of
+    // an odd side holds one cell), while every side of level l is at least 4.  A coarse coupling is
+    // the sum of the fine couplings across the coarse face, the coarse outlet term the sum of its
+    // children's, and the coarse centre -(E + W + N + S) (-(T + B)) + outlet term, so the coarse
+    // matrices keep the fine ones' symmetry and zero row sums.  Smoothing is unrelaxed red-black
+    // Gauss-Seidel, x = (b - sum a_nb x_nb) / aC; the residual is FP64, restricted by summing the
+    // children (FP64, order k, j, i) and rounded once; the coarse correction is x += over * x_c.
+    struct MgLevel {
+        int nx = 0, ny = 0, nz = 0;
+        int64_t n = 0;
+        std::vector<Real> x, b, C, E, N, T, dg;
+    };
+    std::vector<MgLevel> mg;
+
+    static bool mgCoarsens(int nx_, int ny_, int nz_, int dim_) {
+        return nx_ >= 4 && ny_ >= 4 && (dim_ == 2 || nz_ >= 4);
+    }
+    // Outlet term of cell c (faces in order E, W, N, S, T, B).
+    Real mgOutletTerm(int i, int j, int k, int64_t c) const {
+        Real dg = 0;
+        const Real rinv = (Real)1 / ucoef[slot(c, aC)];
+        const bool at[6] = {i == nx - 1, i == 0, j == ny - 1, j == 0, k == nz - 1, k == 0};
+        for (int f = 0; f < 2 * dim; ++f) {
+            if (!at[f] || bct[f] != outlet) continue;
+            const Real A = f < 2 ? areaX : f < 4 ? areaY : areaZ;
+            dg += density * A * A * rinv;
+        }
+        return dg;
+    }
+    void mgBuild() {
+        mg.clear();
+        MgLevel l0;
+        l0.nx = nx; l0.ny = ny; l0.nz = nz; l0.n = Nc;
+        l0.x.assign(Nc, 0);
+        l0.b.assign(bp.begin(), bp.end());
+        l0.dg.assign(Nc, 0);
+        l0.C.assign(Nc, 0);
+        l0.E.assign(pcoef.begin() + Nc * aE, pcoef.begin() + Nc * (aE + 1));
+        l0.N.assign(pcoef.begin() + Nc * aN, pcoef.begin() + Nc * (aN + 1));
+        if (dim == 3) l0.T.assign(pcoef.begin() + Nc * aT, pcoef.begin() + Nc * (aT + 1));
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int64_t c = cid(i, j, k);
+                    l0.dg[c] = mgOutletTerm(i, j, k, c);
+                    l0.C[c] = pcoef[slot(c, aC)] + l0.dg[c];
+                }
+        mg.push_back(std::move(l0));
+        while ((int)mg.size() < 16 && mgCoarsens(mg.back().nx, mg.back().ny, mg.back().nz, dim)) {
+            const MgLevel &F = mg.back();
+            MgLevel Cl;
+            Cl.nx = (F.nx + 1) / 2; Cl.ny = (F.ny + 1) / 2; Cl.nz = dim == 3 ? (F.nz + 1) / 2 : 1;
+            Cl.n = (int64_t)Cl.nx * Cl.ny * Cl.nz;
+            Cl.x.assign(Cl.n, 0); Cl.b.assign(Cl.n, 0); Cl.C.assign(Cl.n, 0);
+            Cl.E.assign(Cl.n, 0); Cl.N.assign(Cl.n, 0); Cl.dg.assign(Cl.n, 0);
+            if (dim == 3) Cl.T.assign(Cl.n, 0);
+            auto fid = [&](int i, int j, int k) { return i + (int64_t)F.nx * (j + (int64_t)F.ny * k); };
+            for (int K = 0; K < Cl.nz; ++K)
+                for (int J = 0; J < Cl.ny; ++J)
+                    for (int I = 0; I < Cl.nx; ++I) {
+                        const int64_t C = I + (int64_t)Cl.nx * (J + (int64_t)Cl.ny * K);
+                        const int i0 = 2 * I, i1 = std::min(2 * I + 1, F.nx - 1);
+                        const int j0 = 2 * J, j1 = std::min(2 * J + 1, F.ny - 1);
+                        const int k0 = dim == 3 ? 2 * K : 0, k1 = dim == 3 ? std::min(2 * K + 1, F.nz - 1) : 0;
+                        Real e = 0, n = 0, t = 0, d = 0;
+                        const int ie = g_fault == FAULT_MG_COARSE_EDGE ? i0 : i1;
+                        for (int k = k0; k <= k1; ++k)
+                            for (int j = j0; j <= j1; ++j) e += F.E[fid(ie, j, k)];
+                        for (int k = k0; k <= k1; ++k)
+                            for (int i = i0; i <= i1; ++i) n += F.N[fid(i, j1, k)];
+                        if (dim == 3)
+                            for (int j = j0; j <= j1; ++j)
+                                for (int i = i0; i <= i1; ++i) t += F.T[fid(i, j, k1)];
+                        for (int k = k0; k <= k1; ++k)
+                            for (int j = j0; j <= j1; ++j)
+                                for (int i = i0; i <= i1; ++i) d += F.dg[fid(i, j, k)];
+                        Cl.E[C] = e; Cl.N[C] = n; Cl.dg[C] = d;
+                        if (dim == 3) Cl.T[C] = t;
+                    }
+            const int64_t pl = (int64_t)Cl.nx * Cl.ny;
+            for (int K = 0; K < Cl.nz; ++K)
+                for (int J = 0; J < Cl.ny; ++J)
+                    for (int I = 0; I < Cl.nx; ++I) {
+                        const int64_t C = I + (int64_t)Cl.nx * (J + (int64_t)Cl.ny * K);
+                        const Real w = I > 0 ? Cl.E[C - 1] : (Real)0, s_ = J > 0 ? Cl.N[C - Cl.nx] : (Real)0;
+                        Real cc = -(Cl.E[C] + w + Cl.N[C] + s_);
+                        if (dim == 3) {
+                            const Real b_ = K > 0 ? Cl.T[C - pl] : (Real)0;
+                            cc += -(Cl.T[C] + b_);
+                        }
+                        Cl.C[C] = cc + Cl.dg[C];
+                    }
+            mg.push_back(std::move(Cl));
+        }
+    }
+    // n unrelaxed red-black sweeps of level L (colour (i+j+k)&1, red first).
+    void mgSmooth(MgLevel &L, int n) {
+        const int64_t pl = (int64_t)L.nx * L.ny;
+        const int rows = L.ny * L.nz;
+        for (int s = 0; s < n; ++s)
+            for (int col = 0; col < 2; ++col) {
+#pragma omp parallel for schedule(static)
+                for (int r = 0; r < rows; ++r) {
+                    const int j = r % L.ny, k = r / L.ny;
+                    for (int i = (col + j + k) & 1; i < L.nx; i += 2) {
+                        const int64_t c = i + (int64_t)L.nx * r;
+                        const Real xE = i < L.nx - 1 ? L.x[c + 1] : (Real)0, xW = i > 0 ? L.x[c - 1] : (Real)0;
+                        const Real xN = j < L.ny - 1 ? L.x[c + L.nx] : (Real)0, xS = j > 0 ? L.x[c - L.nx] : (Real)0;
+                        const Real aW_ = i > 0 ? L.E[c - 1] : (Real)0, aS_ = j > 0 ? L.N[c - L.nx] : (Real)0;
+                        Real v = L.b[c] - L.E[c] * xE - aW_ * xW - L.N[c] * xN - aS_ * xS;
+                        if (dim == 3) {
+                            const Real xT = k < L.nz - 1 ? L.x[c + pl] : (Real)0, xB = k > 0 ? L.x[c - pl] : (Real)0;
+                            const Real aB_ = k > 0 ? L.T[c - pl] : (Real)0;
+                            v -= L.T[c] * xT + aB_ * xB;
+                        }
+                        L.x[c] = v / L.C[c];
+                    }
+                }
+            }
+    }
+    // FP64 residual of level L at cell (i, j, k), terms in the order C, E, W, N, S, T, B.
+    double mgResidual(const MgLevel &L, int i, int j, int k) const {
+        const int64_t pl = (int64_t)L.nx * L.ny, c = i + (int64_t)L.nx * (j + (int64_t)L.ny * k);
+        double r = (double)L.b[c];
+        r -= (double)L.C[c] * (double)L.x[c];
+        r -= (double)L.E[c] * (double)(i < L.nx - 1 ? L.x[c + 1] : (Real)0);
+        r -= (double)(i > 0 ? L.E[c - 1] : (Real)0) * (double)(i > 0 ? L.x[c - 1] : (Real)0);
+        r -= (double)L.N[c] * (double)(j < L.ny - 1 ? L.x[c + L.nx] : (Real)0);
+        r -= (double)(j > 0 ? L.N[c - L.nx] : (Real)0) * (double)(j > 0 ? L.x[c - L.nx] : (Real)0);
+        if (dim == 3) {
+            r -= (double)L.T[c] * (double)(k < L.nz - 1 ? L.x[c + pl] : (Real)0);
+            r -= (double)(k > 0 ? L.T[c - pl] : (Real)0) * (double)(k > 0 ? L.x[c - pl] : (Real)0);
+        }
+        return r;
+    }
+    void mgRestrict(const MgLevel &F, MgLevel &Cl) {
+#pragma omp parallel for schedule(static)
+        for (int R = 0; R < Cl.ny * Cl.nz; ++R) {
+            const int J = R % Cl.ny, K = R / Cl.ny;
+            for (int I = 0; I < Cl.nx; ++I) {
+                const int i1 = std::min(2 * I + 1, F.nx - 1), j1 = std::min(2 * J + 1, F.ny - 1);
+                const int k0 = dim == 3 ? 2 * K : 0, k1 = dim == 3 ? std::min(2 * K + 1, F.nz - 1) : 0;
+                double acc = 0.0;
+                for (int k = k0; k <= k1; ++k)
+                    for (int j = 2 * J; j <= j1; ++j)
+                        for (int i = 2 * I; i <= i1; ++i) acc += mgResidual(F, i, j, k);
+                const int64_t C = I + (int64_t)Cl.nx * R;
+                Cl.b[C] = (Real)acc;
+                Cl.x[C] = 0;
+            }
+        }
+    }
+    void mgProlong(MgLevel &F, const MgLevel &Cl) {
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < F.ny * F.nz; ++r) {
+            const int j = r % F.ny, k = r / F.ny;
+            for (int i = 0; i < F.nx; ++i) {
+                const int64_t c = i + (int64_t)F.nx * r;
+                const int64_t P_ = i / 2 + (int64_t)Cl.nx * (j / 2 + (int64_t)Cl.ny * (dim == 3 ? k / 2 : 0));
+                F.x[c] = g_fault == FAULT_MG_PROLONG_SIGN ? F.x[c] - mgOver * Cl.x[P_] : F.x[c] + mgOver * Cl.x[P_];
+            }
+        }
+    }
+    void mgCycle(size_t l) {
+        if (l + 1 == mg.size()) {
+            mgSmooth(mg[l], mgCoarse);
+            return;
+        }
+        mgSmooth(mg[l], mgSweeps);
+        mgRestrict(mg[l], mg[l + 1]);
+        mgCycle(l + 1);
+        mgProlong(mg[l], mg[l + 1]);
+        mgSmooth(mg[l], mgSweeps);
+    }
+    int solvePressureCorrectionMG() {
+        mgBuild();
+        for (int q = 0; q < mgCycles; ++q) mgCycle(0);
+        std::copy(mg[0].x.begin(), mg[0].x.end(), pc.begin());
+        lastSweeps = mgCycles * 2 * mgSweeps;
+        lastSweepSum = 0;
+        return lastSweeps;
+    }
+
     // Solver.cu:193-196 p' = 0 and its smoother, followed by pRefine FP64-defect refinement
     // cycles (build addition): d = b' - A'p' in FP64, A'delta = (Real)d with the same
     // smoother from delta = 0, p' += delta.  Returns sweeps executed.
     int solvePressureCorrection() {
         std::fill(pc.begin(), pc.end(), (Real)0);
+        if (pSolver == IFL_PSOLVE_MG) return solvePressureCorrectionMG();
         int sw = iterate(pc.data(), pcoef.data(), bp.data(), nIter[3], relax[3], tol[3]);
         double lastSum = lastSweepSum;
         if (pRefine > 0) {

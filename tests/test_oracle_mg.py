@@ -1,0 +1,261 @@
+"""Pins of the oracle's multigrid p' solve (IFL_PSOLVE_MG), on the CPU.
+
+* the level-0 operator is the assembled p' matrix plus the outlet ghost coupling rho A^2 / aC;
+* every coarse operator is the Galerkin product P^T A P of the piecewise-constant aggregation P
+  (dense numpy products on grids with odd sides, 2D and 3D, wall, inlet and outlet faces);
+* one smoothing sweep equals a plain Python red-black loop;
+* V-cycles converge to numpy.linalg.solve of the level-0 operator, with a contraction per cycle;
+* the converged flow does not depend on the p' solver;
+* a near-exact p' solve (500 sweeps or several V-cycles) makes the SIMPLE loop diverge while one
+  V-cycle converges in fewer outer iterations than the reference's 20 sweeps.
+Each structural check is shown to fail under a planted mistake in the oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import cavity, channel, duct3d, load_state, random_state
+from incompflow import _abi
+from oracle.oracle import FAULT_MG_COARSE_EDGE, FAULT_MG_PROLONG_SIGN, Oracle, planted_fault
+
+A = _abi
+
+
+def _p_system(cfg, seed=3):
+    """FP64 oracle holding the p' system of one outer iteration from a random state."""
+    o = Oracle(cfg, fp64=True)
+    load_state(o, random_state(cfg, seed))
+    o.stage(A.STAGE_MOM_ASSEMBLE)
+    o.stage(A.STAGE_RHIE_CHOW)
+    o.stage(A.STAGE_PCORR_ASSEMBLE)
+    return o
+
+
+def _level_matrix(o, l, dim):
+    """Dense matrix of level l from its stored centre and +x/+y(/+z) couplings."""
+    nx, ny, nz = o.mg_dims[l]
+    C = o.mg_get(l, o.MG_C).ravel()
+    E = o.mg_get(l, o.MG_E).ravel()
+    Nn = o.mg_get(l, o.MG_N).ravel()
+    T = o.mg_get(l, o.MG_T).ravel() if dim == 3 else None
+    n = nx * ny * nz
+    M = np.diag(C)
+    idx = np.arange(n)
+    i, j, k = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    for mask, off, a in ((i < nx - 1, 1, E), (j < ny - 1, nx, Nn)) + (((k < nz - 1, nx * ny, T),) if dim == 3 else ()):
+        c = idx[mask]
+        M[c, c + off] = a[c]
+        M[c + off, c] = a[c]           # the coupling towards -x of c + 1 is the +x one of c
+    return M
+
+
+def _aggregation(fine, coarse):
+    (fx, fy, fz), (cx, cy, cz) = fine, coarse
+    n = fx * fy * fz
+    idx = np.arange(n)
+    i, j, k = idx % fx, (idx // fx) % fy, idx // (fx * fy)
+    kk = k // 2 if cz < fz else k
+    par = i // 2 + cx * (j // 2 + cy * kk)
+    P = np.zeros((n, cx * cy * cz))
+    P[idx, par] = 1.0
+    return P
+
+
+def _outlet_channel(nx=19, ny=14):
+    c = channel(nx, ny)
+    c.bc_type[A.NORTH] = A.OUTLET         # a corner cell touches two outlet faces
+    c.p_solver = A.PSOLVE_MG
+    return c
+
+
+def _outlet_box(nx=19, ny=14):
+    """Unit square, west inlet, east and north outlets: near-square cells (the x and y couplings
+    of similar size), where point smoothing and isotropic aggregation contract well."""
+    c = cavity(nx, ny)
+    c.bc_type[A.WEST] = A.INLET
+    c.bc_val[A.WEST][0] = 0.5
+    c.bc_type[A.EAST] = A.OUTLET
+    c.bc_type[A.NORTH] = A.OUTLET
+    c.p_solver = A.PSOLVE_MG
+    return c
+
+
+CASES = {
+    "channel_odd": lambda: _outlet_channel(),
+    "outlet_box": lambda: _outlet_box(),
+    "cavity": lambda: cavity(16, 12),
+    "duct3d": lambda: duct3d(9, 7, 6),
+}
+
+
+def _galerkin_holds(cfg):
+    o = _p_system(cfg)
+    dims = o.mg_build()
+    assert len(dims) >= 2
+    ok = True
+    for l in range(len(dims) - 1):
+        Af = _level_matrix(o, l, cfg.dim)
+        Ac = _level_matrix(o, l + 1, cfg.dim)
+        P = _aggregation(dims[l], dims[l + 1])
+        G = P.T @ Af @ P
+        ok &= np.allclose(Ac, G, rtol=1e-12, atol=1e-13 * np.abs(G).max())
+    return ok, o, dims
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_coarse_operators_are_galerkin_products(name):
+    cfg = CASES[name]()
+    ok, o, dims = _galerkin_holds(cfg)
+    assert ok
+    # coarsening stops once a side would drop below 2; every level halves (rounding up)
+    for (a, b) in zip(dims[:-1], dims[1:]):
+        assert b[0] == (a[0] + 1) // 2 and b[1] == (a[1] + 1) // 2
+    assert min(dims[-1][:cfg.dim]) < 4 and min(dims[-2][:cfg.dim]) >= 4
+    with planted_fault(FAULT_MG_COARSE_EDGE):
+        bad, _, _ = _galerkin_holds(cfg)
+    assert not bad
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_level0_operator_is_the_p_matrix_plus_outlet_coupling(name):
+    cfg = CASES[name]()
+    o = _p_system(cfg)
+    o.mg_build()
+    a = o.get(A.FIELD_PCOEF)
+    aC = o.get(A.FIELD_MCOEF)[A.SLOT_C]
+    dx, dy, dz = ((cfg.xmax - cfg.xmin) / cfg.nx, (cfg.ymax - cfg.ymin) / cfg.ny, (cfg.zmax - cfg.zmin) / cfg.nz)
+    Ax, Ay, Az = dy * dz, dz * dx, dx * dy
+    dg = np.zeros_like(aC)
+    faces = ((A.EAST, (Ellipsis, -1), Ax), (A.WEST, (Ellipsis, 0), Ax), (A.NORTH, (slice(None), -1, slice(None)), Ay),
+             (A.SOUTH, (slice(None), 0, slice(None)), Ay), (A.TOP, (-1,), Az), (A.BOTTOM, (0,), Az))
+    for f, sl, area in faces[:2 * cfg.dim]:
+        if cfg.bc_type[f] == A.OUTLET:
+            dg[sl] += cfg.density * area * area / aC[sl]
+    np.testing.assert_allclose(o.mg_get(0, o.MG_DG), dg, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(o.mg_get(0, o.MG_C), a[A.SLOT_C] + dg, rtol=1e-14)
+    assert np.array_equal(o.mg_get(0, o.MG_E), a[A.SLOT_E])
+    assert np.array_equal(o.mg_get(0, o.MG_N), a[A.SLOT_N])
+    assert np.array_equal(o.mg_get(0, o.MG_B), o.get(A.FIELD_BP))
+    if name == "channel_odd":                           # the north-east corner holds both outlet terms
+        one = cfg.density * Ax * Ax / aC[0, -1, -1]
+        assert dg[0, -1, -1] == pytest.approx(one + cfg.density * Ay * Ay / aC[0, -1, -1], rel=1e-14)
+
+
+def test_smoothing_sweep_matches_textbook_loop():
+    cfg = _outlet_channel(7, 6)
+    o = _p_system(cfg)
+    o.mg_build()
+    rng = np.random.default_rng(4)
+    x0 = rng.uniform(-1, 1, (cfg.ny, cfg.nx))
+    o.mg_set(0, o.MG_X, x0)
+    o.mg_smooth(0, 1)
+    C, E, Nn, b = (o.mg_get(0, w)[0] for w in (o.MG_C, o.MG_E, o.MG_N, o.MG_B))
+    x = x0.copy()
+    for colour in (0, 1):
+        for j in range(cfg.ny):
+            for i in range(cfg.nx):
+                if (i + j) % 2 != colour:
+                    continue
+                acc = b[j, i]
+                if i + 1 < cfg.nx:
+                    acc -= E[j, i] * x[j, i + 1]
+                if i > 0:
+                    acc -= E[j, i - 1] * x[j, i - 1]
+                if j + 1 < cfg.ny:
+                    acc -= Nn[j, i] * x[j + 1, i]
+                if j > 0:
+                    acc -= Nn[j - 1, i] * x[j - 1, i]
+                x[j, i] = acc / C[j, i]
+    np.testing.assert_allclose(o.mg_get(0, o.MG_X)[0], x, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", ["outlet_box", "duct3d"])
+def test_vcycles_converge_to_the_dense_solution(name):
+    """With an outlet the level-0 operator is nonsingular: the cycles converge to its solution.
+    (On the anisotropic channel cells of the reference's 512x128 case, dx/dy = 3.7, the same cycles
+    contract at only ~0.93 per cycle: a point smoother with isotropic aggregation is weak across the
+    strong couplings; the outer iteration still gains, see the divergence test below.)"""
+    cfg = CASES[name]()
+    cfg.mg_over = 1.4
+    o = _p_system(cfg)
+    o.mg_build()
+    M = _level_matrix(o, 0, cfg.dim)
+    b = o.mg_get(0, o.MG_B).ravel()
+    x = np.linalg.solve(M, b)
+    r0 = np.linalg.norm(b)
+    res = [o.mg_cycles(1) / r0 for _ in range(60)]
+    np.testing.assert_allclose(o.mg_get(0, o.MG_X).ravel(), x, rtol=1e-9, atol=1e-10 * np.abs(x).max())
+    # a contraction per cycle, measured over cycles 5..15 (before FP64 rounding is reached)
+    rate = (res[15] / res[5]) ** (1 / 10)
+    assert rate < 0.75, rate
+    with planted_fault(FAULT_MG_PROLONG_SIGN):
+        o2 = _p_system(cfg)
+        o2.mg_build()
+        bad = [o2.mg_cycles(1) / r0 for _ in range(16)]
+    assert (bad[15] / bad[5]) ** (1 / 10) > 0.9
+
+
+def test_cavity_cycles_reduce_the_singular_residual():
+    """All walls: the p' system is singular but consistent (sum b' = 0); the residual still falls."""
+    o = _p_system(cavity(16, 12))
+    o.mg_build()
+    r0 = np.linalg.norm(o.mg_get(0, o.MG_B))
+    r = o.mg_cycles(30)
+    assert r < 1e-8 * r0
+
+
+@pytest.mark.parametrize("mk", [lambda: _abi.cavity_config(n=16, p_refine=0, tol_residual=1e-10, max_iter=20000),
+                                lambda: _outlet_channel(24, 8)])
+def test_fixed_point_independent_of_p_solver(mk):
+    fields = []
+    for ps in (A.PSOLVE_SWEEPS, A.PSOLVE_MG):
+        cfg = mk()
+        cfg.stop_rule = A.STOP_SCALED
+        cfg.tol_residual = 1e-10
+        cfg.num_outer_iter = 20000
+        cfg.p_solver = ps
+        o = Oracle(cfg, fp64=True)
+        r = o.solve(20000, history=False)
+        assert r["converged"], (ps, r)
+        fields.append([o.get(f) for f in (A.FIELD_U, A.FIELD_V, A.FIELD_P)])
+    for q, (a, b) in enumerate(zip(*fields)):
+        if q == 2 and mk().bc_type[A.EAST] == A.WALL:
+            a, b = a - a.mean(), b - b.mean()
+        assert np.max(np.abs(a - b)) <= 1e-7 * np.max(np.abs(b)), q
+
+
+def test_strong_p_solve_diverges_controlled_strength_converges_faster():
+    """The reference applies alpha_p inside the p' sweeps and again in the update, and its outer
+    loop converges because the p' solve is weak: on a 32x32 cavity (Re 100, alpha 0.7/0.3) the
+    reference's 20 sweeps reach a 1e-4 scaled residual, 500 sweeps or four V-cycles (a near-exact
+    p' solve) diverge, and one V-cycle converges in far fewer outer iterations."""
+    base = _abi.cavity_config(n=32, p_refine=0, tol_residual=1e-4, max_iter=3000)
+
+    def run(c, maxit):
+        r = Oracle(c, fp64=True).solve(maxit, history=False)
+        return r["iterations"], r["converged"], r["status"]
+
+    n20, conv, st = run(base, 3000)
+    assert conv and st == 0
+    c = base.copy()
+    c.n_iter[3] = 500
+    c.tol[3] = 0.0
+    n500, conv, st = run(c, 400)
+    assert not conv and st == A.IFL_ERR_DIVERGED
+    c = base.copy()
+    c.p_solver = A.PSOLVE_MG
+    c.mg_cycles = 4
+    _, conv, st = run(c, 1000)
+    assert not conv and st == A.IFL_ERR_DIVERGED
+    c.mg_cycles = 1
+    c.mg_over = 1.4
+    n_mg, conv, st = run(c, 3000)
+    assert conv and st == 0
+    assert n_mg < 0.5 * n20, (n_mg, n20)
+
+
+def test_mg_rejects_refinement():
+    c = cavity(16, 16)
+    c.p_solver = A.PSOLVE_MG
+    c.p_refine = 1
+    with pytest.raises(Exception, match="p_refine"):
+        Oracle(c)
