@@ -293,13 +293,16 @@ enum {
     IFL_KERNEL_MOM_SWEEP = 5,     /* one launch of the momentum smoother (u, v and, in 3D, w swept
                                      together on their shared matrix) */
     IFL_KERNEL_TEMP_SWEEP = 6,    /* one launch of the temperature smoother (energy on) */
-    IFL_KERNEL_COUNT = 7
+    IFL_KERNEL_PSOLVE = 7,        /* the whole p' solve of an outer iteration (its sweeps and
+                                     refinement cycles, or its multigrid cycles), timed as one unit */
+    IFL_KERNEL_COUNT = 8
 };
 
 /* Measurement hook (no reference counterpart): launch one kernel `reps` times on the handle's
  * stream, each between its own pair of HIP events, and return the mean device time per launch
- * in microseconds (*launches_per_rep receives the launches per repetition, 1).  Overwrites the
- * handle's device state of that stage. */
+ * in microseconds (*launches_per_rep receives 1).  The handle's whole device state is restored
+ * afterwards (one copy of its arena is made for that), so a solve continues as if nothing had
+ * been timed. */
 int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *launches_per_rep);
 
 /* ---- Autotuner (no reference counterpart) ----
@@ -312,8 +315,13 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
  * Trials start from the current state (a case that has run fewer than 20 outer iterations is
  * first advanced to 20 with its current settings; those iterations stay) and the state is
  * restored after every trial, so the solve continues from where it was, with the chosen
- * settings written into the handle's configuration.  trial_iters <= 0 selects 300. */
-enum { IFL_TUNE_LAUNCH = 1, IFL_TUNE_SWEEPS = 2 };
+ * settings written into the handle's configuration (p_refine is set to 0 when the multigrid is
+ * chosen).  On an error return the configuration is the one the call started with.
+ * trial_iters <= 0 selects 300. */
+enum { IFL_TUNE_LAUNCH = 1, IFL_TUNE_SWEEPS = 2,
+       IFL_TUNE_PSOLVE = 4 };   /* 2D: the p' solver and its strength (the configured sweeps, or one
+                                   multigrid V-cycle with mg_over 1.0, 1.4 or 1.8), by the drop of the
+                                   scaled residual per second over trial_iters outer iterations */
 
 typedef struct ifl_tune_result {
     int32_t tile;              /* chosen tile (0 in 3D) */
@@ -323,6 +331,9 @@ typedef struct ifl_tune_result {
     double decades_per_s;      /* scaled-residual decades per second with the choice (0: not measured) */
     int32_t trials;            /* configurations measured */
     int32_t sweep_block_mom;   /* chosen momentum sweeps per launch (0: the default rule) */
+    int32_t p_solver;          /* chosen p' solver (IFL_PSOLVE_*) */
+    int32_t mg_cycles;         /* its V-cycles (multigrid) */
+    double mg_over;            /* its coarse-correction factor (multigrid) */
 } ifl_tune_result;
 
 int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_result *out);

@@ -55,7 +55,7 @@ def default_tile(nx, ny):
     return TILE_32x4
 
 (KERNEL_SWEEP, KERNEL_MOM_ASSEMBLE, KERNEL_RHIE_CHOW, KERNEL_PCORR_ASSEMBLE, KERNEL_UPDATE, KERNEL_MOM_SWEEP,
- KERNEL_TEMP_SWEEP) = range(7)
+ KERNEL_TEMP_SWEEP, KERNEL_PSOLVE) = range(8)
 
 
 class Config(C.Structure):
@@ -150,7 +150,7 @@ class Report(C.Structure):
                 ("sweeps_total", C.c_double)]
 
 
-TUNE_LAUNCH, TUNE_SWEEPS = 1, 2
+TUNE_LAUNCH, TUNE_SWEEPS, TUNE_PSOLVE = 1, 2, 4
 OBJ_PRESSURE_DROP, OBJ_LINEAR = 0, 1
 
 
@@ -168,7 +168,7 @@ class AdjointReport(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tile", C.c_int32), ("sweep_block", C.c_int32), ("n_iter", C.c_int32 * 4),
                 ("us_per_iter", C.c_double), ("decades_per_s", C.c_double), ("trials", C.c_int32),
-                ("sweep_block_mom", C.c_int32)]
+                ("sweep_block_mom", C.c_int32), ("p_solver", C.c_int32), ("mg_cycles", C.c_int32), ("mg_over", C.c_double)]
 
 
 def default_config():

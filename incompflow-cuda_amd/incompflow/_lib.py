@@ -111,7 +111,7 @@ class Solver:
         hist = (_abi.IterRecord * max(cap, 1))()
         rep = _abi.Report()
         st = lib().ifl_solve(self.h, max_iter, C.byref(rep), hist, cap)
-        if st not in (0, _abi.IFL_ERR_DIVERGED):
+        if st not in (0, _abi.IFL_ERR_DIVERGED, _abi.IFL_ERR_NOT_CONVERGED):
             _check(st)
         recs = [hist[i] for i in range(min(rep.iterations, cap))]
         return {"iterations": rep.iterations, "converged": bool(rep.converged), "status": st,
@@ -215,8 +215,13 @@ class Solver:
         self.cfg.sweep_block_mom = r.sweep_block_mom
         for q in range(4):
             self.cfg.n_iter[q] = r.n_iter[q]
+        self.cfg.p_solver = r.p_solver
+        self.cfg.mg_cycles = r.mg_cycles
+        self.cfg.mg_over = r.mg_over
+        if r.p_solver == _abi.PSOLVE_MG:
+            self.cfg.p_refine = 0
         return {"tile": r.tile, "sweep_block": r.sweep_block, "sweep_block_mom": r.sweep_block_mom,
-                "n_iter": list(r.n_iter),
+                "n_iter": list(r.n_iter), "p_solver": r.p_solver, "mg_cycles": r.mg_cycles, "mg_over": r.mg_over,
                 "us_per_iter": r.us_per_iter, "decades_per_s": r.decades_per_s, "trials": r.trials}
 
 
@@ -264,7 +269,7 @@ class Batch:
         """Run until every case stopped; returns (status, [dict per case])."""
         reps = (_abi.Report * len(self.cases))()
         st = lib().ifl_batch_solve(self.h, max_iter, reps)
-        if st not in (0, _abi.IFL_ERR_DIVERGED):
+        if st not in (0, _abi.IFL_ERR_DIVERGED, _abi.IFL_ERR_NOT_CONVERGED):
             _check(st)
         out = [{"iterations": r.iterations, "converged": bool(r.converged), "status": r.status,
                 "elapsed_ms": r.elapsed_ms, "final_scaled": r.final_scaled, "sweeps_total": r.sweeps_total}

@@ -49,6 +49,15 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "smoother", _abi.SMOOTHER_LEX), "smoother"),
     (lambda c: setattr(c, "xmax", c.xmin), "domain"),
     (lambda c: setattr(c, "tile", 9), "tile"),
+    (lambda c: setattr(c, "p_solver", 5), "p_solver"),
+    (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "p_refine", 1)), "p_refine"),
+    (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "dim", 3), setattr(c, "nz", 8)), "2D"),
+    (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "nx", 4), setattr(c, "ny", 60000)), "elongated"),
+    (lambda c: setattr(c, "mg_over", 2.5), "mg_over"),
+    (lambda c: setattr(c, "mg_sweeps", 4), "mg_sweeps"),
+    (lambda c: setattr(c, "mg_cycles", 9), "mg_cycles"),
+    (lambda c: setattr(c, "mg_coarse_sweeps", -1), "mg_coarse_sweeps"),
+    (lambda c: setattr(c, "fail_if_not_converged", 2), "fail_if_not_converged"),
 ])
 def test_invalid_config_rejected_before_any_device_work(mutate, msg):
     c = _abi.default_config()
@@ -79,6 +88,7 @@ def test_null_arguments():
     (lambda c: setattr(c, "sweep_block", 2), "must agree"),
     (lambda c: setattr(c, "tile", 3), "must agree"),
     (lambda c: setattr(c, "nx", 2), "case 1: nx and ny"),
+    (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "p_refine", 0)), "p' sweeps"),
 ])
 def test_batch_rejects_mismatched_cases_before_any_device_work(mutate, msg):
     cs = (_abi.Config * 2)(_abi.cavity_config(n=32), _abi.cavity_config(n=48))
