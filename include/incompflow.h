@@ -171,7 +171,8 @@ typedef struct ifl_config {
     int32_t p_solver;            /* IFL_PSOLVE_SWEEPS (default) or IFL_PSOLVE_MG */
     int32_t mg_cycles;           /* V-cycles per p' solve, 1..8 (0 = 1) */
     int32_t mg_sweeps;           /* red-black sweeps before and after each coarse correction, 1..3 (0 = 2) */
-    int32_t mg_coarse_sweeps;    /* red-black sweeps on the coarsest level, 1..200 (0 = 20) */
+    int32_t mg_coarse_sweeps;    /* red-black sweeps on the coarsest level, 1..200 (0 = 4:
+                                    it has at most a few cells across) */
     double mg_over;              /* coarse-correction factor, (0, 2] (0 = 1.0) */
     int32_t fail_if_not_converged; /* 1: ifl_solve and ifl_batch_solve return IFL_ERR_NOT_CONVERGED when
                                       the stop rule was not met within the iterations allowed */

@@ -327,7 +327,7 @@ int mgEnsure(ifl_solver *h) {
     ifl::MgLaunch &G = h->mg->launch;
     G.cycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
     G.nu = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
-    G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 20;
+    G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
     G.over = (float)(c.mg_over > 0.0 ? c.mg_over : 1.0);
     return IFL_OK;
 }

@@ -12,7 +12,8 @@
 // One V-cycle: down the levels (unrelaxed red-black sweeps from x = 0, FP64 residual, the
 // children's residuals summed in FP64 and rounded once into the coarse b), the coarsest level
 // smoothed `ncoarse` times, and up (x += over * x_parent, sweeps).  Levels of at most
-// kMgBottomCells cells run in one single-workgroup kernel with everything in LDS.
+// kMgBottomCells cells run in one single-workgroup kernel with everything in LDS; it also builds
+// the matrices of the levels below its first one, so the coarsening kernel stops at that level.
 #pragma once
 
 #include "ifl_device.hpp"
@@ -45,7 +46,7 @@ struct MgCoarsen {
 struct MgLaunch {
     int nl = 0;                   // levels, level 0 included
     int lb = 0;                   // first level of the bottom kernel
-    int cycles = 1, nu = 2, ncoarse = 20;
+    int cycles = 1, nu = 2, ncoarse = 4;
     float over = 1.0f;
     MgLevel lv[kMgMaxLevels] = {};
     const MgLevel *dev = nullptr; // device copy of lv (the bottom kernel reads its levels there)

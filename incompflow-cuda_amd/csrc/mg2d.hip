@@ -1,10 +1,10 @@
 // mg2d.hip - gfx950 kernels of the multigrid p' solve on 2D grids (IFL_PSOLVE_MG).
 //
-// Per outer iteration: k_mg_coarsen2d builds the coarse matrices (one pass makes six levels from
-// 64x64 tiles of its source level, each tile summing its own cells hierarchically in LDS; the
-// couplings of a tile's west and south edge cells towards -x and -y come from the same sums over
-// the neighbouring column and row).  Per V-cycle: k_mg_down2d on every level above the bottom
-// (NU red-black sweeps from x = 0 on a 64x32 tile staged with a (2NU+1)-cell halo, then the FP64
+// Per outer iteration: k_mg_coarsen2d builds the coarse matrices down to the bottom kernel's first
+// level (one pass makes up to six levels from 64x64 tiles of its source level, each tile summing
+// its own cells hierarchically in LDS; the couplings of a tile's west and south edge cells towards
+// -x and -y come from the same sums over the neighbouring column and row).  Per V-cycle: k_mg_down2d on every level above the bottom
+// (NU red-black sweeps from x = 0 on a 64x32 or 32x16 tile staged with a (2NU+1)-cell halo, then the FP64
 // residual of the tile and its restriction into the coarse b), k_mg_bottom2d for the bottom levels
 // in one workgroup, and k_mg_up2d on the way back (x += over * x_parent on the staged tile with a
 // 2NU halo, NU sweeps).  Halo cells are recomputed from the same values their owning tile uses, so
@@ -37,7 +37,7 @@ __device__ __forceinline__ float mgOutlet0(const CaseParams &P, int i, int j, in
 // ---------------- coarse matrices ----------------
 // One workgroup per 64x64 tile of the source level; levels src+1 .. src+m of the tile's footprint.
 template <bool L0>
-__global__ __launch_bounds__(256) void k_mg_coarsen2d(CaseParams P, MgCoarsen G, int sweepsPerSolve) {
+__global__ __launch_bounds__(256) void k_mg_coarsen2d(CaseParams P, MgCoarsen G) {
     constexpr int T = 64, NT = 256;
     __shared__ float cE[2][T * T], cN[2][T * T], cD[2][T * T];
     __shared__ float hW[2][T], hS[2][T];     // couplings of the west column's +x and south row's +y
@@ -48,11 +48,6 @@ __global__ __launch_bounds__(256) void k_mg_coarsen2d(CaseParams P, MgCoarsen G,
     const int t = xcdSwizzle((int)blockIdx.x, (int)gridDim.x);
     const int by = t / tilesX, bx = t - by * tilesX;
     const int x0 = bx * T, y0 = by * T;
-    if (L0 && t == 0 && tid == 0) {     // the p' solve's sweep count (fine-level sweeps)
-        P.st->sm[3].sweeps = sweepsPerSolve;
-        P.st->sm[3].lastSum = 0.0;
-        P.st->sweepsIter[3] += sweepsPerSolve;
-    }
     const float *srcE = L0 ? P.pcoef + (size_t)P.N * sE : G.s.E;
     const float *srcN = L0 ? P.pcoef + (size_t)P.N * sN : G.s.N;
     for (int i = tid; i < T * T; i += NT) {
@@ -205,9 +200,9 @@ __device__ __forceinline__ void mgHalf(float *tx, const float *tb, const float *
 
 // Down: NU sweeps (x from zero on the first cycle, or from p' on level 0 of a later cycle), the
 // tile's FP64 residual, its x, and the restriction into the coarse b.
-template <int NU, bool L0, bool XZ>
-__global__ __launch_bounds__(512) void k_mg_down2d(CaseParams P, MgLevel F, MgLevel Cc) {
-    constexpr int TX = 64, TY = 32, NT = 512, H = 2 * NU + 1;
+template <int TX, int TY, int NT, int NU, bool L0, bool XZ>
+__global__ __launch_bounds__(NT) void k_mg_down2d(CaseParams P, MgLevel F, MgLevel Cc) {
+    constexpr int H = 2 * NU + 1;
     using Tl = MgTile<TX, TY, H>;
     __shared__ float tx[2 * Tl::CO], tb[2 * Tl::CO], tc[2 * Tl::CO], te[2 * Tl::CO], tn[2 * Tl::CO];
     __shared__ double sr[TX * TY];
@@ -265,9 +260,9 @@ __global__ __launch_bounds__(512) void k_mg_down2d(CaseParams P, MgLevel F, MgLe
 }
 
 // Up: x += over * x_parent on the staged region, NU sweeps, the tile's x.
-template <int NU, bool L0>
-__global__ __launch_bounds__(512) void k_mg_up2d(CaseParams P, MgLevel F, MgLevel Cc, float over) {
-    constexpr int TX = 64, TY = 32, NT = 512, H = 2 * NU;
+template <int TX, int TY, int NT, int NU, bool L0>
+__global__ __launch_bounds__(NT) void k_mg_up2d(CaseParams P, MgLevel F, MgLevel Cc, float over) {
+    constexpr int H = 2 * NU;
     using Tl = MgTile<TX, TY, H>;
     __shared__ float tx[2 * Tl::CO], tb[2 * Tl::CO], tc[2 * Tl::CO], te[2 * Tl::CO], tn[2 * Tl::CO];
     if (caseDone(P)) return;
@@ -292,24 +287,44 @@ __global__ __launch_bounds__(512) void k_mg_up2d(CaseParams P, MgLevel F, MgLeve
 }
 
 // ---------------- the bottom levels, one workgroup ----------------
-constexpr int kMgBottomFloats = 5 * 5600;   // x, b, C, E, N of the bottom levels (row major)
+// Levels lb .. nl-1 in LDS, six row-major planes each (x, b, C, E, N, outlet term).  The kernel
+// coarsens the matrices of levels lb+1.. itself (the whole level is in LDS, so the couplings of a
+// cell towards -x and -y are its neighbours' +x and +y ones directly), then runs the rest of the
+// V-cycle.  Threads form 32 columns x NT/32 rows; thread (tx, ty) takes the cells of colour col in
+// rows ty, ty + NT/32, ... at i = ((col + j) & 1) + 2 tx + 64 m: no division by the level width.
+constexpr int kMgBottomFloats = 6 * 5600;
+constexpr int kMgPlanes = 6;
 
+// `sweeps` red-black sweeps of one level in LDS.  With Lc (the coarser level's x, cnx wide), the
+// first (red) half sweep reads its neighbours as x + over * x_parent: that is the coarse correction
+// of the black cells, which nothing else reads before the black half sweep overwrites them, and the
+// red cells' own corrected values are never read (an unrelaxed update does not use the old value),
+// so this equals correcting every cell first and sweeping after, bit for bit, without its pass.
 template <int NT>
-__device__ __forceinline__ void mgSmoothLds(float *L, int nx, int ny, int sweeps) {
-    const int n = nx * ny;
+__device__ __forceinline__ void mgSmoothLds(float *L, int nx, int ny, int sweeps, const float *Lc = nullptr,
+                                            int cnx = 0, float over = 0.0f) {
+    constexpr int TY = NT / 32;
+    const int n = nx * ny, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     float *x = L;
     const float *b = L + n, *C = L + 2 * n, *E = L + 3 * n, *Nn = L + 4 * n;
     for (int s = 0; s < sweeps; ++s)
         for (int col = 0; col < 2; ++col) {
-            for (int c = threadIdx.x; c < n; c += NT) {
-                const int j = c / nx, i = c - j * nx;
-                if (((i + j) & 1) != col) continue;
-                const float xE = i < nx - 1 ? x[c + 1] : 0.0f, xW = i > 0 ? x[c - 1] : 0.0f;
-                const float xN = j < ny - 1 ? x[c + nx] : 0.0f, xS = j > 0 ? x[c - nx] : 0.0f;
-                const float aW = i > 0 ? E[c - 1] : 0.0f, aS = j > 0 ? Nn[c - nx] : 0.0f;
-                const float v = b[c] - E[c] * xE - aW * xW - Nn[c] * xN - aS * xS;
-                x[c] = v / C[c];
-            }
+            const bool pro = Lc != nullptr && s == 0 && col == 0;
+            for (int j = ty; j < ny; j += TY)
+                for (int i = ((col + j) & 1) + 2 * tx; i < nx; i += 64) {
+                    const int c = i + nx * j;
+                    float xE = i < nx - 1 ? x[c + 1] : 0.0f, xW = i > 0 ? x[c - 1] : 0.0f;
+                    float xN = j < ny - 1 ? x[c + nx] : 0.0f, xS = j > 0 ? x[c - nx] : 0.0f;
+                    if (pro) {
+                        if (i < nx - 1) xE = xE + over * Lc[(j >> 1) * cnx + ((i + 1) >> 1)];
+                        if (i > 0) xW = xW + over * Lc[(j >> 1) * cnx + ((i - 1) >> 1)];
+                        if (j < ny - 1) xN = xN + over * Lc[((j + 1) >> 1) * cnx + (i >> 1)];
+                        if (j > 0) xS = xS + over * Lc[((j - 1) >> 1) * cnx + (i >> 1)];
+                    }
+                    const float aW = i > 0 ? E[c - 1] : 0.0f, aS = j > 0 ? Nn[c - nx] : 0.0f;
+                    const float v = b[c] - E[c] * xE - aW * xW - Nn[c] * xN - aS * xS;
+                    x[c] = v / C[c];
+                }
             __syncthreads();
         }
 }
@@ -326,86 +341,123 @@ __device__ __forceinline__ double mgResidualLds(const float *L, int nx, int ny, 
     return r;
 }
 
+// recordSweeps > 0 (the first cycle): the p' solve's sweep count (fine-level sweeps) for the record.
 template <int NU, bool L0, bool XZ, int NT>
 __global__ __launch_bounds__(NT) void k_mg_bottom2d(CaseParams P, const MgLevel *lv, int lb, int nl, int ncoarse,
-                                                    float over) {
+                                                    float over, int recordSweeps) {
+    constexpr int TY = NT / 32;
     __shared__ float lds[kMgBottomFloats];
     if (caseDone(P)) return;
-    const int tid = threadIdx.x;
-    // load: level lb (x, b, matrix), the coarser levels' matrices (x = 0)
-    int base = 0;
-    for (int l = lb; l < nl; ++l) {
-        const int nx = lv[l].nx, ny = lv[l].ny, n = nx * ny;
-        float *L = lds + base;
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+    if (recordSweeps > 0 && tid == 0) {
+        P.st->sm[3].sweeps = recordSweeps;
+        P.st->sm[3].lastSum = 0.0;
+        P.st->sweepsIter[3] += recordSweeps;
+    }
+    // level lb from global memory (level 0: from the case's p' system)
+    {
+        const int nx = lv[lb].nx, n = nx * lv[lb].ny;
+        float *L = lds;
         for (int c = tid; c < n; c += NT) {
-            float xv = 0.0f, bv = 0.0f, cv, ev, nv;
-            if (L0 && l == 0) {
+            float xv = 0.0f, bv, cv, ev, nv, dv;
+            if (L0) {
                 if (!XZ) xv = P.pc[c];
                 bv = P.bp[c];
                 ev = P.pcoef[c + (size_t)P.N * sE];
                 nv = P.pcoef[c + (size_t)P.N * sN];
-                cv = P.pcoef[c] + mgOutlet0(P, c % nx, c / nx, c);
+                dv = mgOutlet0(P, c % nx, c / nx, c);
+                cv = P.pcoef[c] + dv;
             } else {
-                if (l == lb) {
-                    if (!XZ) xv = lv[l].x[c];
-                    bv = lv[l].b[c];
-                }
-                cv = lv[l].C[c];
-                ev = lv[l].E[c];
-                nv = lv[l].N[c];
+                if (!XZ) xv = lv[lb].x[c];
+                bv = lv[lb].b[c];
+                cv = lv[lb].C[c];
+                ev = lv[lb].E[c];
+                nv = lv[lb].N[c];
+                dv = lv[lb].dg[c];
             }
             L[c] = xv;
             L[n + c] = bv;
             L[2 * n + c] = cv;
             L[3 * n + c] = ev;
             L[4 * n + c] = nv;
+            L[5 * n + c] = dv;
         }
-        base += 5 * n;
     }
     __syncthreads();
+    // the coarser levels' matrices (mgBuild's sums, in its order), x = 0
+    int base = 0;
+    for (int l = lb; l < nl - 1; ++l) {
+        const int nx = lv[l].nx, ny = lv[l].ny, n = nx * ny;
+        const int cnx = lv[l + 1].nx, cny = lv[l + 1].ny, cn = cnx * cny;
+        const float *F = lds + base;
+        float *Lc = lds + base + kMgPlanes * n;
+        for (int J = ty; J < cny; J += TY)
+            for (int I = tx; I < cnx; I += 32) {
+                const bool hasI1 = 2 * I + 1 < nx, hasJ1 = 2 * J + 1 < ny;
+                const int i0 = 2 * I, i1 = hasI1 ? i0 + 1 : i0, j0 = 2 * J, j1 = hasJ1 ? j0 + 1 : j0;
+                const float *fE = F + 3 * n, *fN = F + 4 * n, *fD = F + 5 * n;
+                float e = 0.0f, nn = 0.0f, d = 0.0f;
+                e += fE[i1 + nx * j0];
+                if (hasJ1) e += fE[i1 + nx * j1];
+                nn += fN[i0 + nx * j1];
+                if (hasI1) nn += fN[i1 + nx * j1];
+                d += fD[i0 + nx * j0];
+                if (hasI1) d += fD[i1 + nx * j0];
+                if (hasJ1) {
+                    d += fD[i0 + nx * j1];
+                    if (hasI1) d += fD[i1 + nx * j1];
+                }
+                const int C = I + cnx * J;
+                Lc[C] = 0.0f;
+                Lc[3 * cn + C] = e;
+                Lc[4 * cn + C] = nn;
+                Lc[5 * cn + C] = d;
+            }
+        __syncthreads();
+        for (int J = ty; J < cny; J += TY)
+            for (int I = tx; I < cnx; I += 32) {
+                const int C = I + cnx * J;
+                const float e = Lc[3 * cn + C], nn = Lc[4 * cn + C];
+                const float ew = I > 0 ? Lc[3 * cn + C - 1] : 0.0f, ns = J > 0 ? Lc[4 * cn + C - cnx] : 0.0f;
+                const float cc = -(e + ew + nn + ns);
+                Lc[2 * cn + C] = cc + Lc[5 * cn + C];
+            }
+        __syncthreads();
+        base += kMgPlanes * n;
+    }
     // down
     base = 0;
     for (int l = lb; l < nl - 1; ++l) {
         const int nx = lv[l].nx, ny = lv[l].ny, n = nx * ny;
         const int cnx = lv[l + 1].nx, cny = lv[l + 1].ny, cn = cnx * cny;
-        float *L = lds + base, *Lc = lds + base + 5 * n;
+        float *L = lds + base, *Lc = lds + base + kMgPlanes * n;
         mgSmoothLds<NT>(L, nx, ny, NU);
-        for (int C = tid; C < cn; C += NT) {
-            const int J = C / cnx, I = C - J * cnx;
-            const bool hasI1 = 2 * I + 1 < nx, hasJ1 = 2 * J + 1 < ny;
-            double acc = 0.0;
-            acc += mgResidualLds(L, nx, ny, 2 * I, 2 * J);
-            if (hasI1) acc += mgResidualLds(L, nx, ny, 2 * I + 1, 2 * J);
-            if (hasJ1) {
-                acc += mgResidualLds(L, nx, ny, 2 * I, 2 * J + 1);
-                if (hasI1) acc += mgResidualLds(L, nx, ny, 2 * I + 1, 2 * J + 1);
+        for (int J = ty; J < cny; J += TY)
+            for (int I = tx; I < cnx; I += 32) {
+                const bool hasI1 = 2 * I + 1 < nx, hasJ1 = 2 * J + 1 < ny;
+                double acc = 0.0;
+                acc += mgResidualLds(L, nx, ny, 2 * I, 2 * J);
+                if (hasI1) acc += mgResidualLds(L, nx, ny, 2 * I + 1, 2 * J);
+                if (hasJ1) {
+                    acc += mgResidualLds(L, nx, ny, 2 * I, 2 * J + 1);
+                    if (hasI1) acc += mgResidualLds(L, nx, ny, 2 * I + 1, 2 * J + 1);
+                }
+                Lc[cn + I + cnx * J] = (float)acc;
             }
-            Lc[cn + C] = (float)acc;
-            Lc[C] = 0.0f;
-        }
         __syncthreads();
-        base += 5 * n;
+        base += kMgPlanes * n;
     }
-    {
-        const int nx = lv[nl - 1].nx, ny = lv[nl - 1].ny;
-        mgSmoothLds<NT>(lds + base, nx, ny, ncoarse);
-    }
-    // up
+    mgSmoothLds<NT>(lds + base, lv[nl - 1].nx, lv[nl - 1].ny, ncoarse);
+    // up: the coarse correction folded into the first half sweep (mgSmoothLds)
     for (int l = nl - 2; l >= lb; --l) {
         const int nx = lv[l].nx, ny = lv[l].ny, n = nx * ny, cnx = lv[l + 1].nx;
-        float *Lc = lds + base;
-        base -= 5 * n;
-        float *L = lds + base;
-        for (int c = tid; c < n; c += NT) {
-            const int j = c / nx, i = c - j * nx;
-            L[c] = L[c] + over * Lc[(j >> 1) * cnx + (i >> 1)];
-        }
-        __syncthreads();
-        mgSmoothLds<NT>(L, nx, ny, NU);
+        const float *Lc = lds + base;
+        base -= kMgPlanes * n;
+        mgSmoothLds<NT>(lds + base, nx, ny, NU, Lc, cnx, over);
     }
     {
         const int n = lv[lb].nx * lv[lb].ny;
-        float *xg = (L0 && lb == 0) ? P.pc : lv[lb].xu;      // what the up pass reads as parents
+        float *xg = L0 ? P.pc : lv[lb].xu;      // what the up pass reads as parents
         for (int c = tid; c < n; c += NT) xg[c] = lds[c];
     }
 }
@@ -428,7 +480,7 @@ int mg_bottom_level(int nl, const int (*dims)[2]) {
     // the first level from which the remaining levels' five planes fit in the bottom kernel's LDS
     for (int l = 0; l < nl; ++l) {
         long long tot = 0;
-        for (int m = l; m < nl; ++m) tot += 5LL * dims[m][0] * dims[m][1];
+        for (int m = l; m < nl; ++m) tot += (long long)kMgPlanes * dims[m][0] * dims[m][1];
         if (tot <= kMgBottomFloats && (long long)dims[l][0] * dims[l][1] <= kMgBottomCells) return l;
     }
     return -1;   // the coarsest level alone does not fit (a very elongated grid)
@@ -436,34 +488,51 @@ int mg_bottom_level(int nl, const int (*dims)[2]) {
 
 namespace {
 
+// Tiles of a level above the bottom: 64x32 (512 threads) while that gives at least 256 tiles,
+// else 32x16 (256 threads): on the small levels the kernels are latency bound, and smaller tiles
+// spread the same work over more workgroups with less serial work in each.
+inline bool mgSmallTiles(const MgLevel &F) { return ((F.nx + 63) / 64) * ((F.ny + 31) / 32) < 256; }
+
+template <int TX, int TY, int NT, int NU>
+void launchDownT(const Lc &L, const MgLaunch &G, int l, bool xz) {
+    const MgLevel &F = G.lv[l];
+    const dim3 grid((unsigned)(((F.nx + TX - 1) / TX) * ((F.ny + TY - 1) / TY)));
+    if (l == 0) {
+        if (xz) k_mg_down2d<TX, TY, NT, NU, true, true><<<grid, NT, 0, L.s>>>(*L.P, F, G.lv[1]);
+        else k_mg_down2d<TX, TY, NT, NU, true, false><<<grid, NT, 0, L.s>>>(*L.P, F, G.lv[1]);
+    } else {
+        k_mg_down2d<TX, TY, NT, NU, false, true><<<grid, NT, 0, L.s>>>(*L.P, F, G.lv[l + 1]);
+    }
+}
+
+template <int TX, int TY, int NT, int NU>
+void launchUpT(const Lc &L, const MgLaunch &G, int l) {
+    const MgLevel &F = G.lv[l];
+    const dim3 grid((unsigned)(((F.nx + TX - 1) / TX) * ((F.ny + TY - 1) / TY)));
+    if (l == 0) k_mg_up2d<TX, TY, NT, NU, true><<<grid, NT, 0, L.s>>>(*L.P, F, G.lv[1], G.over);
+    else k_mg_up2d<TX, TY, NT, NU, false><<<grid, NT, 0, L.s>>>(*L.P, F, G.lv[l + 1], G.over);
+}
+
 template <int NU>
 void launchDown(const Lc &L, const MgLaunch &G, int l, bool xz) {
-    const MgLevel &F = G.lv[l];
-    const dim3 grid((unsigned)(((F.nx + 63) / 64) * ((F.ny + 31) / 32)));
-    if (l == 0) {
-        if (xz) k_mg_down2d<NU, true, true><<<grid, 512, 0, L.s>>>(*L.P, F, G.lv[1]);
-        else k_mg_down2d<NU, true, false><<<grid, 512, 0, L.s>>>(*L.P, F, G.lv[1]);
-    } else {
-        k_mg_down2d<NU, false, true><<<grid, 512, 0, L.s>>>(*L.P, F, G.lv[l + 1]);
-    }
+    if (mgSmallTiles(G.lv[l])) launchDownT<32, 16, 256, NU>(L, G, l, xz);
+    else launchDownT<64, 32, 512, NU>(L, G, l, xz);
 }
 
 template <int NU>
 void launchUp(const Lc &L, const MgLaunch &G, int l) {
-    const MgLevel &F = G.lv[l];
-    const dim3 grid((unsigned)(((F.nx + 63) / 64) * ((F.ny + 31) / 32)));
-    if (l == 0) k_mg_up2d<NU, true><<<grid, 512, 0, L.s>>>(*L.P, F, G.lv[1], G.over);
-    else k_mg_up2d<NU, false><<<grid, 512, 0, L.s>>>(*L.P, F, G.lv[l + 1], G.over);
+    if (mgSmallTiles(G.lv[l])) launchUpT<32, 16, 256, NU>(L, G, l);
+    else launchUpT<64, 32, 512, NU>(L, G, l);
 }
 
 template <int NU>
-void launchBottom(const Lc &L, const MgLaunch &G, bool xz) {
-    constexpr int NT = 512;
+void launchBottom(const Lc &L, const MgLaunch &G, bool xz, int recordSweeps) {
+    constexpr int NT = 1024;
     if (G.lb == 0) {
-        if (xz) k_mg_bottom2d<NU, true, true, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, 0, G.nl, G.ncoarse, G.over);
-        else k_mg_bottom2d<NU, true, false, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, 0, G.nl, G.ncoarse, G.over);
+        if (xz) k_mg_bottom2d<NU, true, true, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, 0, G.nl, G.ncoarse, G.over, recordSweeps);
+        else k_mg_bottom2d<NU, true, false, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, 0, G.nl, G.ncoarse, G.over, recordSweeps);
     } else {
-        k_mg_bottom2d<NU, false, true, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, G.lb, G.nl, G.ncoarse, G.over);
+        k_mg_bottom2d<NU, false, true, NT><<<1, NT, 0, L.s>>>(*L.P, G.dev, G.lb, G.nl, G.ncoarse, G.over, recordSweeps);
     }
 }
 
@@ -471,7 +540,7 @@ template <int NU>
 void cycles(const Lc &L, const MgLaunch &G) {
     for (int c = 0; c < G.cycles; ++c) {
         for (int l = 0; l < G.lb; ++l) launchDown<NU>(L, G, l, c == 0);
-        launchBottom<NU>(L, G, G.lb > 0 || c == 0);
+        launchBottom<NU>(L, G, G.lb > 0 || c == 0, c == 0 ? G.cycles * 2 * G.nu : 0);
         for (int l = G.lb - 1; l >= 0; --l) launchUp<NU>(L, G, l);
     }
 }
@@ -479,15 +548,15 @@ void cycles(const Lc &L, const MgLaunch &G) {
 }  // namespace
 
 void launch_mg_solve(const Lc &L, const MgLaunch &G) {
-    const int sweeps = G.cycles * 2 * G.nu;
-    for (int s = 0; s < G.nl - 1; s += kMgCoarsenDepth) {
+    // the matrices of levels 1 .. lb (the bottom kernel makes the coarser ones)
+    for (int s = 0; s < G.lb; s += kMgCoarsenDepth) {
         MgCoarsen cz{};
         cz.s = G.lv[s];
-        cz.m = std::min(kMgCoarsenDepth, G.nl - 1 - s);
+        cz.m = std::min(kMgCoarsenDepth, G.lb - s);
         for (int q = 0; q < cz.m; ++q) cz.d[q] = G.lv[s + 1 + q];
         const dim3 grid((unsigned)(((cz.s.nx + 63) / 64) * ((cz.s.ny + 63) / 64)));
-        if (s == 0) k_mg_coarsen2d<true><<<grid, 256, 0, L.s>>>(*L.P, cz, sweeps);
-        else k_mg_coarsen2d<false><<<grid, 256, 0, L.s>>>(*L.P, cz, sweeps);
+        if (s == 0) k_mg_coarsen2d<true><<<grid, 256, 0, L.s>>>(*L.P, cz);
+        else k_mg_coarsen2d<false><<<grid, 256, 0, L.s>>>(*L.P, cz);
     }
     if (G.nu == 1) cycles<1>(L, G);
     else if (G.nu == 3) cycles<3>(L, G);

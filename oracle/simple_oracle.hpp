@@ -89,7 +89,7 @@ public:
     int numOuterIter = 1;
     // p' solver (build addition): the reference's sweeps, or aggregation-multigrid V-cycles
     int pSolver = IFL_PSOLVE_SWEEPS;
-    int mgCycles = 1, mgSweeps = 2, mgCoarse = 20;
+    int mgCycles = 1, mgSweeps = 2, mgCoarse = 4;
     Real mgOver = 1;
     // temperature (config.h:19-23, 45-63; declared but never solved by the reference): the
     // momentum discretisation with mu -> k / c_p and the TempBCs rows (build addition)
@@ -159,7 +159,7 @@ public:
         if (pSolver != IFL_PSOLVE_SWEEPS && pSolver != IFL_PSOLVE_MG) return fail("bad p_solver");
         mgCycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
         mgSweeps = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
-        mgCoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 20;
+        mgCoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
         mgOver = (Real)(c.mg_over > 0 ? c.mg_over : 1.0);
         if (pSolver == IFL_PSOLVE_MG && c.p_refine > 0) return fail("p_refine needs p_solver = IFL_PSOLVE_SWEEPS");
         energy = c.energy != 0;

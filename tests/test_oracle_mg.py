@@ -176,6 +176,7 @@ def test_vcycles_converge_to_the_dense_solution(name):
     strong couplings; the outer iteration still gains, see the divergence test below.)"""
     cfg = CASES[name]()
     cfg.mg_over = 1.4
+    cfg.mg_coarse_sweeps = 20     # the 3D duct's coarsest level keeps 5x4x3 cells
     o = _p_system(cfg)
     o.mg_build()
     M = _level_matrix(o, 0, cfg.dim)
