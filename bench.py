@@ -6,25 +6,35 @@ over the whole grid, run through the library's captured hipGraph.  Prints one JS
 
 value      : smoother cell-updates per second over the timed steps (cells x sweeps executed / s,
              the whole outer iteration's time in the denominator)
-roofline   : the red-black sweep kernel, timed with HIP events on the library's stream
-cpu_baseline: the FP32 CPU oracle (OpenMP, host cores) on a bounded number of outer iterations
+roofline   : the dominant kernel, the fused red-black p' sweep (k_rb_sweep2d, 5 sweeps per launch),
+             timed with HIP events on the library's stream.  achieved = the launch's compulsory
+             bytes (the field read and written once, b' and the +x/+y coefficient planes: 20 B per
+             cell for all its sweeps) / launch time; traffic = the HBM bytes the rocprofv3 counters
+             measured for the same launch (profiles/); the per-sweep streaming byte model (32 B per
+             cell and sweep) is reported beside it, not as frac
+time_to_residual_s: a separate solve from rest to the scaled residual --target (1e-4).  A 1e-5 scaled
+             residual is below the FP32 floor of this grid (time_to_1e-5: reached false, with the
+             floor measured by continuing the multigrid solve past 1e-4)
+p_solver_mg: the same cavity with the multigrid p' solve, its strength chosen by the autotuner
+             (IFL_TUNE_PSOLVE), solved from rest to the same residual
+fp64_error : a 256x256 cavity (and the 256x64 adjoint channel) converged on the GPU in FP32 against
+             the FP64 oracle converged to 1e-10: max |x - x64| / max |x64| for u, v and p (mean-free),
+             as measured by tests/test_gpu_fp64_error.py and summarised under profiles/
+cpu_baseline: the FP32 CPU oracle (OpenMP, host cores) on a bounded number of outer iterations, with
+             the host's CPU model and core counts and a 1-thread figure
 batch      : the seeded 64-case uneven batch (2D cavities and channels, 64..512 cells a side,
-             Re 10..1000, SplitMix64 seed 20261203) solved to the same scaled residual in one batch,
-             against the same cases solved one after another (unless --no-batch).  Each case of
-             the batch equals its single solve bit for bit, so both runs execute the same iterations;
-             the sequential time is measured in full with --batch-sequential full, else it is each
-             case's measured single-solve time per iteration times its iteration count.
+             Re 10..1000, SplitMix64 seed 20261203) solved to the same scaled residual in one batch
+             with launch settings chosen by a batch-level trial, against the same cases solved one
+             after another in full (unless --no-batch).  Each case of the batch equals its single
+             solve bit for bit, so both runs execute the same iterations.
 energy     : the same cavity with heated walls and the temperature equation on: ms per outer
              iteration with and without it, and the temperature sweep launch (unless --no-energy).
 adjoint    : the 1024x256 channel with a noisy parabolic inlet solved to the scaled residual
-             --adjoint-forward-target (1e-5: FP32 stalls near 9.4e-6 on this grid), then the discrete
-             adjoint of the pressure drop until dJ/dmu settles; wall times of both, and with
-             --adjoint-fd central differences of device solves along mu and along the profile.
-time_to_residual_s: a separate solve from rest to the scaled residual --target (unless --no-converge).
-             The default target is 1e-4: in FP32 the 1024x1024 cavity's scaled residual stalls at
-             ~4.5e-5 (v-momentum; continuity at ~1.3e-5), so 1e-5 cannot be reached at this size.
-             The GPU and the FP32 oracle produce the same iterates bit for bit, so the CPU time to
-             the same residual is the oracle's time per iteration times the same iteration count.
+             --adjoint-forward-target, then the discrete adjoint of the pressure drop until dJ/dmu and
+             the profile gradient settle; wall times of both, and central differences of FP32 device
+             solves (warm-started from the converged state file) along mu and along the profile.
+The GPU and the FP32 oracle produce the same iterates bit for bit, so the CPU time to the same
+residual is the oracle's time per iteration times the same iteration count.
 """
 import argparse
 import json
@@ -66,6 +76,11 @@ def batch_leg(args, incompflow, _abi):
     cells = sum(c.nx * c.ny for c in cfgs)
     log(f"[bench] batch: {len(cfgs)} cases, {cells} cells, to scaled residual {args.batch_target:g}")
     b = incompflow.Batch(cfgs)
+    tune = b.autotune(args.batch_trial_iters)
+    log(f"[bench] batch launch settings by the batch autotuner ({tune['trials']} trials): tile {tune['tile']}, "
+        f"sweep_block {tune['sweep_block']}, sweep_block_mom {tune['sweep_block_mom']}, {tune['us_per_iter']:.1f} us/iteration")
+    for c in cfgs:
+        c.tile, c.sweep_block, c.sweep_block_mom = tune["tile"], tune["sweep_block"], tune["sweep_block_mom"]
     device_sync()
     t0 = time.perf_counter()
     st, reps = b.solve()
@@ -88,7 +103,7 @@ def batch_leg(args, incompflow, _abi):
                 raise RuntimeError(f"case {i}: single solve ran {r['iterations']} iterations, batch {its[i]}")
             if i % 8 == 7:
                 log(f"[bench]   sequential: {i + 1} cases, {seq:.1f} s")
-        kind = "measured: every case solved alone to the same residual"
+        kind = "measured: every case solved alone, with the same settings, to the same residual"
     else:
         for i, c in enumerate(cfgs):
             c2 = c.copy()
@@ -104,10 +119,14 @@ def batch_leg(args, incompflow, _abi):
         kind = (f"estimated: each case's single-solve wall time per iteration "
                 f"({args.batch_probe_iters} iterations) x its iteration count")
     log(f"[bench] sequential {seq:.2f} s ({kind.split(':')[0]}), speedup {seq / tb:.2f}x")
+    stalled = [{"case": i, "nx": cfgs[i].nx, "ny": cfgs[i].ny, "re": round(1.0 / cfgs[i].viscosity, 1),
+                "kind": "cavity" if cfgs[i].bc_type[_abi.WEST] == _abi.WALL else "channel",
+                "final_scaled": reps[i]["final_scaled"]} for i in range(len(cfgs)) if not reps[i]["converged"]]
     return {"cases": len(cfgs), "cells": cells, "residual_target": args.batch_target,
-            "max_iter": args.batch_max_iter, "converged": conv, "iterations_max": max(its),
-            "iterations_sum": sum(its), "batch_s": tb, "sequential_s": seq, "sequential_kind": kind,
-            "speedup": seq / tb, "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
+            "max_iter": args.batch_max_iter, "converged": conv, "not_converged": stalled,
+            "iterations_max": max(its), "iterations_sum": sum(its), "batch_s": tb, "sequential_s": seq,
+            "sequential_kind": kind, "speedup": seq / tb, "launch_settings": tune,
+            "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
 
 
 # algorithmic bytes per cell of one launch (FP32 words x 4): the compulsory reads and writes
@@ -167,6 +186,12 @@ def cavity3d_leg(args, incompflow, _abi):
 
 
 def adjoint_leg(args, incompflow, _abi):
+    """The 1024x256 channel (C5): forward solve from rest, the discrete adjoint of the pressure drop
+    (stopped when dJ/dmu and the profile gradient change by less than 1e-6 over 500 iterations), and
+    central differences of FP32 device solves warm-started from the converged state file, along mu
+    and along the profile itself."""
+    import tempfile
+
     import numpy as np
     c = _abi.adjoint_channel_config(1024, 256)
     c.stop_rule = _abi.STOP_SCALED
@@ -183,6 +208,9 @@ def adjoint_leg(args, incompflow, _abi):
     device_sync()
     tf = time.perf_counter() - t0
     log(f"[bench]   forward {rf['iterations']} iterations, {tf:.2f} s, converged {rf['converged']}")
+    tmp = tempfile.mkdtemp(prefix="ifl_bench_")
+    base_state = os.path.join(tmp, "base.state")
+    s.save_state(base_state)
     t0 = time.perf_counter()
     ra = s.adjoint(_abi.OBJ_PRESSURE_DROP, _abi.WEST, tol=1e-7, grad_tol=1e-6, check_every=500, max_iter=200000)
     device_sync()
@@ -192,34 +220,52 @@ def adjoint_leg(args, incompflow, _abi):
     log(f"[bench]   adjoint {ra['iterations']} iterations, {ta:.2f} s, J {J:.6e}, dJ/dmu {gmu:.6e}")
     out = {"workload": "2D channel 1024x256 on [0,0.5]x[0,0.05], rho 1, mu 1e-3, parabolic inlet "
                        "(u_max 0.02) + 1e-3 SplitMix64/Box-Muller noise, J = mean p(i=0) - mean p(i=nx-1)",
-           "forward_target": c.tol_residual, "forward_iterations": rf["iterations"],
-           "forward_converged": rf["converged"], "forward_s": tf,
+           "forward_target": c.tol_residual, "forward_target_requested": 1e-6,
+           "forward_target_note": "the workload names 1e-6; the FP32 forward solve stalls above it at "
+                                  "this size, so the forward stops at 1e-5 (its error against FP64 at "
+                                  "256x64 is in fp64_error)",
+           "forward_iterations": rf["iterations"], "forward_converged": rf["converged"],
+           "forward_final_scaled": rf["final_scaled"], "forward_s": tf,
            "adjoint_iterations": ra["iterations"], "adjoint_converged": ra["converged"], "adjoint_s": ta,
+           "adjoint_grad_change": ra["grad_change"],
            "adjoint_over_forward": ta / tf, "adjoint_ms_per_iteration": ra["elapsed_ms"] / max(1, ra["iterations"]),
            "J": J, "dJ_dmu": gmu, "dJ_dprofile_norm": float(np.linalg.norm(gp)),
            # creeping-flow identities (J linear in mu and in the inflow at Re ~ 1)
            "mu_dJdmu_over_J": c.viscosity * gmu / J, "theta_dJdtheta_over_J": float(np.dot(prof, gp)) / J,
            "fd": None}
-    if args.adjoint_fd:
+    if not args.no_adjoint_fd:
+        fd_its = []
+
         def J_of(mu, p):
             c2 = c.copy()
             c2.viscosity = mu
             s2 = incompflow.Solver(c2)
             s2.set_face_profile(_abi.WEST, p)
-            s2.solve(0, history=False)
+            s2.load_state(base_state)        # warm start from the converged base state
+            r2 = s2.solve(0, history=False)
+            fd_its.append(r2["iterations"])
             pp = s2.get(_abi.FIELD_P).astype(np.float64)
             s2.close()
             return float(pp[:, :, 0].mean() - pp[:, :, c.nx - 1].mean())
-        h = 1e-2
+        h = args.adjoint_fd_step
+        t0 = time.perf_counter()
         fd_mu = (J_of(c.viscosity * (1 + h), prof) - J_of(c.viscosity * (1 - h), prof)) / (2 * h * c.viscosity)
         fd_dir = (J_of(c.viscosity, prof * (1 + h)) - J_of(c.viscosity, prof * (1 - h))) / (2 * h)
+        tfd = time.perf_counter() - t0
         adj_dir = float(np.dot(prof, gp))
         out["fd"] = {"step": h, "dJ_dmu_fd": fd_mu, "dJ_dmu_rel_err": abs(gmu - fd_mu) / abs(fd_mu),
                      "dir_profile_fd": fd_dir, "dir_profile_adjoint": adj_dir,
                      "dir_profile_rel_err": abs(adj_dir - fd_dir) / abs(fd_dir),
-                     "kind": "central differences of FP32 device solves to the same target"}
+                     "solves": 4, "solve_iterations": fd_its, "wall_s": tfd,
+                     "kind": "central differences of FP32 device solves to the forward target, warm-started "
+                             "from the converged state file"}
         log(f"[bench]   FD check: dJ/dmu rel err {out['fd']['dJ_dmu_rel_err']:.2e}, "
-            f"profile direction rel err {out['fd']['dir_profile_rel_err']:.2e}")
+            f"profile direction rel err {out['fd']['dir_profile_rel_err']:.2e} ({tfd:.1f} s)")
+    try:
+        os.remove(base_state)
+        os.rmdir(tmp)
+    except OSError:
+        pass
     return out
 
 
@@ -274,6 +320,101 @@ def copy_probe():
     log(f"[bench] copied 5 x {n * 4} bytes")
 
 
+def host_cpu():
+    """CPU model, sockets, physical cores and logical CPUs of this host (from /proc/cpuinfo)."""
+    model, phys, logical = None, set(), 0
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if not line.strip():
+                    if "processor" in cur:
+                        logical += 1
+                        phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name" and model is None:
+                    model = v.strip()
+        if "processor" in cur:
+            logical += 1
+            phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+    except OSError:
+        pass
+    sockets = len({p for p, _ in phys}) or None
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = None
+    return {"cpu_model": model, "sockets": sockets, "physical_cores": len(phys) or None, "logical_cpus": logical or None,
+            "cpus_allowed": allowed}
+
+
+def solve_tracking(solver, targets, chunk, budget_s, extra_after):
+    """Run a solver (whose own stop rule never triggers) in chunks with its history; return the
+    iteration and device time at which each target scaled residual was first reached, the minimum
+    reached, the iterations run and their device time.  After the first target is reached the solve
+    goes on for `extra_after` iterations (to observe the floor), or until the wall budget ends."""
+    hit = {t: None for t in targets}
+    done, best, ms, t0 = 0, float("inf"), 0.0, time.perf_counter()
+    stop_at = None
+    while stop_at is None or done < stop_at:
+        r = solver.solve(chunk if stop_at is None else min(chunk, stop_at - done), history=True)
+        h = r["history"]
+        per = r["elapsed_ms"] / max(1, len(h))
+        for k, rec in enumerate(h):
+            for t in targets:
+                if hit[t] is None and rec.scaled < t:
+                    hit[t] = (done + k + 1, (ms + per * (k + 1)) / 1000.0)
+        best = min([best] + [x.scaled for x in h])
+        done += r["iterations"]
+        ms += r["elapsed_ms"]
+        log(f"[bench]   {done} iterations, scaled residual {h[-1].scaled:.3e} (min {best:.3e}), {ms / 1000:.1f} s")
+        if stop_at is None and hit[targets[0]] is not None:
+            stop_at = hit[targets[0]][0] + extra_after
+        if r["status"] != 0 or time.perf_counter() - t0 > budget_s:
+            break
+    return hit, best, done, ms / 1000.0
+
+
+def mg_leg(args, incompflow, _abi):
+    """The 1024^2 cavity with the multigrid p' solve: the autotuner's choice of strength
+    (IFL_TUNE_PSOLVE, from iteration 20), then a solve from rest with it, past 1e-4 to the FP32 floor."""
+    c = _abi.cavity_config(n=args.n, p_refine=0, tol_residual=1e-12, max_iter=10 ** 7)
+    c.sweep_block, c.sweep_block_mom, c.tile = args.sweep_block, args.sweep_block_mom, args.tile
+    t = incompflow.Solver(c)
+    t.solve(20, history=False)
+    choice = t.autotune(_abi.TUNE_PSOLVE, trial_iters=args.tune_iters)
+    t.close()
+    log(f"[bench] p' solver chosen by the autotuner: {choice['p_solver']} (mg_over {choice['mg_over']}, "
+        f"{choice['decades_per_s']:.2f} decades/s)")
+    c.p_solver, c.mg_cycles, c.mg_over = choice["p_solver"], max(1, choice["mg_cycles"]), choice["mg_over"]
+    if c.p_solver == _abi.PSOLVE_SWEEPS:
+        c.p_refine = 1
+    s = incompflow.Solver(c)
+    s.solve(args.warmup, history=False)
+    r = s.solve(args.steps, history=False)
+    ms = r["elapsed_ms"] / args.steps
+    us, _ = s.time_kernel(_abi.KERNEL_PSOLVE, reps=20)
+    s.close()
+    s = incompflow.Solver(c)
+    log(f"[bench] multigrid p' solve from rest past {args.target:g} (floor: {args.floor_extra} more iterations)")
+    hit, best, done, dev_s = solve_tracking(s, [args.target, 1e-5], 10000, args.converge_budget_s,
+                                            extra_after=args.floor_extra)
+    s.close()
+    h4 = hit[args.target]
+    out = {"p_solver": "multigrid" if c.p_solver == _abi.PSOLVE_MG else "sweeps", "mg_cycles": c.mg_cycles,
+           "mg_over": c.mg_over, "tuner_trials": choice["trials"], "ms_per_step": ms, "p_solve_us": us,
+           "iterations_to_residual": h4[0] if h4 else None, "time_to_residual_s": h4[1] if h4 else None,
+           "residual_target": args.target, "reached_1e-5": hit[1e-5] is not None,
+           "time_to_1e-5_s": hit[1e-5][1] if hit[1e-5] else None,
+           "fp32_floor_observed": best, "iterations_run": done}
+    log(f"[bench] multigrid: {out['iterations_to_residual']} iterations, {out['time_to_residual_s']} s to "
+        f"{args.target:g}; minimum scaled residual {best:.3e} after {done} iterations")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,21 +426,27 @@ def main():
     ap.add_argument("--converge-budget-s", type=float, default=120.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=600)
+    ap.add_argument("--cpu-iters-1thread", type=int, default=12)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--sweep-block", type=int, default=5,
                     help="p' sweeps per launch on the 2D cavity")
     ap.add_argument("--sweep-block-mom", type=int, default=5,
                     help="momentum/temperature sweeps per launch on the 2D cavity (batch and adjoint legs keep 2)")
     ap.add_argument("--sweep-block-batch", type=int, default=4,
-                    help="p' sweeps per launch of the batch and adjoint legs (small tiles: a smaller halo pays)")
+                    help="p' sweeps per launch of the adjoint leg (small tiles: a smaller halo pays)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
+    ap.add_argument("--no-mg", action="store_true")
+    ap.add_argument("--tune-iters", type=int, default=300)
+    ap.add_argument("--floor-extra", type=int, default=60000,
+                    help="multigrid iterations past the target, to observe the FP32 floor")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-target", type=float, default=1e-4)
     ap.add_argument("--batch-max-iter", type=int, default=20000)
-    ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="estimate")
+    ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="full")
     ap.add_argument("--batch-probe-iters", type=int, default=300)
+    ap.add_argument("--batch-trial-iters", type=int, default=150)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device 5 times (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--no-cavity3d", action="store_true")
@@ -309,7 +456,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
-    ap.add_argument("--adjoint-fd", action="store_true")
+    ap.add_argument("--no-adjoint-fd", action="store_true")
+    ap.add_argument("--adjoint-fd-step", type=float, default=1e-2)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -359,53 +507,55 @@ def main():
     words = (38 + 8 * sweeps / args.steps) if cfg.dim == 2 else (51 + 10 * sweeps / args.steps)
     iter_model_bytes = 4.0 * words * cells
 
-    # ---- roofline of the dominant kernel (one red-black sweep of the p' system) ----
+    # ---- roofline of the dominant kernel (one launch of the fused red-black p' sweeps) ----
     us, per = s.time_kernel(_abi.KERNEL_SWEEP, reps=args.kernel_reps)
     kb = max(1, args.sweep_block) if cfg.dim == 2 else 1   # sweeps per launch of the p' kernel
-    alg_bytes = kb * sweep_bytes_per_cell(cfg.dim) * cells / per   # one launch updates 1/per of the cells
+    # compulsory bytes of one launch: p' in/out, b' and the coefficient planes towards +x, +y (+z)
+    # (those towards -x, -y, -z are the neighbours'), once for all kb sweeps of the launch
+    alg_bytes = (20.0 * cells if cfg.dim == 2 else 24.0 * cells) / per
     achieved = alg_bytes / (us * 1e-6) / 1e9
-    # the kernel's own compulsory traffic per launch: p' in/out, b' and the coefficient planes
-    # towards +x, +y (+z) (those towards -x, -y, -z are the neighbours'), once for all kb sweeps
-    own_bytes = (20.0 * cells if cfg.dim == 2 else 24.0 * cells) / per
-    own_gbs = own_bytes / (us * 1e-6) / 1e9
+    # the per-sweep streaming model (a stored 5/7-point sweep reads its matrix every sweep)
+    model_bytes = kb * sweep_bytes_per_cell(cfg.dim) * cells / per
     # HBM traffic of the same kernel on the same workload, from the committed PMC summary
     # (FETCH_SIZE and WRITE_SIZE runs, scaled by a known-size copy): null on other workloads
     traffic = None
     traffic_source = None
+    iteration_traffic = None
     tile = (args.tile or _abi.default_tile(cfg.nx, cfg.ny)) if cfg.dim == 2 else 0
     tname = {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16"}.get(tile)
     tfile = os.path.join(ROOT, "profiles", f"k_rb_sweep2d_pc_{tname}_cavity1024_k{kb}_traffic.json")
-    if cfg.dim == 2 and n == 1024 and tname and os.path.exists(tfile):
+    if cfg.dim == 3:
+        tfile = os.path.join(ROOT, "profiles", "k_rb_zmarch3d_pc_64x8x32_cavity256_traffic.json")
+    if ((cfg.dim == 2 and n == 1024 and tname) or (cfg.dim == 3 and n == 256 and args.p_refine == 0)) \
+            and os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh)["traffic_bytes_per_launch"]
+            tj = json.load(fh)
+        traffic = tj["traffic_bytes_per_launch"]
+        iteration_traffic = tj.get("iteration_traffic_bytes")
         traffic_source = os.path.relpath(tfile, ROOT)
     other = {}
     for name, kid in (("mom_assemble", _abi.KERNEL_MOM_ASSEMBLE), ("rhie_chow", _abi.KERNEL_RHIE_CHOW),
                       ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE),
-                      ("momentum_sweep", _abi.KERNEL_MOM_SWEEP)):
+                      ("momentum_sweep", _abi.KERNEL_MOM_SWEEP), ("p_solve", _abi.KERNEL_PSOLVE)):
         other[name] = round(s.time_kernel(kid, reps=max(10, args.kernel_reps // 4))[0], 3)
     s.close()
-    log(f"[bench] sweep launch {us:.3f} us -> {achieved:.1f} GB/s algorithmic")
+    log(f"[bench] sweep launch {us:.3f} us -> {achieved:.1f} GB/s of its compulsory bytes")
 
-    # ---- time to residual ----
+    # ---- time to residual (the precision form: sweeps + one refinement cycle) ----
     ttr = None
     iters_to_res = None
     converged = None
     if not args.no_converge:
         c = incompflow.Solver(cfg)
         log(f"[bench] solving from rest to scaled residual {cfg.tol_residual:g} (budget {args.converge_budget_s:.0f} s)")
-        ttr_sweeps = 0.0
         device_sync()
         t0 = time.perf_counter()
         total = 0
         converged = False
-        last = None
         while True:
             rr = c.solve(5000, history=False)
             total += rr["iterations"]
-            ttr_sweeps += rr["sweeps_total"]
-            last = rr["final_scaled"]
-            log(f"[bench]   {total} iterations, scaled residual {last:.3e}, {time.perf_counter() - t0:.1f} s")
+            log(f"[bench]   {total} iterations, scaled residual {rr['final_scaled']:.3e}, {time.perf_counter() - t0:.1f} s")
             if rr["converged"] or rr["status"] != 0:
                 converged = rr["converged"]
                 break
@@ -415,6 +565,19 @@ def main():
         ttr = time.perf_counter() - t0
         iters_to_res = total
         c.close()
+
+    mg = None
+    if not args.no_mg and cfg.dim == 2:
+        mg = mg_leg(args, incompflow, _abi)
+
+    # error against the FP64 oracle: measured by tests/test_gpu_fp64_error.py (the oracle is the
+    # checker there), summarised under profiles/
+    fp64 = None
+    ffile = os.path.join(ROOT, "profiles", "fp64_error_cavity256_channel256x64.json")
+    if cfg.dim == 2 and os.path.exists(ffile):
+        with open(ffile) as fh:
+            fp64 = json.load(fh)
+        fp64["source"] = os.path.relpath(ffile, ROOT)
 
     # ---- CPU baseline: FP32 oracle on the host cores, bounded sample ----
     cpu = None
@@ -431,11 +594,22 @@ def main():
         cpu = {"value": cells * csweeps / tc, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                "sample": f"outer iterations 2..{it_cpu + 1} of the {n}x{n} cavity (FP32 oracle, OpenMP, "
                          f"same settings), {1000 * tc / it_cpu:.2f} ms/iteration",
-               "ms_per_step": 1000 * tc / it_cpu}
+               "ms_per_step": 1000 * tc / it_cpu, "threads": threads}
+        cpu.update(host_cpu())
         if iters_to_res:
             cpu["time_to_residual_s_extrapolated"] = tc / it_cpu * iters_to_res
+        if args.cpu_iters_1thread > 0:
+            orc.set_threads(1)
+            t0 = time.perf_counter()
+            r1 = o.solve(args.cpu_iters_1thread, history=True)
+            t1c = time.perf_counter() - t0
+            orc.set_threads(threads)
+            s1 = sum(sum(rec.sweeps) for rec in r1["history"])
+            cpu["one_thread"] = {"ms_per_step": 1000 * t1c / args.cpu_iters_1thread, "value": cells * s1 / t1c,
+                                 "sample": f"{args.cpu_iters_1thread} further outer iterations on 1 thread"}
         o.close()
-        log(f"[bench] cpu oracle {1000 * tc / it_cpu:.1f} ms/iteration on {threads} threads")
+        log(f"[bench] cpu oracle {1000 * tc / it_cpu:.1f} ms/iteration on {threads} threads"
+            + (f", {cpu['one_thread']['ms_per_step']:.0f} ms on 1 thread" if "one_thread" in cpu else ""))
 
     batch = None
     if not args.no_batch and cfg.dim == 2:
@@ -450,6 +624,7 @@ def main():
     if not args.no_energy and cfg.dim == 2:
         energy = energy_leg(args, incompflow, _abi, ms_per_step)
 
+    traffic_gbs = traffic / (us * 1e-6) / 1e9 if traffic else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -473,20 +648,33 @@ def main():
                             _abi.TILE_32x8: "32x8", _abi.TILE_32x4: "32x4"}.get(tile, "3d")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
-                     "kernel": ("k_rb_sweep2d<COEF_PCORR> (fused red-black sweeps of p')" if cfg.dim == 2 else
-                                "k_rb_zmarch3d (one fused red-black sweep of p')"), "avg_launch_us": us,
-                     "alg_bytes_per_launch": alg_bytes, "own_bytes_per_launch": own_bytes,
-                     "own_gbs": own_gbs,
-                     "note": ("algorithmic bytes count 32 B (2D) / 40 B (3D) per cell and sweep, the traffic of a "
-                              "sweep that streams its stored matrix; one launch runs %d fused sweeps on one staged "
-                              "copy of the field, source and coefficients, so frac can exceed 1; 'traffic' and "
-                              "'own_gbs' are the bytes the launch really moves" % kb),
-                     "iteration_model_frac": iter_model_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)},
+                     "traffic_gbs": traffic_gbs,
+                     "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None,
+                     "kernel": ("k_rb_sweep2d<COEF_PCORR> (%d fused red-black sweeps of p' per launch)" % kb
+                                if cfg.dim == 2 else "k_rb_zmarch3d (one fused red-black sweep of p')"),
+                     "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes,
+                     "sweep_model_bytes_per_launch": model_bytes,
+                     "sweep_model_frac": model_bytes / (us * 1e-6) / (HBM_PEAK_GBS * 1e9),
+                     "note": ("achieved/frac: the launch's compulsory bytes (field in and out, b' and the +x/+y "
+                              "coefficient planes, once for its %d sweeps) over its time; traffic: the bytes the "
+                              "HBM counters measured for it; sweep_model: the per-sweep streaming model (32 B per "
+                              "cell and sweep), which a launch that fuses sweeps on one staged copy can exceed" % kb),
+                     "iteration_model_frac": iter_model_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                     "iteration_traffic_bytes": iteration_traffic,
+                     "iteration_traffic_frac": (iteration_traffic / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                                                if iteration_traffic else None)},
         "kernel_us": other,
         "time_to_residual_s": ttr,
         "residual_target": cfg.tol_residual,
         "iterations_to_residual": iters_to_res,
         "converged": converged,
+        "time_to_1e-5": {"reached": bool(mg and mg["reached_1e-5"]),
+                         "time_s": mg["time_to_1e-5_s"] if mg else None,
+                         "fp32_floor_observed": mg["fp32_floor_observed"] if mg else None,
+                         "how": "the multigrid solve continued past 1e-4 (its minimum scaled residual); the "
+                                "precision form's sweeps reach no lower (their floor is higher)"},
+        "p_solver_mg": mg,
+        "fp64_error": fp64,
         "gpu_vs_cpu_per_step": None,
         "cpu_baseline": cpu,
         "batch": batch,

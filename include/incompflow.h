@@ -354,10 +354,11 @@ typedef struct ifl_adjoint_opts {
     int32_t objective;             /* IFL_OBJ_* */
     int32_t max_iter;              /* adjoint iterations at most (0: 200000) */
     double tol;                    /* stop when the scaled adjoint increment is below tol (0: 1e-7) */
-    double grad_tol;               /* or when dJ/dmu changed by less than grad_tol (relative) over
-                                      the last check_every iterations (0: off).  In FP32 the
-                                      increment stalls near 1e-4 while the gradient has long
-                                      settled; this is the practical stop rule there. */
+    double grad_tol;               /* or when dJ/dmu and (if dJ_dprofile is asked for) the profile
+                                      gradient changed by less than grad_tol (relative; the profile
+                                      gradient in the L2 norm) over the last check_every iterations
+                                      (0: off).  In FP32 the increment stalls near 1e-4 while the
+                                      gradient has long settled; this is the practical stop rule. */
     const float *weights[4];       /* IFL_OBJ_LINEAR: weights of u, v, w, p (cells each; NULL = 0) */
     int32_t warm_start;            /* 1: continue from the previous call's lambda */
     int32_t check_every;           /* iterations between host reads of the monitor (0: 64) */
@@ -373,6 +374,8 @@ typedef struct ifl_adjoint_report {
     double J;                      /* objective at the current state */
     double final_scaled;           /* last scaled increment */
     double elapsed_ms;
+    double grad_change;            /* grad_tol rule: the larger relative change of dJ/dmu and of the
+                                      profile gradient over the last check (-1: not watched) */
 } ifl_adjoint_report;
 
 void ifl_adjoint_opts_default(ifl_adjoint_opts *o);
@@ -404,6 +407,13 @@ int ifl_batch_case(ifl_batch *b, int i, ifl_solver **out);
  * num_outer_iter) iterations ran.  reports (optional, n entries) receive each case's report;
  * elapsed_ms is the batch's wall time.  Returns IFL_ERR_DIVERGED if any case diverged. */
 int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports);
+/* Autotuner of a batch (no reference counterpart): the tile, sweep_block and sweep_block_mom that
+ * every case of the batch shares, by the device time of trial_iters (<= 0: 30) outer iterations of
+ * the whole batch per candidate (2D red-black; tiles 64x16, 64x8, 32x8, 32x4 x sweep_block 2, 4, 5
+ * x sweep_block_mom 0 or 4).  Every case is restored after each trial, so the batch solves on from
+ * where it was, with the chosen settings; each case still equals its own ifl_solve with the same
+ * settings, bit for bit.  On an error return the batch keeps its previous settings. */
+int ifl_batch_autotune(ifl_batch *b, int trial_iters, ifl_tune_result *out);
 
 #ifdef __cplusplus
 }

@@ -181,7 +181,9 @@ extern "C" int ifl_adjoint(ifl_solver *h, const ifl_adjoint_opts *o, int face, d
     ifl::AdjState as{};
     long long done = 0;
     bool conv = false;
-    double prevMub = 0.0;
+    double prevMub = 0.0, change = -1.0;
+    // the grad_tol rule watches dJ/dmu and, when the call asks for it, the profile gradient
+    std::vector<double> gNow(dJ_dprofile ? n_profile : 0), gPrev(gNow.size(), 0.0);
     while (done < maxIter) {
         const int n = (int)std::min<long long>(chunk, maxIter - done);
         for (int q = 0; q < n; ++q) IFL_HIP(hipGraphLaunch(A.graph, h->stream));
@@ -193,9 +195,28 @@ extern "C" int ifl_adjoint(ifl_solver *h, const ifl_adjoint_opts *o, int face, d
             conv = true;
             break;
         }
-        if (o->grad_tol > 0 && done > n && std::fabs(as.mub - prevMub) <= o->grad_tol * std::fabs(as.mub)) {
-            conv = true;
-            break;
+        if (o->grad_tol > 0) {
+            double dg = std::fabs(as.mub - prevMub), ng = std::fabs(as.mub);
+            bool settled = dg <= o->grad_tol * ng;
+            if (!gNow.empty()) {
+                IFL_HIP(hipMemcpyAsync(gNow.data(), B.gprof[face], n_profile * 8, hipMemcpyDeviceToHost, h->stream));
+                IFL_HIP(hipStreamSynchronize(h->stream));
+                double d2 = 0.0, g2 = 0.0;
+                for (size_t q = 0; q < n_profile; ++q) {
+                    d2 += (gNow[q] - gPrev[q]) * (gNow[q] - gPrev[q]);
+                    g2 += gNow[q] * gNow[q];
+                }
+                settled = settled && std::sqrt(d2) <= o->grad_tol * std::sqrt(g2);
+                dg = std::max(dg / (ng > 0 ? ng : 1.0), std::sqrt(d2) / (g2 > 0 ? std::sqrt(g2) : 1.0));
+                gPrev.swap(gNow);
+            } else {
+                dg = dg / (ng > 0 ? ng : 1.0);
+            }
+            change = dg;
+            if (done > n && settled) {
+                conv = true;
+                break;
+            }
         }
         prevMub = as.mub;
     }
@@ -226,6 +247,7 @@ extern "C" int ifl_adjoint(ifl_solver *h, const ifl_adjoint_opts *o, int face, d
         report->J = J;
         report->final_scaled = as.scaled;
         report->elapsed_ms = ms;
+        report->grad_change = change;
     }
     return IFL_OK;
 }

@@ -298,4 +298,115 @@ int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports) {
     return IFL_OK;
 }
 
+
+// The batch's launch settings by measurement: every (tile, sweep_block, sweep_block_mom) of the
+// candidates runs trial_iters outer iterations of the whole batch from its current state (after 3
+// warm-up iterations), every case is restored afterwards, and the fastest per iteration is kept.
+// The fixed point of each case does not depend on these settings; n_iter is left as configured.
+int ifl_batch_autotune(ifl_batch *b, int trial_iters, ifl_tune_result *out) {
+    if (!b || !out) return fail(IFL_ERR_ARG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    const int n = (int)b->cases.size();
+    const int trial = trial_iters > 0 ? trial_iters : 30;
+    out->tile = b->tile;
+    out->sweep_block = std::max(1, b->cfg.sweep_block);
+    out->sweep_block_mom = b->cfg.sweep_block_mom;
+    for (int q = 0; q < 4; ++q) out->n_iter[q] = b->cfg.n_iter[q];
+    out->p_solver = IFL_PSOLVE_SWEEPS;
+    if (b->cfg.dim != 2 || b->cfg.smoother != IFL_SMOOTHER_RBGS) return IFL_OK;   // nothing to choose
+    int st;
+    for (ifl_solver *h : b->cases) IFL_HIP(hipStreamSynchronize(h->stream));
+    std::vector<Snapshot> snaps(n);
+    for (int i = 0; i < n; ++i)
+        if ((st = snapTake(b->cases[i], snaps[i])) != IFL_OK) return st;
+    struct Restore {
+        ifl_batch *b;
+        std::vector<ifl_config> c0;
+        std::vector<int> t0;
+        ifl_config bc0;
+        int bt0;
+        bool armed = true;
+        ~Restore() {
+            if (!armed) return;
+            for (size_t i = 0; i < c0.size(); ++i) {
+                b->cases[i]->cfg = c0[i];
+                b->cases[i]->tile = t0[i];
+                b->cases[i]->graphDirty = true;
+            }
+            b->cfg = bc0;
+            b->tile = bt0;
+        }
+    } guard{b, {}, {}, b->cfg, b->tile};
+    for (ifl_solver *h : b->cases) {
+        guard.c0.push_back(h->cfg);
+        guard.t0.push_back(h->tile);
+    }
+    auto apply = [&](int tile, int K, int Km) {
+        for (ifl_solver *h : b->cases) {
+            h->cfg.tile = tile;
+            h->tile = tile;
+            h->cfg.sweep_block = K;
+            h->cfg.sweep_block_mom = Km;
+            h->graphDirty = true;
+        }
+        b->cfg.tile = tile;
+        b->cfg.sweep_block = K;
+        b->cfg.sweep_block_mom = Km;
+        b->tile = tile;
+    };
+    std::vector<int> all(n);
+    for (int i = 0; i < n; ++i) all[i] = i;
+    auto timeOne = [&](double &us) -> int {
+        int r;
+        for (ifl_solver *h : b->cases)
+            if ((r = armCase(h, b->stream, 0)) != IFL_OK) return r;
+        if ((r = batchBuild(b, all)) != IFL_OK) return r;
+        for (ifl_solver *h : b->cases)
+            if (h->iter == 0) {
+                Plan pl = planOf(h);
+                pl.L.s = b->stream;
+                ifl::launch_init_faces(pl.L);
+            }
+        for (int q = 0; q < 3; ++q) IFL_HIP(hipGraphLaunch(b->graph, b->stream));
+        IFL_HIP(hipEventRecord(b->ev0, b->stream));
+        for (int q = 0; q < trial; ++q) IFL_HIP(hipGraphLaunch(b->graph, b->stream));
+        IFL_HIP(hipEventRecord(b->ev1, b->stream));
+        IFL_HIP(hipEventSynchronize(b->ev1));
+        float ms = 0;
+        IFL_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        us = 1000.0 * ms / trial;
+        for (int i = 0; i < n; ++i)
+            if ((r = snapRestore(b->cases[i], snaps[i])) != IFL_OK) return r;
+        out->trials++;
+        return IFL_OK;
+    };
+    double bestUs = 1e300;
+    int bt = b->tile, bK = out->sweep_block, bKm = out->sweep_block_mom;
+    for (int tile : {ifl::TILE_64x16, ifl::TILE_64x8, ifl::TILE_32x8, ifl::TILE_32x4})
+        for (int K : {2, 4, 5})
+            for (int Km : {0, 4}) {
+                if (Km > K) continue;
+                apply(tile, K, Km);
+                double us;
+                if ((st = timeOne(us)) != IFL_OK) return st;
+                if (us < bestUs) {
+                    bestUs = us;
+                    bt = tile;
+                    bK = K;
+                    bKm = Km;
+                }
+            }
+    apply(bt, bK, bKm);
+    guard.armed = false;
+    if (b->graph) {
+        IFL_HIP(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    out->tile = bt;
+    out->sweep_block = bK;
+    out->sweep_block_mom = bKm;
+    out->us_per_iter = bestUs;
+    return IFL_OK;
+}
+
 }  // extern "C"

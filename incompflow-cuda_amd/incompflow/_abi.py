@@ -162,7 +162,7 @@ class AdjointOpts(C.Structure):
 
 class AdjointReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("J", C.c_double),
-                ("final_scaled", C.c_double), ("elapsed_ms", C.c_double)]
+                ("final_scaled", C.c_double), ("elapsed_ms", C.c_double), ("grad_change", C.c_double)]
 
 
 class TuneResult(C.Structure):

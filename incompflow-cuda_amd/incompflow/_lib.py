@@ -41,6 +41,7 @@ EXPORTS = [
     ("ifl_batch_size", C.c_int, [C.c_void_p]),
     ("ifl_batch_case", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     ("ifl_batch_solve", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.Report)]),
+    ("ifl_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.TuneResult)]),
 ]
 
 _lib = None
@@ -196,7 +197,7 @@ class Solver:
                                  nprof, C.byref(gmu), C.byref(rep)))
         return {"dJ_dprofile": gp, "dJ_dmu": gmu.value, "iterations": rep.iterations,
                 "converged": bool(rep.converged), "J": rep.J, "final_scaled": rep.final_scaled,
-                "elapsed_ms": rep.elapsed_ms}
+                "elapsed_ms": rep.elapsed_ms, "grad_change": rep.grad_change}
 
     def adjoint_lambda(self, field, shaped=True):
         n = self.cfg.field_size(field)
@@ -275,3 +276,13 @@ class Batch:
                 "elapsed_ms": r.elapsed_ms, "final_scaled": r.final_scaled, "sweeps_total": r.sweeps_total}
                for r in reps]
         return st, out
+
+    def autotune(self, trial_iters=0):
+        """Pick the tile, sweep_block and sweep_block_mom shared by every case by measurement
+        (ifl_batch_autotune); the choice is applied to the batch and mirrored into each case's cfg."""
+        r = _abi.TuneResult()
+        _check(lib().ifl_batch_autotune(self.h, trial_iters, C.byref(r)))
+        for s in self.cases:
+            s.cfg.tile, s.cfg.sweep_block, s.cfg.sweep_block_mom = r.tile, r.sweep_block, r.sweep_block_mom
+        return {"tile": r.tile, "sweep_block": r.sweep_block, "sweep_block_mom": r.sweep_block_mom,
+                "us_per_iter": r.us_per_iter, "trials": r.trials}

@@ -186,3 +186,49 @@ def test_torch_binding_on_the_device():
     c2.num_outer_iter = 50
     with pytest.raises(RuntimeError, match="did not meet its stop rule"):
         HipBackend(c2).solve(prof, c.viscosity)
+
+
+def _fp64_mg(c, mu, prof, tol=1e-10):
+    """Converged FP64 oracle solve with the multigrid p' solve (the converged state does not depend
+    on the p' solver), returning the pressure drop."""
+    c2 = c.copy()
+    c2.viscosity = mu
+    c2.p_solver, c2.p_refine, c2.mg_over = _abi.PSOLVE_MG, 0, 1.0
+    c2.stop_rule, c2.tol_residual, c2.num_outer_iter = _abi.STOP_SCALED, tol, 200000
+    o = Oracle(c2, fp64=True)
+    o.set_face_profile(_abi.WEST, prof)
+    r = o.solve(200000, history=False)
+    assert r["converged"], "FP64 solve did not converge"
+    J = pressure_drop(o, c2)
+    o.close()
+    return J
+
+
+def test_gradient_at_a_reduced_adjoint_channel_matches_fp64_differences():
+    """The adjoint channel's physics on a 128x32 grid (the 1024x256 case reduced 8x): the device's
+    FP32 adjoint gradient along mu, along the profile itself and for three single inlet faces agrees
+    with central differences (step 1e-4 relative) of FP64 oracle solves converged to 1e-10, within
+    1e-3 relative (the gradient's FP32 accuracy: the forward state stalls near the FP32 floor)."""
+    c = _abi.adjoint_channel_config(128, 32)
+    prof = np.array(_abi.adjoint_inlet_profile(c))
+    c.stop_rule = _abi.STOP_NONE
+    c.num_outer_iter = 20000
+    _log("[adjoint] 128x32 channel: FP32 forward 20000 iterations, adjoint until dJ/dmu and dJ/dprofile settle")
+    with Solver(c) as s:
+        s.set_face_profile(_abi.WEST, prof)
+        s.solve(0, history=False)
+        r = s.adjoint(_abi.OBJ_PRESSURE_DROP, _abi.WEST, tol=1e-8, grad_tol=1e-6, check_every=500, max_iter=100000)
+    assert r["converged"] and r["grad_change"] <= 1e-6
+    mu, h = c.viscosity, 1e-4
+    fd_mu = (_fp64_mg(c, mu * (1 + h), prof) - _fp64_mg(c, mu * (1 - h), prof)) / (2 * h * mu)
+    assert abs(r["dJ_dmu"] - fd_mu) <= 1e-3 * abs(fd_mu), (r["dJ_dmu"], fd_mu)
+    g = r["dJ_dprofile"]
+    fd_dir = (_fp64_mg(c, mu, prof * (1 + h)) - _fp64_mg(c, mu, prof * (1 - h))) / (2 * h)
+    assert abs(float(np.dot(prof, g)) - fd_dir) <= 1e-3 * abs(fd_dir), (float(np.dot(prof, g)), fd_dir)
+    hp = 1e-4 * 0.02
+    for j in (0, c.ny // 3, c.ny - 1):
+        pp, pm = prof.copy(), prof.copy()
+        pp[j] += hp
+        pm[j] -= hp
+        fd = (_fp64_mg(c, mu, pp) - _fp64_mg(c, mu, pm)) / (2 * hp)
+        assert abs(g[j] - fd) <= 1e-3 * np.abs(g).max(), (j, g[j], fd)

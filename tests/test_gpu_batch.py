@@ -140,3 +140,53 @@ def test_batch_inlet_profile_and_reuse():
             want = _fields(s, c)
         for f in want:
             assert_bitwise(got[i][f], want[f], f"profile case {i} field {f}")
+
+
+def _uneven(k, max_iter):
+    cs = _abi.uneven_batch_configs(tol_residual=1e-4, max_iter=max_iter)[:k]
+    for c in cs:
+        c.sweep_block = 4
+    return cs
+
+
+def test_uneven_batch_subset_equals_single_solves_and_oracle():
+    """The first eight cases of the seeded uneven batch (cavities and channels, 64..512 cells a
+    side, Re 10..1000), 200 iterations each: every case equals its single solve, and the smallest
+    one equals the oracle."""
+    cs = _uneven(8, 200)
+    _check_batch(cs, 200, "uneven subset")
+    i = min(range(len(cs)), key=lambda q: cs[q].nx * cs[q].ny)
+    with Batch(cs) as b:
+        b.solve(200)
+        got = _fields(b.cases[i], cs[i])
+    o = Oracle(cs[i])
+    o.solve(200, history=False)
+    for f in got:
+        assert_bitwise(got[f], o.get(f).ravel(), f"uneven case {i} oracle field {f}")
+
+
+def test_batch_autotune_restores_state_and_applies_its_choice():
+    """ifl_batch_autotune: the trials leave every case's fields as they were, the choice is applied
+    to every case, and the batch then continues exactly as single solves with the chosen settings."""
+    cs = _uneven(6, 60)
+    with Batch(cs) as b:
+        b.solve(20)
+        before = [_fields(b.cases[i], cs[i]) for i in range(len(cs))]
+        r = b.autotune(12)
+        assert r["trials"] == 20, r         # 4 tiles x (K 2 | K 4, 5 with Km 0, 4)
+        assert r["tile"] in (1, 2, 3, 4) and r["sweep_block"] in (2, 4, 5) and r["sweep_block_mom"] in (0, 4)
+        for i, c in enumerate(cs):
+            assert b.cases[i].cfg.tile == r["tile"] and b.cases[i].cfg.sweep_block == r["sweep_block"]
+            for f, v in _fields(b.cases[i], c).items():
+                assert_bitwise(v, before[i][f], f"restored case {i} field {f}")
+        b.solve(40)
+        got = [_fields(b.cases[i], cs[i]) for i in range(len(cs))]
+    for i, c in enumerate(cs):
+        c2 = c.copy()
+        c2.tile, c2.sweep_block, c2.sweep_block_mom = r["tile"], r["sweep_block"], r["sweep_block_mom"]
+        with Solver(c2) as s:
+            s.solve(20, history=False)
+            s.solve(40, history=False)
+            want = _fields(s, c2)
+        for f in want:
+            assert_bitwise(got[i][f], want[f], f"tuned case {i} field {f}")
