@@ -230,6 +230,28 @@ int ifl_set_face_profile(ifl_solver *h, int face, const double *vals, size_t n);
 int ifl_solve(ifl_solver *h, int max_iter, ifl_report *report,
               ifl_iter_record *history, size_t hist_cap);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (serving
+ * loops, code that a harness forbids to call hipMalloc).  One solve of cfg from the caller's device
+ * fields u, v, p (w in 3D, temp with energy = 1; each nx*ny*nz floats, read as the initial state
+ * and overwritten with the result) on `stream` (a hipStream_t of the current device; NULL: the
+ * null stream), with max_iter, report and history as in ifl_solve.  face_profiles (NULL, or 6
+ * device pointers, NULL per face for none) holds the face-normal velocity of each face's boundary
+ * cells as ifl_set_face_profile would, in float.  Every other buffer of the case (faces,
+ * matrices, partials, monitor, multigrid levels) lives in a static device array of
+ * IFL_NOALLOC_SCRATCH_BYTES held by each GPU the library runs on until the process ends; a case
+ * that needs more fails with IFL_ERR_ARG, before any device call (about 121 bytes per cell: up to
+ * some 550k cells in 2D, e.g. 740x740, and 530k in 3D, e.g. 80^3).  No device
+ * allocation, no graph: the kernels of ifl_solve are launched directly, and the fields equal those
+ * of ifl_create + ifl_set_field + ifl_solve from the same fields, bit for bit.  Calls on one GPU
+ * share the array: within a process they are serialised by a lock (each call drains its stream
+ * before returning); overlapping calls from other processes have their own copies.  The call
+ * starts at iteration 0, as the first ifl_solve of a new handle does (the face velocities are
+ * built from the cell fields), so it does not continue an earlier call. */
+#define IFL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ifl_solve_noalloc(const ifl_config *cfg, void *stream, float *u, float *v, float *w, float *p, float *temp,
+                      const float *const *face_profiles, int max_iter, ifl_report *report,
+                      ifl_iter_record *history, size_t hist_cap);
+
 /* Host copies of device state.  n must equal the field's element count. */
 int ifl_get_field(ifl_solver *h, int field, float *dst, size_t n);
 int ifl_set_field(ifl_solver *h, int field, const float *src, size_t n);

@@ -274,6 +274,67 @@ void enqueueOuter(const Plan &pl) {
     ifl::launch_reduce_cells(pl.L, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH);  // :200-268
 }
 
+size_t arenaLayout(ifl_solver *h, char *base, const bool *external) {
+    ifl::CaseParams &P = h->P;
+    const size_t N = (size_t)P.N, nx = P.nx, ny = P.ny, nz = P.nz;
+    const size_t ncoef = N * (1 + 2 * P.dim);
+    const size_t pstride = (size_t)partStride(P), ncb = (size_t)ifl::cell_blocks(P);
+    P.partStride = (int)pstride;
+    struct Part { void **dst; size_t bytes; };
+    const size_t nT = h->cfg.energy ? N : 0;     // temperature matrix, source and partner: energy only
+    auto own = [&](int q) { return external && external[q] ? size_t(0) : N * 4; };
+    const Part parts[] = {
+        {(void **)&P.u, own(0)}, {(void **)&P.v, own(1)}, {(void **)&P.w, own(2)}, {(void **)&P.p, own(3)},
+        {(void **)&P.temp, own(4)}, {(void **)&P.pc, N * 4},
+        {(void **)&P.uf, (nx + 1) * ny * nz * 4}, {(void **)&P.vf, nx * (ny + 1) * nz * 4},
+        {(void **)&P.wf, nx * ny * (nz + 1) * 4},
+        {(void **)&P.mcoef, ncoef * 4}, {(void **)&P.pcoef, ncoef * 4},
+        {(void **)&P.bu, N * 4}, {(void **)&P.bv, N * 4}, {(void **)&P.bw, N * 4}, {(void **)&P.bp, N * 4},
+        {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
+        {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
+        {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
+        {(void **)&P.tcoef, nT * (1 + 2 * P.dim) * 4}, {(void **)&P.bt, nT * 4}, {(void **)&P.alt[5], nT * 4},
+        {(void **)&P.spart, (size_t)ifl::kSlots * 2 * ifl::kMaxBlock * pstride * 8},
+        {(void **)&P.partMom, 4 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
+        {(void **)&P.st, sizeof(ifl::CaseState)},
+        {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
+    };
+    size_t off = 0;
+    for (const Part &p : parts) {
+        if (base) *p.dst = p.bytes ? base + off : nullptr;
+        off += align256(p.bytes);
+    }
+    return off;
+}
+
+size_t mgLayout(const ifl_config &c, ifl::MgLaunch &G, char *base) {
+    int dims[ifl::kMgMaxLevels][2];
+    G.nl = ifl::mg_levels(c.nx, c.ny, dims);
+    G.lb = ifl::mg_bottom_level(G.nl, dims);
+    size_t off = align256(sizeof(ifl::MgLevel) * ifl::kMgMaxLevels);
+    for (int l = 0; l < G.nl; ++l) {
+        ifl::MgLevel &L = G.lv[l];
+        L.nx = dims[l][0];
+        L.ny = dims[l][1];
+        if (l == 0) continue;
+        float **pl[7] = {&L.x, &L.xu, &L.b, &L.C, &L.E, &L.N, &L.dg};
+        const size_t b = align256((size_t)L.nx * L.ny * 4);
+        for (float **q : pl) {
+            *q = base ? (float *)(base + off) : nullptr;
+            off += b;
+        }
+    }
+    G.dev = (const ifl::MgLevel *)base;
+    return off;
+}
+
+void mgSettings(const ifl_config &c, ifl::MgLaunch &G) {
+    G.cycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
+    G.nu = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
+    G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
+    G.over = (float)(c.mg_over > 0.0 ? c.mg_over : 1.0);
+}
+
 Plan planOf(ifl_solver *h) {
     Plan pl;
     pl.L.P = &h->P;
@@ -292,43 +353,21 @@ int mgEnsure(ifl_solver *h) {
     if (!h->mg) {
         MgHost *m = new (std::nothrow) MgHost();
         if (!m) return fail(IFL_ERR_NOMEM, "out of host memory");
-        ifl::MgLaunch &G = m->launch;
-        int dims[ifl::kMgMaxLevels][2];
-        G.nl = ifl::mg_levels(c.nx, c.ny, dims);
-        G.lb = ifl::mg_bottom_level(G.nl, dims);
-        size_t total = align256(sizeof(ifl::MgLevel) * ifl::kMgMaxLevels);
-        for (int l = 1; l < G.nl; ++l) total += 7 * align256((size_t)dims[l][0] * dims[l][1] * 4);
+        const size_t total = mgLayout(c, m->launch, nullptr);
         hipError_t e = hipMalloc((void **)&m->arena, total);
         if (e != hipSuccess) {
             delete m;
             return fail(IFL_ERR_DEVICE, std::string("multigrid levels: ") + hipGetErrorString(e));
         }
-        size_t off = align256(sizeof(ifl::MgLevel) * ifl::kMgMaxLevels);
-        for (int l = 0; l < G.nl; ++l) {
-            ifl::MgLevel &L = G.lv[l];
-            L.nx = dims[l][0];
-            L.ny = dims[l][1];
-            if (l == 0) continue;
-            float **pl[7] = {&L.x, &L.xu, &L.b, &L.C, &L.E, &L.N, &L.dg};
-            const size_t b = align256((size_t)L.nx * L.ny * 4);
-            for (float **q : pl) {
-                *q = (float *)(m->arena + off);
-                off += b;
-            }
-        }
-        G.dev = (const ifl::MgLevel *)m->arena;
+        mgLayout(c, m->launch, m->arena);
         h->mg = m;
         e = hipMemsetAsync(m->arena, 0, total, h->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(m->arena, G.lv, sizeof(ifl::MgLevel) * ifl::kMgMaxLevels,
+        if (e == hipSuccess) e = hipMemcpyAsync(m->arena, m->launch.lv, sizeof(ifl::MgLevel) * ifl::kMgMaxLevels,
                                                 hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail(IFL_ERR_DEVICE, std::string("multigrid levels: ") + hipGetErrorString(e));
     }
-    ifl::MgLaunch &G = h->mg->launch;
-    G.cycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
-    G.nu = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
-    G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
-    G.over = (float)(c.mg_over > 0.0 ? c.mg_over : 1.0);
+    mgSettings(c, h->mg->launch);
     return IFL_OK;
 }
 
@@ -455,6 +494,23 @@ size_t fieldSize(const ifl_config &c, int id) {
     }
 }
 
+int finishSolve(ifl_solver *h, long long it0, const SolveTally &t, const ifl::CaseState &s, double ms,
+                ifl_report *report) {
+    if (report) {
+        report->iterations = (int)(h->iter - it0);
+        report->converged = s.done == 1;
+        report->status = s.done == 2 ? IFL_ERR_DIVERGED : IFL_OK;
+        report->elapsed_ms = ms;
+        report->final_scaled = s.lastScaled;
+        report->sweeps_total = t.sweeps;
+    }
+    if (s.done == 2) return fail(IFL_ERR_DIVERGED, "non-finite residual: the solve diverged");
+    if (h->cfg.fail_if_not_converged && s.done != 1)
+        return fail(IFL_ERR_NOT_CONVERGED, "the stop rule was not met in " + std::to_string(h->iter - it0)
+                                               + " outer iterations");
+    return IFL_OK;
+}
+
 int snapTake(ifl_solver *h, Snapshot &s) {
     s.bytes = h->arenaBytes;
     s.iter = h->iter;
@@ -529,32 +585,7 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
     h->cfg = *cfg;
     fillParams(h);
     h->tile = (cfg->dim == 2) ? (cfg->tile ? cfg->tile : ifl::default_tile(cfg->nx, cfg->ny)) : 0;
-    ifl::CaseParams &P = h->P;
-    const size_t N = (size_t)P.N, nx = P.nx, ny = P.ny, nz = P.nz;
-    const size_t ncoef = N * (1 + 2 * P.dim);
-    const size_t pstride = (size_t)partStride(P), ncb = (size_t)ifl::cell_blocks(P);
-    P.partStride = (int)pstride;
-    // arena layout
-    struct Part { void **dst; size_t bytes; };
-    const size_t nT = cfg->energy ? N : 0;     // temperature matrix, source and partner: energy only
-    std::vector<Part> parts = {
-        {(void **)&P.u, N * 4}, {(void **)&P.v, N * 4}, {(void **)&P.w, N * 4}, {(void **)&P.p, N * 4},
-        {(void **)&P.temp, N * 4}, {(void **)&P.pc, N * 4},
-        {(void **)&P.uf, (nx + 1) * ny * nz * 4}, {(void **)&P.vf, nx * (ny + 1) * nz * 4},
-        {(void **)&P.wf, nx * ny * (nz + 1) * 4},
-        {(void **)&P.mcoef, ncoef * 4}, {(void **)&P.pcoef, ncoef * 4},
-        {(void **)&P.bu, N * 4}, {(void **)&P.bv, N * 4}, {(void **)&P.bw, N * 4}, {(void **)&P.bp, N * 4},
-        {(void **)&P.tmp, N * 4}, {(void **)&P.tmp2, N * 4},
-        {(void **)&P.alt[0], N * 4}, {(void **)&P.alt[1], N * 4}, {(void **)&P.alt[2], N * 4},
-        {(void **)&P.alt[3], N * 4}, {(void **)&P.alt[4], N * 4}, {(void **)&P.rinv, N * 4},
-        {(void **)&P.tcoef, nT * (1 + 2 * P.dim) * 4}, {(void **)&P.bt, nT * 4}, {(void **)&P.alt[5], nT * 4},
-        {(void **)&P.spart, (size_t)ifl::kSlots * 2 * ifl::kMaxBlock * pstride * 8},
-        {(void **)&P.partMom, 4 * ncb * 8}, {(void **)&P.partMass, ncb * 8}, {(void **)&P.partUpd, 7 * ncb * 8},
-        {(void **)&P.st, sizeof(ifl::CaseState)},
-        {(void **)&P.hist, sizeof(ifl_iter_record) * kHistCap},
-    };
-    size_t total = 0;
-    for (auto &p : parts) total += align256(p.bytes);
+    const size_t total = arenaLayout(h, nullptr, nullptr);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&h->arena, total);
     h->arenaBytes = total;
@@ -565,11 +596,9 @@ int ifl_create(const ifl_config *cfg, ifl_solver **out) {
         ifl_destroy(h);
         return fail(IFL_ERR_DEVICE, m);
     }
-    size_t off = 0;
-    for (auto &p : parts) {
-        *p.dst = p.bytes ? h->arena + off : nullptr;
-        off += align256(p.bytes);
-    }
+    arenaLayout(h, h->arena, nullptr);
+    ifl::CaseParams &P = h->P;
+    const size_t N = (size_t)P.N;
     // Solver.cu:13-104: uniform initial u, v, w, p, temp; everything else zero.
     e = hipMemsetAsync(h->arena, 0, total, h->stream);
     std::vector<float> host(N);
@@ -671,46 +700,22 @@ int ifl_solve(ifl_solver *h, int max_iter, ifl_report *report, ifl_iter_record *
     t.history = history;
     t.histCap = hist_cap;
     const long long it0 = h->iter;
-    int chunk = h->cfg.check_every > 0 ? std::min(h->cfg.check_every, kHistCap) : 64;
-    std::vector<ifl_iter_record> ring;
+    const int chunk = h->cfg.check_every > 0 ? std::min(h->cfg.check_every, kHistCap) : 64;
+    std::vector<ifl_iter_record> ring((size_t)chunk);
     IFL_HIP(hipEventRecord(h->ev0, h->stream));
     if (h->iter == 0) ifl::launch_init_faces(planOf(h).L);           // Solver.cu:162-164
-    long long launched = 0;
     ifl::CaseState s{};
-    while (launched < max_iter) {
-        int n = (int)std::min<long long>(chunk, max_iter - launched);
+    auto replay = [h](int n) -> int {
         for (int q = 0; q < n; ++q) IFL_HIP(hipGraphLaunch(h->graph, h->stream));
-        launched += n;
-        st = readState(h, s);
-        if (st != IFL_OK) return st;
-        const long long a = h->iter, b = s.iter;
-        if (b > a) {
-            ring.resize((size_t)(b - a));
-            if ((st = copyHistory(h, h->stream, a, b, ring.data())) != IFL_OK) return st;
-            IFL_HIP(hipStreamSynchronize(h->stream));
-            tally(h, t, ring.data(), a, b, s.done != 0, h->cfg.log_every);
-        }
-        h->iter = b;
-        if (s.done) break;
-    }
-    const double sweepsTotal = t.sweeps;
+        return IFL_OK;
+    };
+    st = runChunks(h, max_iter, replay, t, ring.data(), chunk, s);
+    if (st != IFL_OK) return st;
     IFL_HIP(hipEventRecord(h->ev1, h->stream));
     IFL_HIP(hipEventSynchronize(h->ev1));
     float ms = 0;
     IFL_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    if (report) {
-        report->iterations = (int)(h->iter - it0);
-        report->converged = s.done == 1;
-        report->status = s.done == 2 ? IFL_ERR_DIVERGED : IFL_OK;
-        report->elapsed_ms = ms;
-        report->final_scaled = s.lastScaled;
-        report->sweeps_total = sweepsTotal;
-    }
-    if (s.done == 2) return fail(IFL_ERR_DIVERGED, "non-finite residual: the solve diverged");
-    if (h->cfg.fail_if_not_converged && s.done != 1)
-        return fail(IFL_ERR_NOT_CONVERGED, "the stop rule was not met in " + std::to_string(h->iter - it0)
-                                               + " outer iterations");
-    return IFL_OK;
+    return finishSolve(h, it0, t, s, ms, report);
 }
 
 int ifl_run_stage(ifl_solver *h, int stage, double *out, size_t n_out) {

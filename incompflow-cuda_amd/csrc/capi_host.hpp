@@ -134,7 +134,45 @@ struct SolveTally {
 };
 void tally(ifl_solver *h, SolveTally &t, const ifl_iter_record *recs, long long a, long long b, bool doneAtEnd,
            int logEvery);
+// The host loop of a solve: outer iterations in chunks of check_every (64 when 0), each chunk
+// enqueued by launch(n) on h->stream; after each chunk the monitor is read and the new records go,
+// ringCap at a time through ring, to tally.  Stops at max_iter or when the device sets its stop flag.
+template <class Launch>
+int runChunks(ifl_solver *h, int max_iter, Launch &&launch, SolveTally &t, ifl_iter_record *ring, int ringCap,
+              ifl::CaseState &s) {
+    const int chunk = h->cfg.check_every > 0 ? std::min(h->cfg.check_every, kHistCap) : 64;
+    long long launched = 0;
+    int st;
+    while (launched < max_iter) {
+        const int n = (int)std::min<long long>(chunk, max_iter - launched);
+        if ((st = launch(n)) != IFL_OK) return st;
+        launched += n;
+        if ((st = readState(h, s)) != IFL_OK) return st;
+        for (long long a = h->iter; a < s.iter; a += ringCap) {
+            const long long b = std::min<long long>(s.iter, a + ringCap);
+            if ((st = copyHistory(h, h->stream, a, b, ring)) != IFL_OK) return st;
+            IFL_HIP(hipStreamSynchronize(h->stream));
+            tally(h, t, ring, a, b, s.done != 0 && b == s.iter, h->cfg.log_every);
+        }
+        h->iter = s.iter;
+        if (s.done) break;
+    }
+    return IFL_OK;
+}
+
+// The report and return status of a solve that ran from iteration it0 (ifl_solve's contract).
+int finishSolve(ifl_solver *h, long long it0, const SolveTally &t, const ifl::CaseState &s, double ms,
+                ifl_report *report);
 float *fieldPtr(ifl_solver *h, int id);
+// Lay out the arena of a case (every device buffer but the face profiles and the multigrid levels)
+// from base, or only size it when base is null; returns its bytes.  external[q] (u, v, w, p, temp;
+// external may be null) marks a cell field that the caller owns: it takes no space and its pointer
+// is left to the caller.  Sets P.partStride; fillParams must have run.
+size_t arenaLayout(ifl_solver *h, char *base, const bool *external);
+// Lay out the multigrid levels 1.. of a 2D case and their device table from base (null: size only).
+size_t mgLayout(const ifl_config &c, ifl::MgLaunch &G, char *base);
+// Copy the multigrid settings of a configuration into its launch description.
+void mgSettings(const ifl_config &c, ifl::MgLaunch &G);
 size_t fieldSize(const ifl_config &c, int id);
 
 // A copy of the whole device arena (fields, matrices, monitor, history) and the iteration count.

@@ -42,6 +42,9 @@ EXPORTS = [
     ("ifl_batch_case", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     ("ifl_batch_solve", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.Report)]),
     ("ifl_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_abi.TuneResult)]),
+    ("ifl_solve_noalloc", C.c_int, [C.POINTER(_abi.Config), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_int,
+                                    C.POINTER(_abi.Report), C.POINTER(_abi.IterRecord), C.c_size_t]),
 ]
 
 _lib = None
@@ -286,3 +289,45 @@ class Batch:
             s.cfg.tile, s.cfg.sweep_block, s.cfg.sweep_block_mom = r.tile, r.sweep_block, r.sweep_block_mom
         return {"tile": r.tile, "sweep_block": r.sweep_block, "sweep_block_mom": r.sweep_block_mom,
                 "us_per_iter": r.us_per_iter, "trials": r.trials}
+
+
+def solve_noalloc(cfg, u, v, p, w=None, temp=None, face_profiles=None, max_iter=0, stream=None, history=False):
+    """One solve with no device allocation (ifl_solve_noalloc) on caller-owned torch CUDA tensors.
+
+    u, v, p (w in 3D, temp with energy on) are contiguous float32 tensors of nx*ny*nz elements on
+    one device, read as the initial state and overwritten with the result.  face_profiles maps a
+    face to a float32 CUDA tensor of that face's cell count.  stream: a torch.cuda.Stream (default:
+    torch's current stream on the fields' device).  Returns the dict of Solver.solve."""
+    import torch
+    n = cfg.nx * cfg.ny * cfg.nz
+    dev = u.device if isinstance(u, torch.Tensor) else None
+
+    def ptr(t, name, count):
+        if t is None:
+            return None
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and t.numel() == count and t.device == dev):
+            raise IflError(_abi.IFL_ERR_ARG, f"{name} must be a contiguous float32 CUDA tensor of {count} "
+                                             f"elements on the device of u")
+        return C.c_void_p(t.data_ptr())
+
+    args = [ptr(t, nm, n) for t, nm in ((u, "u"), (v, "v"), (w, "w"), (p, "p"), (temp, "temp"))]
+    prof = None
+    if face_profiles:
+        prof = (C.c_void_p * 6)()
+        for face, t in face_profiles.items():
+            cells = (cfg.ny * cfg.nz if face < 2 else cfg.nx * cfg.nz if face < 4 else cfg.nx * cfg.ny)
+            prof[face] = ptr(t, f"face profile {face}", cells)
+    cap = (max_iter if max_iter > 0 else cfg.num_outer_iter) if history else 0
+    hist = (_abi.IterRecord * max(cap, 1))()
+    rep = _abi.Report()
+    with torch.cuda.device(dev):
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        st = lib().ifl_solve_noalloc(C.byref(cfg), C.c_void_p(s.cuda_stream), args[0], args[1], args[2], args[3],
+                                     args[4], prof, max_iter, C.byref(rep), hist, cap)
+    if st not in (0, _abi.IFL_ERR_DIVERGED, _abi.IFL_ERR_NOT_CONVERGED):
+        _check(st)
+    recs = [hist[i] for i in range(min(rep.iterations, cap))]
+    return {"iterations": rep.iterations, "converged": bool(rep.converged), "status": st,
+            "elapsed_ms": rep.elapsed_ms, "final_scaled": rep.final_scaled,
+            "sweeps_total": rep.sweeps_total, "history": recs}
