@@ -380,11 +380,11 @@ def solve_tracking(solver, targets, chunk, budget_s, extra_after):
 
 def mg_leg(args, incompflow, _abi):
     """The 1024^2 cavity with the multigrid p' solve: the autotuner's choice of strength
-    (IFL_TUNE_PSOLVE, from iteration 20), then a solve from rest with it, past 1e-4 to the FP32 floor."""
+    (IFL_TUNE_PSOLVE, from iteration --tune-start), then a solve from rest with it, past 1e-4 to the FP32 floor."""
     c = _abi.cavity_config(n=args.n, p_refine=0, tol_residual=1e-12, max_iter=10 ** 7)
     c.sweep_block, c.sweep_block_mom, c.tile = args.sweep_block, args.sweep_block_mom, args.tile
     t = incompflow.Solver(c)
-    t.solve(20, history=False)
+    t.solve(args.tune_start, history=False)
     choice = t.autotune(_abi.TUNE_PSOLVE, trial_iters=args.tune_iters)
     t.close()
     log(f"[bench] p' solver chosen by the autotuner: {choice['p_solver']} (mg_over {choice['mg_over']}, "
@@ -438,7 +438,9 @@ def main():
     ap.add_argument("--p-refine", type=int, default=1)
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
     ap.add_argument("--no-mg", action="store_true")
-    ap.add_argument("--tune-iters", type=int, default=300)
+    ap.add_argument("--tune-iters", type=int, default=2000)
+    ap.add_argument("--tune-start", type=int, default=2000,
+                    help="outer iterations run before the p' solver trials (past the start-up transient)")
     ap.add_argument("--floor-extra", type=int, default=60000,
                     help="multigrid iterations past the target, to observe the FP32 floor")
     ap.add_argument("--no-batch", action="store_true")
@@ -457,7 +459,9 @@ def main():
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
     ap.add_argument("--no-adjoint-fd", action="store_true")
-    ap.add_argument("--adjoint-fd-step", type=float, default=1e-2)
+    ap.add_argument("--adjoint-fd-step", type=float, default=0.1,
+                    help="relative central-difference step; J is linear in mu and in the inflow up to "
+                         "O(Re) terms, so a large step keeps the FP32 solve noise small against the difference")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()

@@ -213,6 +213,31 @@ __device__ __forceinline__ void blockSumWave0(const double (&v)[K], double (&out
     }
 }
 
+// The sums of blockSumWave0 (every wave adding: the four row totals of each wave, then a wave64
+// sum of the 4 NW row totals in lane order), with value s finished by wave s % NW, so the final
+// wave sums of different values run in different waves.  Every lane of wave w receives in out[r]
+// the total of value w + r NW (where that is < K).  v is used as scratch.  All threads call.
+// Instantiations with different TAGs use LDS arrays of their own, so they may follow each other
+// without a barrier between them.
+template <int K, int NW, int TAG = 0>
+__device__ __forceinline__ void blockSumSpread(double (&v)[K], double (&out)[(K + NW - 1) / NW]) {
+    static_assert(NW <= 16, "64 row totals");
+    __shared__ double lds[4 * NW][K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    rowSums<K>(v);
+    if ((lane & 15) == 15)
+#pragma unroll
+        for (int s = 0; s < K; ++s) lds[4 * wave + (lane >> 4)][s] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < (K + NW - 1) / NW; ++r) {
+        const int s = wave + r * NW;
+        double x[1] = {(s < K && lane < 4 * NW) ? lds[lane][s < K ? s : 0] : 0.0};
+        waveSums<1>(x);
+        out[r] = laneValue(x[0], 63);
+    }
+}
+
 // The same with every thread returning the K totals.
 template <int K, int NW = kWaves>
 __device__ __forceinline__ void blockSum(const double (&v)[K], double (&out)[K]) {

@@ -80,18 +80,22 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     constexpr int HX = TX + 2 * H, HY = TY + 2 * H, HX2 = HX / 2;
     constexpr int SX = TX + 2 * R0, SY = TY + 2 * R0;          // cells ever relaxed
     constexpr int NST = (HX * HY + NT - 1) / NT;
-    constexpr int NSR = (SX * SY + NT - 1) / NT;
     constexpr int NPF = NT >= 1024 ? 1 : 4;                    // prefetched partials per thread and sweep
     constexpr int NSL = (RO::NPC + NT - 1) / NT;               // cells per thread and colour
     constexpr int CO = HY * HX2;                               // colour stride of a field plane
     constexpr int PL = 2 * CO;                                 // field plane stride
+    constexpr int UX = SX + 1, UY = SY, VX = SX, VY = SY + 1;  // face velocities of the relaxed cells
+    constexpr int NUT = PC ? 1 : (UX * UY + NT - 1) / NT, NVT = PC ? 1 : (VX * VY + NT - 1) / NT;
     constexpr int CS = (HY - 2) * HX2;                         // colour stride of a source plane (rows 1..HY-2)
     constexpr int oSRC = NEQ * PL;                             // p' source plane
     constexpr int oEX = NEQ * PL + 2 * CS, oEY = oEX + PL;     // p' links towards +x, +y
-    constexpr int UX = SX + 1, UY = SY, VX = SX, VY = SY + 1;  // face velocities of the relaxed cells
-    constexpr int NUT = PC ? 1 : (UX * UY + NT - 1) / NT, NVT = PC ? 1 : (VX * VY + NT - 1) / NT;
+    constexpr int NSR = PC ? (SX * SY + NT - 1) / NT : 1;      // staged p' sources per thread
     constexpr int oUF = NEQ * PL, oVF = oUF + UY * UX;         // momentum / temperature face planes
     constexpr int NLDS = PC ? oEX + 2 * PL : oVF + VY * VX;
+    // byte offsets, from a cell's base address, of its other-colour north/south neighbours and
+    // of the cell itself (red cells: the base is the cell; black cells: its south neighbour)
+    constexpr int bN0 = 4 * (CO + HX2), bS0 = 4 * (CO - HX2), bN1 = 4 * 2 * HX2, bO1 = 4 * (CO + HX2);
+    static_assert(4 * (NEQ - 1) * PL + bN0 < 65536, "LDS immediate offsets are 16 bits");
     static_assert(HX % 2 == 0, "colour split needs an even width");
     __shared__ float lds[NLDS];
     Blk blk;
@@ -128,13 +132,10 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             cell[c][sl] = ((sl + 1) * NT <= RO::NPC || i < RO::NPC) ? RO::table.cell[c][i] : 0xffff;
         }
 
-    // ---- every global load of the tile first ----
-    // the p' coefficients towards +x and +y are the stored planes of k_pcorr_assemble
-    // (calcPresCorrLinkCoefKernel, kernels.cuh:517-529; 0 on the high boundary and off the grid)
+    // ---- every global load of the tile first: the matrix data, then the field ----
     // Raw buffer loads: one descriptor per array (N floats) and a 32-bit byte offset per cell (a
     // case holds at most 2^28 cells, so 4 N < 2^31); a cell outside the grid gets an offset past
     // the array, which the hardware reads as 0, so no branch surrounds a load.
-    const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
     const int nbytes = P.N * 4;
     __amdgpu_buffer_rsrc_t bIn[NEQ], bSrc[NEQ];
 #pragma unroll
@@ -142,32 +143,32 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         bIn[q] = bufferOf(in[q], nbytes);
         bSrc[q] = bufferOf(src[q], nbytes);
     }
-    const __amdgpu_buffer_rsrc_t bE = bufferOf(cfE, nbytes), bN = bufferOf(cfN, nbytes);
-    float rphi[NEQ][NST], rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[PC ? 1 : 1][PC ? NSR : 1];
+    // matrix rows in registers: the four links, the centre coefficient and the source of each cell
+    float cE[2][NSL], cW[2][NSL], cN[2][NSL], cS[2][NSL], cC[2][NSL], cB[NEQ][2][NSL];
+    float rex[PC ? NST : 1], rey[PC ? NST : 1], rsrc[NSR], ruf[NUT], rvf[NVT];
+    if constexpr (PC) {
+        // the stored links towards +x and +y of k_pcorr_assemble (calcPresCorrLinkCoefKernel,
+        // kernels.cuh:517-529; 0 on the high boundary and off the grid) over the staged region, the
+        // source over the relaxed cells
+        const float *cfE = P.pcoef + (size_t)P.N * sE, *cfN = P.pcoef + (size_t)P.N * sN;
+        const __amdgpu_buffer_rsrc_t bE = bufferOf(cfE, nbytes), bN = bufferOf(cfN, nbytes);
 #pragma unroll
-    for (int m = 0; m < NST; ++m) {
-        const int i = tid + m * NT;
-        const int ly = i / HX, lx = i - ly * HX;
-        const int gx = x0 - H + lx, gy = y0 - H + ly;
-        // one unsigned compare per axis, no short circuit: no branch around the offset either
-        const bool ok = (i < HX * HY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
-        const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
-#pragma unroll
-        for (int q = 0; q < NEQ; ++q) rphi[q][m] = bufferLoad(bIn[q], off);
-        if constexpr (PC) {
+        for (int m = 0; m < NST; ++m) {
+            const int i = tid + m * NT;
+            const int ly = i / HX, lx = i - ly * HX;
+            const int gx = x0 - H + lx, gy = y0 - H + ly;
+            const bool ok = (i < HX * HY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+            const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
             rex[m] = bufferLoad(bE, off);
             rey[m] = bufferLoad(bN, off);
         }
-    }
-    float ruf[NUT], rvf[NVT], csrc[PC ? 1 : NEQ][2][PC ? 1 : NSL];
-    if constexpr (PC) {
 #pragma unroll
         for (int m = 0; m < NSR; ++m) {
             const int i = tid + m * NT;
             const int ly = i / SX, lx = i - ly * SX;
             const int gx = x0 - R0 + lx, gy = y0 - R0 + ly;
             const bool ok = (i < SX * SY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
-            rsrc[0][m] = bufferLoad(bSrc[0], ok ? 4 * (gy * nx + gx) : kNoOffset);
+            rsrc[m] = bufferLoad(bSrc[0], ok ? 4 * (gy * nx + gx) : kNoOffset);
         }
     } else {
         // face velocities over the relaxed cells, coalesced, for LDS; each cell's sources straight
@@ -198,36 +199,46 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 const bool ok = (e != 0xffff) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
                 const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
 #pragma unroll
-                for (int q = 0; q < NEQ; ++q) csrc[q][c][sl] = bufferLoad(bSrc[q], off);
+                for (int q = 0; q < NEQ; ++q) cB[q][c][sl] = bufferLoad(bSrc[q], off);
             }
     }
-    // ---- stage to LDS ----
+    // the field last
+    float rphi[NEQ][NST];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int i = tid + m * NT;
-        if (i < HX * HY) {
-            const int ly = i / HX, lx = i - ly * HX;
-            const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
+        const int ly = i / HX, lx = i - ly * HX;
+        const int gx = x0 - H + lx, gy = y0 - H + ly;
+        // one unsigned compare per axis, no short circuit: no branch around the offset either
+        const bool ok = (i < HX * HY) & ((unsigned)gx < (unsigned)nx) & ((unsigned)gy < (unsigned)ny);
+        const int off = ok ? 4 * (gy * nx + gx) : kNoOffset;
 #pragma unroll
-            for (int q = 0; q < NEQ; ++q) lds[q * PL + a] = rphi[q][m];
-            if constexpr (PC) {
+        for (int q = 0; q < NEQ; ++q) rphi[q][m] = bufferLoad(bIn[q], off);
+    }
+    // ---- stage the matrix data to LDS ----
+    if constexpr (PC) {
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int i = tid + m * NT;
+            if (i < HX * HY) {
+                const int ly = i / HX, lx = i - ly * HX;
+                const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
                 lds[oEX + a] = rex[m];
                 lds[oEY + a] = rey[m];
             }
         }
-    }
 #pragma unroll
-    for (int m = 0; m < (PC ? NSR : 0); ++m) {
-        const int i = tid + m * NT;
-        if (i < SX * SY) {
-            const int sy = i / SX, sx = i - sy * SX;
-            const int lx = sx + 1, ly = sy + 1;            // field coordinates (R0 = H - 1)
-            const int a = ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1);
-            lds[oSRC + a] = rsrc[0][m];
+        for (int m = 0; m < NSR; ++m) {
+            const int i = tid + m * NT;
+            if (i < SX * SY) {
+                const int sy = i / SX, sx = i - sy * SX;
+                const int lx = sx + 1, ly = sy + 1;            // field coordinates (R0 = H - 1)
+                const int a = ((lx + ly) & 1) * CS + (ly - 1) * HX2 + (lx >> 1);
+                lds[oSRC + a] = rsrc[m];
+            }
         }
-    }
-    // the face velocities of the momentum and temperature rows
-    if constexpr (!PC) {
+    } else {
+        // the face velocities of the momentum and temperature rows
 #pragma unroll
         for (int m = 0; m < NUT; ++m) {
             const int i = tid + m * NT;
@@ -239,6 +250,22 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
             if (i < VX * VY) lds[oVF + i] = rvf[m];
         }
     }
+    // one equation: the field is staged after the row setup, its loads overlapping that work; two
+    // (u and v) stage it now, which keeps their registers for the rows
+    constexpr bool kLateField = NEQ == 1;
+    auto stageField = [&]() {
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int i = tid + m * NT;
+            if (i < HX * HY) {
+                const int ly = i / HX, lx = i - ly * HX;
+                const int a = ((lx + ly) & 1) * CO + ly * HX2 + (lx >> 1);
+#pragma unroll
+                for (int q = 0; q < NEQ; ++q) lds[q * PL + a] = rphi[q][m];
+            }
+        }
+    };
+    if constexpr (!kLateField) stageField();
     // ---- partial sums of the previous launch's sweeps (the stop decision): the first NPF*NT
     // records are loaded now and added after the relax phases, so their latency overlaps them ----
     const int nprev = blk.n;
@@ -262,23 +289,22 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
     if (done) return;                      // a finished case: nothing is written
     __syncthreads();
 
-    // ---- this thread's cells: LDS positions and matrix rows, in registers ----
-    // own: the cell's entry in its colour plane (-1: none, or outside the grid); wn: its west
-    // neighbour's entry in the other colour's plane (the east neighbour follows it).
-    int own[2][NSL], wn[2][NSL];
+    // ---- this thread's cells: LDS addresses and matrix rows ----
+    // ab: the byte address of the cell's base in its equation's field plane (red cells: the cell;
+    // black cells: its south neighbour in the red plane), -1 for none or outside the grid; wb: the
+    // byte address of its west neighbour in the other colour's plane (the east neighbour follows).
+    int ab[2][NSL], wb[2][NSL];
     const LinkConsts lk = linkConsts<2>(P, KIND == COEF_TEMP ? P.gam : P.mu);
-    float cE[2][NSL], cW[2][NSL], cN[2][NSL], cS[2][NSL], cC[2][NSL], cB[NEQ][2][NSL];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int s = 0; s < NSL; ++s) {
             const int i = s * NT + tid;
-            own[c][s] = -1;
-            wn[c][s] = 0;
+            ab[c][s] = -1;
+            wb[c][s] = 0;
             cE[c][s] = cW[c][s] = cN[c][s] = cS[c][s] = 0.0f;
             cC[c][s] = 1.0f;
-#pragma unroll
-            for (int q = 0; q < NEQ; ++q) cB[q][c][s] = 0.0f;
+            if constexpr (PC) cB[0][c][s] = 0.0f;
             if ((s + 1) * NT <= RO::NPC || i < RO::NPC) {
                 const int e = cell[c][s];
                 const int lx = e & 255, ly = e >> 8;
@@ -287,8 +313,8 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                     const int o = c * CO + ly * HX2 + (lx >> 1);
                     const int ot = (1 - c) * CO + ly * HX2 + (lx >> 1);
                     const int w = ot + (lx & 1) - 1;
-                    own[c][s] = o;
-                    wn[c][s] = w;
+                    ab[c][s] = 4 * (c == 0 ? o : o - CO - HX2);
+                    wb[c][s] = 4 * w;
                     if constexpr (PC) {
                         cB[0][c][s] = lds[oSRC + o - c * (CO - CS) - HX2];
                         const float aE = lds[oEX + o], aW = lds[oEX + w];
@@ -299,8 +325,6 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                         cS[c][s] = aS;
                         cC[c][s] = -(aE + aW + aN + aS);
                     } else {
-#pragma unroll
-                        for (int q = 0; q < NEQ; ++q) cB[q][c][s] = csrc[q][c][s];
                         const int sx = lx - 1, sy = ly - 1;    // face coordinates (R0 = H - 1)
                         const float uw = lds[oUF + sy * UX + sx], ue = lds[oUF + sy * UX + sx + 1];
                         const float vs = lds[oVF + sy * VX + sx], vn = lds[oVF + (sy + 1) * VX + sx];
@@ -321,6 +345,9 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 }
             }
         }
+    // ---- stage the field (its own LDS planes: the rows above read only the matrix planes) ----
+    if constexpr (kLateField) stageField();
+    __syncthreads();
 
     // ---- the 2K half sweeps ----
     double acc[K][NEQ];
@@ -334,25 +361,28 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         constexpr int p = decltype(pcst)::value;
         constexpr int col = p & 1, ring = 2 * K - 1 - p;
         constexpr int cnt = RO::upTo(ring), core = RO::upTo(0);
-        constexpr int dO = (1 - 2 * col) * CO;              // own entry -> same position, other colour
+        // byte offsets from the base: the cell itself, its north and south neighbours
+        constexpr int bO = col == 0 ? 0 : bO1, bN = col == 0 ? bN0 : bN1, bS = col == 0 ? bS0 : 0;
 #pragma unroll
         for (int s = 0; s < NSL; ++s) {
             if (s * NT < cnt) {          // constant after unrolling
                 const int i = s * NT + tid;
-                const int o = own[col][s];
-                if (o >= 0 && ((s + 1) * NT <= cnt || i < cnt)) {
-                    const int w = wn[col][s];
+                const int a = ab[col][s];
+                if (a >= 0 && ((s + 1) * NT <= cnt || i < cnt)) {
+                    const int w = wb[col][s];
                     const bool isCore = (s + 1) * NT <= core || i < core;
 #pragma unroll
                     for (int q = 0; q < NEQ; ++q) {
                         if (2 * eqs[q].k <= p) continue;
-                        float *f = lds + q * PL;
-                        const float phi_C = f[o];
-                        float newPhi = cB[q][col][s] - cE[col][s] * f[w + 1] - cW[col][s] * f[w]
-                                       - cN[col][s] * f[o + dO + HX2] - cS[col][s] * f[o + dO - HX2];
+                        char *f = (char *)(lds + q * PL);
+                        const float phi_C = *(const float *)(f + a + bO);
+                        const float fE = *(const float *)(f + w + 4), fW = *(const float *)(f + w);
+                        const float fN = *(const float *)(f + a + bN), fS = *(const float *)(f + a + bS);
+                        float newPhi = cB[q][col][s] - cE[col][s] * fE - cW[col][s] * fW - cN[col][s] * fN
+                                       - cS[col][s] * fS;
                         newPhi /= cC[col][s];
                         const float d = rlx[q] * (newPhi - phi_C);
-                        f[o] = phi_C + d;
+                        *(float *)(f + a + bO) = phi_C + d;
                         if (isCore) acc[p >> 1][q] = sqAdd(acc[p >> 1][q], d);
                     }
                 }
@@ -364,16 +394,24 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
         ((Ps < nph ? (phase(std::integral_constant<int, Ps>{}), 0) : 0), ...);
     }(std::make_integer_sequence<int, 2 * K>{});
 
-    // ---- stop decision (launchRuns) from the prefetched and remaining partials ----
+    // ---- the workgroup sums: the stop decision's (launchRuns: the previous launch's partials,
+    // prefetched and remaining) and this launch's sweep norms, each total finished in a wave of its
+    // own.  Every wave adds: the previous launch's records past nprev are +0, which leaves the sums
+    // as they are. ----
+    constexpr int NW = NT / 64, ND = NEQ * K, NR = (ND + NW - 1) / NW;
+    const int wave = tid >> 6, lane = tid & 63;
     bool act[2] = {true, true};
-    {
-        double t[NEQ * K];
+    double totD[NR], totN[NR];      // tot[r]: the total of value wave + r NW
+    __shared__ double sTot[ND];
+    __shared__ int sAct[2];
+    if (launch > 0) {
+        double t[ND];
 #pragma unroll
         for (int q = 0; q < NEQ; ++q)
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
                 double sacc = 0.0;
-                if (launch > 0 && kk < eqs[q].kPrev) {
+                if (kk < eqs[q].kPrev) {
                     const double *part = sweepPart(P, eqs[q].slot, launch - 1, kk);
 #pragma unroll
                     for (int f = 0; f < NPF; ++f) sacc += (tid + f * NT < nprev) ? pf[q][kk][f] : 0.0;
@@ -381,32 +419,49 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 }
                 t[q * K + kk] = sacc;
             }
-        if (launch > 0) {
-            double tot[NEQ * K];
-            // only the waves that hold records (threads below nprev, or all past NPF * NT) add
-            const int active = nprev > NT ? NT / 64 : (nprev + 63) / 64;
-            blockSumWave0<NEQ * K, NT / 64>(t, tot, active);
-            // the decision's FP64 square roots and divisions run in wave 0 only, one sweep per lane
-            // (the leader, thread 0, publishes the state), and reach the other waves through LDS
-            __shared__ int sAct[2];
-            if (tid < 64) {
+        blockSumSpread<ND, NW, 0>(t, totD);
+        // the decision's totals go to wave 0 (read after the next reduction's barrier)
 #pragma unroll
-                for (int q = 0; q < NEQ; ++q) {
-                    double tq[K];
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) tq[kk] = tot[q * K + kk];
-                    const bool a = eqs[q].k > 0 && launchDecideWave<K>(P, blk, eqs[q].slot, launch, eqs[q].kPrev, prev[q], tq, P.tol[eqs[q].q]);
-                    if (tid == 0) sAct[q] = a ? 1 : 0;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < NEQ; ++q) act[q] = sAct[q] != 0;
-        } else {
-#pragma unroll
-            for (int q = 0; q < NEQ; ++q) act[q] = eqs[q].k > 0;
-        }
+        for (int s = 0; s < ND; ++s)
+            if (s % NW == wave && wave != 0 && lane == 0) sTot[s] = totD[s / NW];
     }
+    {
+        double v[ND];
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q)
+#pragma unroll
+            for (int j = 0; j < K; ++j) v[q * K + j] = acc[j][q];
+        blockSumSpread<ND, NW, 1>(v, totN);
+    }
+    if (launch > 0) {
+        // wave 0's lanes take the decision's FP64 square roots and divisions one sweep per lane
+        // (the leader, thread 0, publishes the state); the decision reaches the other waves through LDS
+        if (wave == 0) {
+#pragma unroll
+            for (int q = 0; q < NEQ; ++q) {
+                double tq[K];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) tq[kk] = (q * K + kk) % NW == 0 ? totD[(q * K + kk) / NW] : sTot[q * K + kk];
+                const bool a = eqs[q].k > 0 && launchDecideWave<K>(P, blk, eqs[q].slot, launch, eqs[q].kPrev, prev[q], tq, P.tol[eqs[q].q]);
+                if (tid == 0) sAct[q] = a ? 1 : 0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q) act[q] = sAct[q] != 0;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NEQ; ++q) act[q] = eqs[q].k > 0;
+    }
+    // this launch's partials, each stored by the wave that finished it
+#pragma unroll
+    for (int q = 0; q < NEQ; ++q)
+#pragma unroll
+        for (int j = 0; j < K; ++j) {   // constant indices: no private array in scratch
+            const int sIdx = q * K + j;
+            if (sIdx % NW == wave && lane == 0 && act[q] && j < eqs[q].k)
+                sweepPart(P, eqs[q].slot, launch, j)[blk.flat] = totN[sIdx / NW];
+        }
 
     // ---- write the tile: computed values, or the input unchanged for a stopped equation (keeps
     // the ping-pong parity) ----
@@ -423,20 +478,6 @@ __global__ __launch_bounds__(NT) void k_rb_sweep2d(CaseParams P0, BatchMap M, Sw
                 if (eqs[q].accum) P.pc[g] += val;       // the accumulation of k_pcorr_accumulate
             }
         }
-    }
-    // ---- this launch's sweep norms: one workgroup reduction for all equations and sweeps ----
-    {
-        double v[NEQ * K], tot[NEQ * K];
-#pragma unroll
-        for (int q = 0; q < NEQ; ++q)
-#pragma unroll
-            for (int j = 0; j < K; ++j) v[q * K + j] = acc[j][q];
-        blockSumWave0<NEQ * K, NT / 64>(v, tot);      // the writing threads are in wave 0
-#pragma unroll
-        for (int q = 0; q < NEQ; ++q)
-#pragma unroll
-            for (int j = 0; j < K; ++j)     // constant indices: no private array in scratch
-                if (tid == q * K + j && act[q] && j < eqs[q].k) sweepPart(P, eqs[q].slot, launch, j)[blk.flat] = tot[q * K + j];
     }
 }
 
