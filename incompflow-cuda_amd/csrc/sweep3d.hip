@@ -11,28 +11,29 @@
 // bit (GaussSeidelIterateKernel, kernels.cuh:926-1005, in red-black order, 7-point stencil).
 // Reads come from one buffer and writes go to the other (ping-pong).
 //
-// The momentum equations (u, v, w: one matrix, the seven stored planes) are swept together,
-// NEQ of them per launch, so each coefficient is read once for all of them.  The p' system
-// reads only its +x, +y and +z planes: its matrix is symmetric bit for bit (the coefficient of
-// a cell towards -x is the +x coefficient of its west neighbour, kernels.cuh:517-529 computes
-// both from the same commutative sum) and its diagonal is -(E+W+N+S) - (T+B) as assembled.
-// The coefficients and sources of the cells a step relaxes are loaded ahead, into registers,
-// so their latency overlaps the previous steps.
+// k_rb_zmarch3d sweeps the momentum equations (u, v, w: one matrix, the seven stored planes)
+// together, NEQ of them per launch, so each coefficient is read once for all of them, and the
+// temperature equation; the coefficients and sources of the cells a step relaxes are loaded
+// ahead, into registers, so their latency overlaps the previous steps.  k_rb_zmarch3d_p sweeps
+// the p' system, which reads only its +x, +y and +z planes: its matrix is symmetric bit for bit
+// (the coefficient of a cell towards -x is the +x coefficient of its west neighbour,
+// kernels.cuh:517-529 computes both from the same commutative sum) and its diagonal is
+// -(E+W+N+S) - (T+B) as assembled.
 #include "ifl_device.hpp"
 #include "ifl_kernels.hpp"
 #include "ifl_stencil.hpp"
 
 namespace ifl {
 
-// Column shapes, measured per launch on the 256^3 cavity (us; p' sweep / u+v+w sweep):
-//   128x8x32 162 / 310, 128x8x16 173 / 334, 64x8x32 149 / 461, 64x8x16 154 / 393,
-//   128x16x16 239 / 448, 256x4x32 186 / 522, 256x8x16 253 / 479.
-// The momentum (and temperature) sweep moves three fields and seven coefficient planes per cell
-// and prefers wide columns; the p' sweep (one field, three planes) prefers more workgroups.
+// Column shapes, measured per launch on the 256^3 cavity (us).  u+v+w sweep: 128x8x32 310,
+// 128x8x16 334, 64x8x32 461, 64x8x16 393, 128x16x16 448, 256x4x32 522, 256x8x16 479: it moves three
+// fields and seven coefficient planes per cell and prefers wide columns.  p' sweep (k_rb_zmarch3d_p,
+// links staged in LDS): 32x8x32 138, 64x8x43 139, 64x16x32 139, 64x4x32 146, 32x8x64 145,
+// 64x8x32 165, 64x8x64 166, 128x8x32 216 (64x8x32 with per-cell link loads: 149).
 constexpr int Z_TX = 128, Z_TY = 8, Z_ZC = 32;     // momentum, temperature
-constexpr int ZP_TX = 64, ZP_TY = 8, ZP_ZC = 32;   // p'
+constexpr int ZP_TX = 32, ZP_TY = 8, ZP_ZC = 32;   // p'
 
-template <int TX, int TY, int ZC, int NEQ, bool PC, bool BATCH>
+template <int TX, int TY, int ZC, int NEQ, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap M, SweepSel e0, SweepSel e1,
                                                         SweepSel e2, int launch) {
     constexpr int HX = TX + 4, HY = TY + 4, PL = HX * HY;
@@ -40,13 +41,12 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;       // red cells of tile + ring per plane
     constexpr int BW = TX / 2, NB = TY * BW;                 // black cells of the tile per plane
     constexpr int NRT = (NR + kBlock - 1) / kBlock, NBT = (NB + kBlock - 1) / kBlock;
-    constexpr int NA = PC ? 6 : 7;      // p': E, W, N, S, T, B (C from them); momentum: C, E, W, N, S, T, B
-    static_assert(!PC || NEQ == 1, "one p' equation per launch");
+    constexpr int NA = 7;               // C, E, W, N, S, T, B
     __shared__ float win[NEQ][4][HY][HX];                    // plane z in slot z & 3
     Blk blk;
-    const CaseParams &P = caseOf<BATCH>(P0, M, PC ? MAP_ZMARCH_P : MAP_ZMARCH, blk);
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH, blk);
     if (caseDone(P)) return;
-    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N, nxy = nx * ny;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
     const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
     const int bx = blk.flat % gxc, byz = blk.flat / gxc;
     const int by = byz % gyc, bz = byz / gyc;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
         esrc[q] = slotSource(P, eqs[q].slot);
         erlx[q] = P.relax[eqs[q].q];
     }
-    const float *coef = PC ? P.pcoef : slotCoef(P, e0.slot);    // momentum or temperature planes
+    const float *coef = slotCoef(P, e0.slot);    // momentum or temperature planes
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
     const int z1 = min(z0 + ZC, nz);
     const int tid = threadIdx.x;
@@ -115,17 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     };
     auto loadOps = [&](bool ok, int gx, int gy, int z, Ops &o) {
         const int c = gx + nx * (gy + ny * z);
-        if constexpr (PC) {
-            o.a[0] = ok ? coef[c + N * sE] : 0.0f;
-            o.a[1] = ok && gx > 0 ? coef[c - 1 + N * sE] : 0.0f;
-            o.a[2] = ok ? coef[c + N * sN] : 0.0f;
-            o.a[3] = ok && gy > 0 ? coef[c - nx + N * sN] : 0.0f;
-            o.a[4] = ok ? coef[c + N * sT] : 0.0f;
-            o.a[5] = ok && z > 0 ? coef[c - nxy + N * sT] : 0.0f;
-        } else {
 #pragma unroll
-            for (int s = 0; s < 7; ++s) o.a[s] = ok ? coef[c + N * s] : 0.0f;
-        }
+        for (int s = 0; s < 7; ++s) o.a[s] = ok ? coef[c + N * s] : 0.0f;
 #pragma unroll
         for (int e = 0; e < NEQ; ++e) o.b[e] = ok ? esrc[e][c] : 0.0f;
     };
@@ -167,25 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     for (int e = 0; e < NEQ; ++e) acc[e] = 0.0;
     auto relax = [&](int gx, int gy, int z, bool owned, const Ops &o) {
         const int lx = gx - x0 + 2, ly = gy - y0 + 2;
-        float aC, aE, aW, aN, aS, aT, aB;
-        if constexpr (PC) {
-            aE = o.a[0];
-            aW = o.a[1];
-            aN = o.a[2];
-            aS = o.a[3];
-            aT = o.a[4];
-            aB = o.a[5];
-            aC = -(aE + aW + aN + aS);
-            aC += -(aT + aB);
-        } else {
-            aC = o.a[0];
-            aE = o.a[1];
-            aW = o.a[2];
-            aN = o.a[3];
-            aS = o.a[4];
-            aT = o.a[5];
-            aB = o.a[6];
-        }
+        const float aC = o.a[0], aE = o.a[1], aW = o.a[2], aN = o.a[3], aS = o.a[4], aT = o.a[5], aB = o.a[6];
 #pragma unroll
         for (int e = 0; e < NEQ; ++e) {
             if (!run[e]) continue;
@@ -280,6 +253,225 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
     }
 }
 
+// One red-black sweep of the p' system per launch, on TX x TY columns of ZC planes.  As in
+// k_rb_zmarch3d the column marches up with a rolling window of the field (four planes, 2-cell halo),
+// relaxing at step t the red cells of plane t+1 (tile plus a 1-cell ring) and then the black cells of
+// plane t; here the matrix also comes through LDS: the stored links towards +x and +y (three-plane
+// windows) and towards +z (four planes) are staged over the same region as the field with coalesced
+// loads, one step ahead, so each value is fetched once per plane instead of once per cell that reads
+// it (a cell's -x, -y and -z links are the +x, +y and +z links of its west, south and lower
+// neighbours, bit for bit, kernels.cuh:517-529).  Only the sources are read per cell, into registers.
+template <int TX, int TY, int ZC, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p(CaseParams P0, BatchMap M, SweepSel e, int launch) {
+    constexpr int HX = TX + 4, HY = TY + 4, PL = HX * HY;
+    constexpr int NPL = (PL + kBlock - 1) / kBlock;
+    constexpr int RW = TX / 2 + 1, NR = (TY + 2) * RW;       // red cells of tile + ring per plane
+    constexpr int BW = TX / 2, NB = TY * BW;                 // black cells of the tile per plane
+    constexpr int NRT = (NR + kBlock - 1) / kBlock, NBT = (NB + kBlock - 1) / kBlock;
+    __shared__ float win[4][HY][HX];                          // p', plane z in slot z & 3
+    __shared__ float lT[4][HY][HX];                           // links towards +z, slot z & 3
+    __shared__ float lE[3][HY][HX], lN[3][HY][HX];            // towards +x, +y, slot (z + 3) % 3
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH_P, blk);
+    if (caseDone(P)) return;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
+    const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;   // column grid of the case
+    const int bx = blk.flat % gxc, byz = blk.flat / gxc;
+    const int by = byz % gyc, bz = byz / gyc;
+    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
+    const bool odd = ((launch + e.flip) & 1) != 0;
+    const float *in = odd ? alt : fld;
+    float *out = odd ? fld : alt;
+    const float rlx = P.relax[e.q];
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+    const int z1 = min(z0 + ZC, nz);
+    const int tid = threadIdx.x;
+    // raw buffer loads: an offset past the array reads 0, so no branch surrounds a load
+    const int nbytes = N * 4;
+    const __amdgpu_buffer_rsrc_t bIn = bufferOf(in, nbytes), bSrc = bufferOf(slotSource(P, e.slot), nbytes);
+    const __amdgpu_buffer_rsrc_t bE = bufferOf(P.pcoef + (size_t)N * sE, nbytes);
+    const __amdgpu_buffer_rsrc_t bN = bufferOf(P.pcoef + (size_t)N * sN, nbytes);
+    const __amdgpu_buffer_rsrc_t bT = bufferOf(P.pcoef + (size_t)N * sT, nbytes);
+    auto s3 = [](int z) { return (z + 3) % 3; };              // z >= -3
+
+    // the staged region of plane z (0 off the grid and outside [0, nz))
+    auto loadPlane = [&](__amdgpu_buffer_rsrc_t b, int z, float (&r)[NPL]) {
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int q = tid + m * kBlock;
+            const int ly = q / HX, lx = q - ly * HX;
+            const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+            const bool ok = (q < PL) & ((unsigned)z < (unsigned)nz) & ((unsigned)gx < (unsigned)nx)
+                            & ((unsigned)gy < (unsigned)ny);
+            r[m] = bufferLoad(b, ok ? 4 * (gx + nx * (gy + ny * z)) : kNoOffset);
+        }
+    };
+    auto storePlane = [&](float *w, const float (&r)[NPL]) {
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int q = tid + m * kBlock;
+            if (q < PL) w[q] = r[m];
+        }
+    };
+    struct Links {
+        float e[NPL], n[NPL], t[NPL];
+    };
+    auto loadLinks = [&](int z, Links &l) {
+        loadPlane(bE, z, l.e);
+        loadPlane(bN, z, l.n);
+        loadPlane(bT, z, l.t);
+    };
+    auto storeLinks = [&](int z, const Links &l) {
+        storePlane(&lE[s3(z)][0][0], l.e);
+        storePlane(&lN[s3(z)][0][0], l.n);
+        storePlane(&lT[z & 3][0][0], l.t);
+    };
+    // cell m of this thread in the red (ring) or black (tile) set of plane z; false if off the grid
+    auto redCell = [&](int z, int m, int &gx, int &gy) {
+        const int q = tid + m * kBlock;
+        const int r = q / RW, kk = q - r * RW;
+        gy = y0 - 1 + r;
+        gx = x0 - 1 + 2 * kk + ((x0 - 1 + gy + z) & 1);
+        return q < NR && z >= 0 && z < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    };
+    auto blackCell = [&](int z, int m, int &gx, int &gy) {
+        const int q = tid + m * kBlock;
+        const int r = q / BW, kk = q - r * BW;
+        gy = y0 + r;
+        gx = x0 + 2 * kk + ((x0 + gy + z + 1) & 1);
+        return q < NB && z < nz && gx < nx && gy < ny;
+    };
+    auto loadSrcRed = [&](int z, float (&b)[NRT]) {
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) {
+            int gx, gy;
+            const bool ok = redCell(z, m, gx, gy);
+            b[m] = bufferLoad(bSrc, ok ? 4 * (gx + nx * (gy + ny * z)) : kNoOffset);
+        }
+    };
+    auto loadSrcBlack = [&](int z, float (&b)[NBT]) {
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) {
+            int gx, gy;
+            const bool ok = blackCell(z, m, gx, gy);
+            b[m] = bufferLoad(bSrc, ok ? 4 * (gx + nx * (gy + ny * z)) : kNoOffset);
+        }
+    };
+    auto writePlane = [&](int z, const float *from) {
+        for (int q = tid; q < TX * TY; q += kBlock) {
+            const int r = q / TX, xx = q - r * TX;
+            const int gx = x0 + xx, gy = y0 + r;
+            if (gx < nx && gy < ny) {
+                const int g = gx + nx * (gy + ny * z);
+                out[g] = from ? from[g] : win[z & 3][r + 2][xx + 2];
+            }
+        }
+    };
+
+    double acc = 0.0;
+    auto relax = [&](int gx, int gy, int z, bool owned, float b) {
+        const int lx = gx - x0 + 2, ly = gy - y0 + 2;
+        const int c3 = s3(z);
+        const float aE = lE[c3][ly][lx], aW = lE[c3][ly][lx - 1];
+        const float aN = lN[c3][ly][lx], aS = lN[c3][ly - 1][lx];
+        const float aT = lT[z & 3][ly][lx], aB = lT[(z - 1) & 3][ly][lx];
+        float aC = -(aE + aW + aN + aS);
+        aC += -(aT + aB);
+        float(*wc)[HX] = win[z & 3];
+        const float phi_C = wc[ly][lx];
+        const float phi_E = wc[ly][lx + 1], phi_W = wc[ly][lx - 1];
+        const float phi_N = wc[ly + 1][lx], phi_S = wc[ly - 1][lx];
+        const float phi_T = (z != nz - 1) ? win[(z + 1) & 3][ly][lx] : 0.0f;
+        const float phi_B = (z != 0) ? win[(z - 1) & 3][ly][lx] : 0.0f;
+        float newPhi = b - aE * phi_E - aW * phi_W - aN * phi_N - aS * phi_S;
+        newPhi -= aT * phi_T + aB * phi_B;
+        newPhi /= aC;
+        const float d = rlx * (newPhi - phi_C);
+        wc[ly][lx] = phi_C + d;
+        if (owned) acc = sqAdd(acc, d);
+    };
+    auto redPlane = [&](int z, const float (&b)[NRT]) {
+        const bool ownPlane = z >= z0 && z < z1;
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) {
+            int gx, gy;
+            if (redCell(z, m, gx, gy))
+                relax(gx, gy, z, ownPlane && gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY, b[m]);
+        }
+    };
+    auto blackPlane = [&](int z, const float (&b)[NBT]) {
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) {
+            int gx, gy;
+            if (blackCell(z, m, gx, gy)) relax(gx, gy, z, true, b[m]);
+        }
+    };
+
+    // field planes z0-2..z0+1 and the links of planes z0-2 (+z only), z0-1 and z0 now; the field
+    // of plane z0+2 and the links of plane z0+1 stay in flight until the first step stores them
+    float pend[NPL];
+    Links pendL;
+    float redB[NRT], blkB[NBT], redN[NRT], blkN[NBT];
+    {
+        float f[4][NPL];
+        Links l[2];
+        float tlow[NPL];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) loadPlane(bIn, z0 - 2 + s, f[s]);
+        loadPlane(bT, z0 - 2, tlow);
+        loadLinks(z0 - 1, l[0]);
+        loadLinks(z0, l[1]);
+        loadPlane(bIn, z0 + 2, pend);
+        loadLinks(z0 + 1, pendL);
+        float r0[NRT];
+        loadSrcRed(z0 - 1, r0);
+        loadSrcRed(z0, redB);
+        loadSrcBlack(z0, blkB);
+        // the stop decision, with these loads in flight
+        if (!launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q])) {
+            // a stopped call: the input unchanged (keeps the ping-pong parity)
+            for (int z = z0; z < z1; ++z) writePlane(z, in);
+            return;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) storePlane(&win[(z0 - 2 + s) & 3][0][0], f[s]);
+        storePlane(&lT[(z0 - 2) & 3][0][0], tlow);
+        storeLinks(z0 - 1, l[0]);
+        storeLinks(z0, l[1]);
+        __syncthreads();
+        redPlane(z0 - 1, r0);
+    }
+    redPlane(z0, redB);
+    loadSrcRed(z0 + 1, redB);
+    __syncthreads();
+    // Step t: store old plane t+2 (over t-2) and the links of plane t+1 (over those of t-2, and of
+    // t-3 for +z), barrier, red(t+1) (field planes t..t+2, links of t+1 and +z of t), barrier,
+    // black(t) (field t-1..t+1, links of t and +z of t-1), barrier, write plane t.
+    for (int t = z0; t < z1; ++t) {
+        storePlane(&win[(t + 2) & 3][0][0], pend);
+        storeLinks(t + 1, pendL);
+        loadPlane(bIn, t + 3, pend);
+        loadLinks(t + 2, pendL);
+        loadSrcRed(t + 2, redN);
+        loadSrcBlack(t + 1, blkN);
+        __syncthreads();
+        redPlane(t + 1, redB);
+        __syncthreads();
+        blackPlane(t, blkB);
+        __syncthreads();
+        writePlane(t, nullptr);
+#pragma unroll
+        for (int m = 0; m < NRT; ++m) redB[m] = redN[m];
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) blkB[m] = blkN[m];
+    }
+    {
+        double v[1] = {acc}, tot[1];
+        blockSum<1>(v, tot);
+        if (tid == 0) sweepPart(P, e.slot, launch, 0)[blk.flat] = tot[0];
+    }
+}
+
 int zmarch_blocks(const CaseParams &P, bool pc) {
     if (pc) return ((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY) * ((P.nz + ZP_ZC - 1) / ZP_ZC);
     return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
@@ -292,9 +484,9 @@ static void launchZ(const Lc &L, const SweepSel *e, int launch) {
     const CaseParams &P = *L.P;
     const dim3 grid = BATCH ? dim3((unsigned)L.total[PC ? MAP_ZMARCH_P : MAP_ZMARCH]) : dim3((unsigned)zmarch_blocks(P, PC));
     if constexpr (PC)
-        k_rb_zmarch3d<ZP_TX, ZP_TY, ZP_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[0], e[0], launch);
+        k_rb_zmarch3d_p<ZP_TX, ZP_TY, ZP_ZC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], launch);
     else
-        k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, PC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
+        k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
                                                                               e[NEQ > 2 ? 2 : 0], launch);
 }
 
