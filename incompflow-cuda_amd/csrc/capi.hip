@@ -129,6 +129,24 @@ int sweepBlockOf(const ifl_config &c, int dim, int slot) {
     return std::max(c.sweep_block, 1);
 }
 
+// Whether an outer iteration's momentum call has an odd launch count for some component, so that
+// it starts in the partner buffers (SmoothMode::fast), which must then equal the fields.
+bool momentumStartsInPartner(const ifl_config &c, int dim) {
+    for (int q = 0; q < dim; ++q) {
+        const int n = c.n_iter[q], K = sweepBlockOf(c, dim, q);
+        if (n > 0 && ((n + K - 1) / K) % 2 == 1) return true;
+    }
+    return false;
+}
+
+// Whether an outer iteration's p' smoother call (the reference's sweeps) has an odd launch count,
+// so that it starts in the partner buffer, which k_pcorr_assemble then clears as well.
+bool pressureStartsInPartner(const ifl_config &c, int dim) {
+    if (c.p_solver != IFL_PSOLVE_SWEEPS) return false;
+    const int n = c.n_iter[3], K = sweepBlockOf(c, dim, 3);
+    return ((n + K - 1) / K) % 2 == 1;
+}
+
 // Equation index of a smoother slot (relax / tol / sweep counts): u, v, w, p' (p' and its
 // refinement correction), T.
 int slotEq(int slot) { return slot < 3 ? slot : slot == ifl::kSlotT ? 4 : 3; }
@@ -268,9 +286,9 @@ void enqueueOuter(const Plan &pl) {
     enqueueMomentumSmoothers(pl, md);                                // :178-184
     enqueueTemperatureSmoother(pl, md);
     ifl::launch_rhie_chow(pl.L, fm);                                 // :186-187
-    ifl::launch_pcorr_assemble(pl.L);                                // :189-193
+    ifl::launch_pcorr_assemble(pl.L, pressureStartsInPartner(*pl.cfg, pl.dim));   // :189-193
     const ifl::Finish fp = enqueuePressureCorrection(pl, true);      // :196
-    ifl::launch_update(pl.L, fp);                                    // :198
+    ifl::launch_update(pl.L, fp, momentumStartsInPartner(*pl.cfg, pl.dim));   // :198
     ifl::launch_reduce_cells(pl.L, ifl::RED_MOM | ifl::RED_MASS | ifl::RED_UPD | ifl::RED_FINISH);  // :200-268
 }
 
@@ -427,6 +445,14 @@ int armCase(ifl_solver *h, hipStream_t s, long long cap) {
     static const int zero = 0;
     IFL_HIP(hipMemcpyAsync(&h->P.st->done, &zero, sizeof(int), hipMemcpyHostToDevice, s));
     IFL_HIP(hipMemcpyAsync(&h->P.st->iterCap, &cap, sizeof(long long), hipMemcpyHostToDevice, s));
+    // an outer iteration whose momentum call starts in the partner buffers needs them equal to
+    // the fields; the iterations keep them so (k_update's writeAlt), a call starting now sets them
+    if (momentumStartsInPartner(h->cfg, h->P.dim)) {
+        Plan pl = planOf(h);
+        pl.L.s = s;
+        for (int q = 0; q < h->P.dim; ++q) ifl::launch_copy_back(pl.L, q, 1);
+        IFL_HIP(hipGetLastError());
+    }
     IFL_HIP(hipStreamSynchronize(s));
     return IFL_OK;
 }

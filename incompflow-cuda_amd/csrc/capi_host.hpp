@@ -89,6 +89,8 @@ struct Plan {
 };
 
 int sweepBlockOf(const ifl_config &c, int dim, int slot);
+bool momentumStartsInPartner(const ifl_config &c, int dim);
+bool pressureStartsInPartner(const ifl_config &c, int dim);
 int slotEq(int slot);
 int slotSweeps(const ifl_config &c, int slot);
 int sweepKind(const Plan &pl, int slot);

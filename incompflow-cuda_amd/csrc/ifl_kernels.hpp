@@ -47,8 +47,10 @@ void launch_face_bc(const Lc &L);
 void launch_mom_assemble(const Lc &L);
 // fin: smoother calls closed by workgroup 0 of this launch (instead of k_smooth_finish)
 void launch_rhie_chow(const Lc &L, const Finish &fin = Finish{});
-void launch_pcorr_assemble(const Lc &L);
-void launch_update(const Lc &L, const Finish &fin = Finish{});
+// writeAlt: also clear the p' smoother's partner buffer (a p' call with an odd launch count starts there)
+void launch_pcorr_assemble(const Lc &L, bool writeAlt = true);
+// writeAlt: also store the corrected u, v, w into the momentum smoothers' partner buffers
+void launch_update(const Lc &L, const Finish &fin = Finish{}, bool writeAlt = true);
 void launch_pcorr_defect(const Lc &L, const Finish &fin = Finish{});
 void launch_pcorr_accumulate(const Lc &L);
 void launch_reduce_cells(const Lc &L, int mask);

@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_rhie_chow(CaseParams P0, BatchMap M,
 
 // ---------------- pressure-correction assembly, kernels.cuh:502-581, and p' = 0 ----------------
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchMap M) {
+__global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchMap M, int writeAlt) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_pcorr_assemble(CaseParams P0, BatchM
         if (DIM == 3) b += rho * Az * (wfB - wfT);
         P.bp[c] = b;
         P.pc[c] = 0.0f;                       // Solver.cu:193
-        P.alt[3][c] = 0.0f;                   // the partner buffer too (odd launch counts start there)
+        if (writeAlt) P.alt[3][c] = 0.0f;     // the partner too, when the p' call starts there
         s[0] += (double)b * (double)b;
     });
     storeBlockSums<1>(s, P.partMass + blk.flat);
@@ -541,7 +541,7 @@ __device__ __forceinline__ float faceCorr7(const CaseParams &P, int d, float aC_
 }
 
 template <int DIM, bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M, Finish fin) {
+__global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M, Finish fin, int writeAlt) {
     Blk blk;
     const CaseParams &P = caseOf<BATCH>(P0, M, MAP_CELL, blk);
     if (caseDone(P)) return;
@@ -559,23 +559,23 @@ __global__ __launch_bounds__(kBlock) void k_update(CaseParams P0, BatchMap M, Fi
         const float dp = P.relax[3] * pc[0];            // kernels.cuh:825
         P.p[c] = p0 + dp;
         s[0] += (double)dp * dp;
-        // corrected velocities also go to the smoothers' partner buffers, which a momentum call
-        // with an odd launch count reads first
+        // corrected velocities also go to the smoothers' partner buffers when a momentum call
+        // with an odd launch count reads them first (writeAlt)
         const float du = velCorr7(P, 0, i, P.nx, pc, aC[0]);
         const float un = u0 + du;
         P.u[c] = un;
-        P.alt[0][c] = un;
+        if (writeAlt) P.alt[0][c] = un;
         s[1] += (double)du * du;
         const float dv = velCorr7(P, 1, j, P.ny, pc, aC[0]);
         const float vn = v0 + dv;
         P.v[c] = vn;
-        P.alt[1][c] = vn;
+        if (writeAlt) P.alt[1][c] = vn;
         s[2] += (double)dv * dv;
         if (DIM == 3) {
             const float dw = velCorr7(P, 2, k, P.nz, pc, aC[0]);
             const float wn = w0 + dw;
             P.w[c] = wn;
-            P.alt[2][c] = wn;
+            if (writeAlt) P.alt[2][c] = wn;
             s[3] += (double)dw * dw;
         }
         if (i >= 1) {
@@ -761,8 +761,10 @@ void launch_init_faces(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_init_faces);
 void launch_face_bc(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_face_bc); }
 void launch_mom_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_mom_assemble); }
 void launch_rhie_chow(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_rhie_chow, fin); }
-void launch_pcorr_assemble(const Lc &L) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_assemble); }
-void launch_update(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_update, fin); }
+void launch_pcorr_assemble(const Lc &L, bool writeAlt) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_assemble, writeAlt ? 1 : 0); }
+void launch_update(const Lc &L, const Finish &fin, bool writeAlt) {
+    IFL_LAUNCH_DIM(L, MAP_CELL, k_update, fin, writeAlt ? 1 : 0);
+}
 void launch_pcorr_defect(const Lc &L, const Finish &fin) { IFL_LAUNCH_DIM(L, MAP_CELL, k_pcorr_defect, fin); }
 void launch_pcorr_accumulate(const Lc &L) { IFL_LAUNCH(L, MAP_CELL, k_pcorr_accumulate); }
 void launch_reduce_cells(const Lc &L, int mask) { IFL_LAUNCH(L, MAP_ONE, k_reduce_cells, cell_blocks(*L.P), mask); }
