@@ -12,7 +12,8 @@ import numpy as np
 from . import _abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libincompflow.so")
+# IFL_LIBRARY selects another build of the same sources (the host-sanitizer build, csrc/_san/)
+LIB_PATH = os.environ.get("IFL_LIBRARY") or os.path.join(_HERE, "libincompflow.so")
 
 # (name, restype, argtypes) for every entry point declared in include/incompflow.h
 EXPORTS = [

@@ -15,7 +15,8 @@ _ROOT = os.path.dirname(_HERE)
 sys.path.insert(0, os.path.join(_ROOT, "incompflow-cuda_amd"))
 from incompflow import _abi  # noqa: E402
 
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# IFL_ORACLE_LIB selects another build of the same sources (the sanitizer build, oracle/_san/)
+LIB_PATH = os.environ.get("IFL_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 

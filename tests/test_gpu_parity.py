@@ -178,6 +178,27 @@ def test_nan_initial_field_reports_divergence():
     assert r["iterations"] == 1
 
 
+@pytest.mark.parametrize("which", ["u", "p"])
+@pytest.mark.parametrize("name", ["cavity2d", "channel2d", "cavity3d"])
+def test_over_relaxation_diverges_at_the_oracles_iteration(name, which):
+    """alpha = 1.9 blows the iteration up; the device must return IFL_ERR_DIVERGED at the same
+    outer iteration as the FP32 oracle (their iterates are the same bits until then)."""
+    cfg = CASES[name]()
+    cfg.stop_rule = _abi.STOP_NONE
+    if which == "u":
+        for q in range(3):
+            cfg.relax[q] = 1.9
+    else:
+        cfg.relax[3] = 1.9
+    _log(f"[divergence] case={name} alpha_{which}=1.9 grid={cfg.nx}x{cfg.ny}x{cfg.nz}")
+    g = Solver(cfg)
+    r = g.solve(3000)
+    ro = Oracle(cfg).solve(3000, history=False)
+    assert ro["status"] == _abi.IFL_ERR_DIVERGED
+    assert r["status"] == _abi.IFL_ERR_DIVERGED
+    assert r["iterations"] == ro["iterations"] < 200
+
+
 def test_vtk_output_format(tmp_path):
     cfg = CASES["cavity2d"]()
     g = Solver(cfg)
