@@ -171,6 +171,17 @@ def cavity3d_leg(args, incompflow, _abi):
     out = {"workload": f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
                        "FP64 reductions", "cells": cells, "ms_per_step": ms, "sweeps_per_step": sweeps,
            "cell_updates_per_s": cells * sweeps / (ms * 1e-3), "kernels": kern, "cpu_ms_per_step": None}
+    # time to residual: measured by the opt-in --c3-converge run and kept under profiles/
+    cfile = os.path.join(ROOT, "profiles", f"cavity3d_{n}_time_to_residual.json")
+    if args.c3_converge:
+        out["time_to_residual"] = cavity3d_converge(args, incompflow, _abi, n)
+        if args.c3_converge_out:
+            with open(args.c3_converge_out, "w") as fh:
+                json.dump(out["time_to_residual"], fh, indent=1)
+    elif os.path.exists(cfile):
+        with open(cfile) as fh:
+            out["time_to_residual"] = json.load(fh)
+        out["time_to_residual"]["source"] = os.path.relpath(cfile, ROOT)
     if not args.no_cpu and args.c3_cpu_iters > 0:
         from oracle import oracle as orc
         o = orc.Oracle(c)
@@ -183,6 +194,34 @@ def cavity3d_leg(args, incompflow, _abi):
         o.close()
         log(f"[bench] cavity3d cpu oracle {out['cpu_ms_per_step']:.1f} ms/iteration")
     return out
+
+
+def cavity3d_converge(args, incompflow, _abi, n):
+    """The 3D cavity from rest towards the scaled residuals 1e-3, 1e-4 and 1e-5 (stop rule off, so
+    the run continues to its budget and shows where FP32 stalls): outer iterations and device time
+    at which each was first reached, and the smallest scaled residual seen."""
+    c = _abi.cavity3d_config(n=n, tol_residual=1e-12, max_iter=10 ** 7)
+    c.p_refine = 0
+    c.stop_rule = _abi.STOP_NONE
+    c.num_outer_iter = 10 ** 7
+    if args.c3_mg_over > 0:
+        c.p_solver, c.mg_cycles, c.mg_over = _abi.PSOLVE_MG, 1, args.c3_mg_over
+    targets = [1e-3, 1e-4, 1e-5]
+    log(f"[bench] cavity3d {n}^3 from rest towards {targets} (budget {args.c3_converge_budget_s:.0f} s)")
+    s = incompflow.Solver(c)
+    hit, best, done, dev_s = solve_tracking(s, targets, 2000, args.c3_converge_budget_s, extra_after=10 ** 7,
+                                            first=None)
+    s.close()
+    psolve = (f"one multigrid V-cycle for p' (mg_over {args.c3_mg_over:g})" if args.c3_mg_over > 0
+              else "20 p' sweeps")
+    return {"workload": f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10 momentum sweeps, {psolve}, "
+                        "alpha 0.7/0.3",
+            "command": f"python3 bench.py --c3-converge --c3-converge-budget-s {args.c3_converge_budget_s:g} "
+                       + (f"--c3-mg-over {args.c3_mg_over:g} " if args.c3_mg_over > 0 else "")
+                       + "--no-converge --no-mg --no-cpu --no-batch --no-adjoint --no-energy --steps 5 --warmup 2",
+            "targets": {f"{t:g}": ({"iterations": hit[t][0], "device_s": hit[t][1]} if hit[t] else None)
+                        for t in targets},
+            "min_scaled": best, "iterations_run": done, "device_s": dev_s}
 
 
 def adjoint_leg(args, incompflow, _abi):
@@ -233,12 +272,29 @@ def adjoint_leg(args, incompflow, _abi):
            # creeping-flow identities (J linear in mu and in the inflow at Re ~ 1)
            "mu_dJdmu_over_J": c.viscosity * gmu / J, "theta_dJdtheta_over_J": float(np.dot(prof, gp)) / J,
            "fd": None}
-    if not args.no_adjoint_fd:
+    # the gradient against central differences of converged FP64 oracle solves at this size (step 1e-3
+    # relative), measured by tests/c5_adjoint_fd.py and kept under profiles/
+    ffile = os.path.join(ROOT, "profiles", "adjoint_fd_fp64_c5_1024x256.json")
+    if os.path.exists(ffile):
+        with open(ffile) as fh:
+            ff = json.load(fh)
+        out["fd_fp64_oracle"] = {k: ff[k] for k in ("fd_step_relative", "fd_tol", "dJ_dmu_fd", "dJ_dmu_adjoint",
+                                                     "dJ_dmu_rel_err", "dir_profile_fd", "dir_profile_adjoint",
+                                                     "dir_profile_rel_err", "J_oracle_fp64", "J_device_fp32")}
+        out["fd_fp64_oracle"]["source"] = os.path.relpath(ffile, ROOT)
+        out["fd_fp64_oracle"]["this_run"] = {
+            "dJ_dmu_rel_err": abs(gmu - ff["dJ_dmu_fd"]) / abs(ff["dJ_dmu_fd"]),
+            "dir_profile_rel_err": abs(float(np.dot(prof, gp)) - ff["dir_profile_fd"]) / abs(ff["dir_profile_fd"])}
+        log(f"[bench]   against FP64-oracle differences: dJ/dmu rel err "
+            f"{out['fd_fp64_oracle']['this_run']['dJ_dmu_rel_err']:.2e}, profile direction rel err "
+            f"{out['fd_fp64_oracle']['this_run']['dir_profile_rel_err']:.2e}")
+    if args.adjoint_fd32:
         fd_its = []
 
         def J_of(mu, p):
             c2 = c.copy()
             c2.viscosity = mu
+            c2.num_outer_iter = 60000          # a solve that stalls at its FP32 floor stops here
             s2 = incompflow.Solver(c2)
             s2.set_face_profile(_abi.WEST, p)
             s2.load_state(base_state)        # warm start from the converged base state
@@ -351,7 +407,7 @@ def host_cpu():
             "cpus_allowed": allowed}
 
 
-def solve_tracking(solver, targets, chunk, budget_s, extra_after):
+def solve_tracking(solver, targets, chunk, budget_s, extra_after, first=0):
     """Run a solver (whose own stop rule never triggers) in chunks with its history; return the
     iteration and device time at which each target scaled residual was first reached, the minimum
     reached, the iterations run and their device time.  After the first target is reached the solve
@@ -371,7 +427,7 @@ def solve_tracking(solver, targets, chunk, budget_s, extra_after):
         done += r["iterations"]
         ms += r["elapsed_ms"]
         log(f"[bench]   {done} iterations, scaled residual {h[-1].scaled:.3e} (min {best:.3e}), {ms / 1000:.1f} s")
-        if stop_at is None and hit[targets[0]] is not None:
+        if stop_at is None and first is not None and hit[targets[first]] is not None:
             stop_at = hit[targets[0]][0] + extra_after
         if r["status"] != 0 or time.perf_counter() - t0 > budget_s:
             break
@@ -455,10 +511,17 @@ def main():
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--c3-cpu-iters", type=int, default=2)
+    ap.add_argument("--c3-converge", action="store_true",
+                    help="solve the 3D cavity from rest towards 1e-5 within --c3-converge-budget-s")
+    ap.add_argument("--c3-converge-budget-s", type=float, default=600.0)
+    ap.add_argument("--c3-converge-out", default=None, help="write the --c3-converge result to this JSON file")
+    ap.add_argument("--c3-mg-over", type=float, default=0.0,
+                    help="> 0: the --c3-converge run solves p' by one multigrid V-cycle with this mg_over")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
-    ap.add_argument("--no-adjoint-fd", action="store_true")
+    ap.add_argument("--adjoint-fd32", action="store_true",
+                    help="also difference FP32 device solves (their residual floor limits them to ~1e-3)")
     ap.add_argument("--adjoint-fd-step", type=float, default=0.1,
                     help="relative central-difference step; J is linear in mu and in the inflow up to "
                          "O(Re) terms, so a large step keeps the FP32 solve noise small against the difference")

@@ -167,7 +167,7 @@ typedef struct ifl_config {
      * step applies there (kernels.cuh:713-727).  A converged state is the same for every p' solver
      * (b' = 0 gives p' = 0); the solver only changes how fast it is reached.  A near-exact p' solve
      * makes this SIMPLE variant diverge, so the strength (mg_cycles, mg_over) is a tuning choice
-     * (ifl_autotune, IFL_TUNE_PSOLVE).  2D only; not with p_refine or in a batch. */
+     * (ifl_autotune, IFL_TUNE_PSOLVE).  2D and 3D; not with p_refine or in a batch. */
     int32_t p_solver;            /* IFL_PSOLVE_SWEEPS (default) or IFL_PSOLVE_MG */
     int32_t mg_cycles;           /* V-cycles per p' solve, 1..8 (0 = 1) */
     int32_t mg_sweeps;           /* red-black sweeps before and after each coarse correction, 1..3 (0 = 2) */
@@ -342,7 +342,7 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
  * chosen).  On an error return the configuration is the one the call started with.
  * trial_iters <= 0 selects 300. */
 enum { IFL_TUNE_LAUNCH = 1, IFL_TUNE_SWEEPS = 2,
-       IFL_TUNE_PSOLVE = 4 };   /* 2D: the p' solver and its strength (the configured sweeps, or one
+       IFL_TUNE_PSOLVE = 4 };   /* the p' solver and its strength (the configured sweeps, or one
                                    multigrid V-cycle with mg_over 1.0, 1.4 or 1.8), by the drop of the
                                    scaled residual per second over trial_iters outer iterations */
 

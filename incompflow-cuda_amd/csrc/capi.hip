@@ -48,10 +48,9 @@ int validate(const ifl_config &c) {
     if (c.mg_coarse_sweeps < 0 || c.mg_coarse_sweeps > 200) return fail(IFL_ERR_ARG, "mg_coarse_sweeps must be 0..200");
     if (!(c.mg_over >= 0.0 && c.mg_over <= 2.0)) return fail(IFL_ERR_ARG, "mg_over must be in [0, 2] (0: 1.0)");
     if (c.p_solver == IFL_PSOLVE_MG) {
-        if (c.dim != 2) return fail(IFL_ERR_ARG, "IFL_PSOLVE_MG is built for 2D grids");
         if (c.p_refine != 0) return fail(IFL_ERR_ARG, "p_refine needs p_solver = IFL_PSOLVE_SWEEPS");
         int dims[ifl::kMgMaxLevels][2];
-        if (ifl::mg_bottom_level(ifl::mg_levels(c.nx, c.ny, dims), dims) < 0)
+        if (c.dim == 2 && ifl::mg_bottom_level(ifl::mg_levels(c.nx, c.ny, dims), dims) < 0)
             return fail(IFL_ERR_ARG, "IFL_PSOLVE_MG: the grid is too elongated (its coarsest level has more than 5600 cells)");
     }
     if (c.energy) {
@@ -255,8 +254,9 @@ void enqueueTemperatureSmoother(const Plan &pl, const SmoothMode &md) {
 // Solver.cu:196 plus the FP64-defect refinement cycles.  Returns the calls still to be closed
 // (fast path) by the next kernel.
 ifl::Finish enqueuePressureCorrection(const Plan &pl, bool fast) {
-    if (pl.cfg->p_solver == IFL_PSOLVE_MG && pl.mg) {    // mgEnsure made the levels
-        ifl::launch_mg_solve(pl.L, *pl.mg);
+    if (pl.cfg->p_solver == IFL_PSOLVE_MG && (pl.mg || pl.mg3)) {    // mgEnsure made the levels
+        if (pl.dim == 3) ifl::launch_mg3_solve(pl.L, *pl.mg3);
+        else ifl::launch_mg_solve(pl.L, *pl.mg);
         return ifl::Finish{};
     }
     const int sp = 3, sd = 4;
@@ -325,7 +325,28 @@ size_t arenaLayout(ifl_solver *h, char *base, const bool *external) {
     return off;
 }
 
-size_t mgLayout(const ifl_config &c, ifl::MgLaunch &G, char *base) {
+size_t mgLayout(const ifl_config &c, MgHost &m, char *base) {
+    if (c.dim == 3) {
+        ifl::MgLaunch3 &G = m.launch3;
+        int dims[ifl::kMgMaxLevels][3];
+        G.nl = ifl::mg3_levels(c.nx, c.ny, c.nz, dims);
+        size_t off = 0;
+        for (int l = 0; l < G.nl; ++l) {
+            ifl::MgLevel3 &L = G.lv[l];
+            L.nx = dims[l][0];
+            L.ny = dims[l][1];
+            L.nz = dims[l][2];
+            if (l == 0) continue;
+            float **pl[7] = {&L.x, &L.b, &L.C, &L.E, &L.N, &L.T, &L.dg};
+            const size_t b = align256((size_t)L.nx * L.ny * L.nz * 4);
+            for (float **q : pl) {
+                *q = base ? (float *)(base + off) : nullptr;
+                off += b;
+            }
+        }
+        return off ? off : 256;       // a grid too small to coarsen has level 0 only
+    }
+    ifl::MgLaunch &G = m.launch;
     int dims[ifl::kMgMaxLevels][2];
     G.nl = ifl::mg_levels(c.nx, c.ny, dims);
     G.lb = ifl::mg_bottom_level(G.nl, dims);
@@ -346,11 +367,15 @@ size_t mgLayout(const ifl_config &c, ifl::MgLaunch &G, char *base) {
     return off;
 }
 
-void mgSettings(const ifl_config &c, ifl::MgLaunch &G) {
-    G.cycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
-    G.nu = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
-    G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
-    G.over = (float)(c.mg_over > 0.0 ? c.mg_over : 1.0);
+void mgSettings(const ifl_config &c, MgHost &m) {
+    auto set = [&](auto &G) {
+        G.cycles = c.mg_cycles > 0 ? c.mg_cycles : 1;
+        G.nu = c.mg_sweeps > 0 ? c.mg_sweeps : 2;
+        G.ncoarse = c.mg_coarse_sweeps > 0 ? c.mg_coarse_sweeps : 4;
+        G.over = (float)(c.mg_over > 0.0 ? c.mg_over : 1.0);
+    };
+    set(m.launch);
+    set(m.launch3);
 }
 
 Plan planOf(ifl_solver *h) {
@@ -361,7 +386,9 @@ Plan planOf(ifl_solver *h) {
     pl.L.s = h->stream;
     pl.cfg = &h->cfg;
     pl.dim = h->P.dim;
-    pl.mg = (h->cfg.p_solver == IFL_PSOLVE_MG && h->mg) ? &h->mg->launch : nullptr;
+    const bool mg = h->cfg.p_solver == IFL_PSOLVE_MG && h->mg;
+    pl.mg = mg && pl.dim == 2 ? &h->mg->launch : nullptr;
+    pl.mg3 = mg && pl.dim == 3 ? &h->mg->launch3 : nullptr;
     return pl;
 }
 
@@ -371,21 +398,22 @@ int mgEnsure(ifl_solver *h) {
     if (!h->mg) {
         MgHost *m = new (std::nothrow) MgHost();
         if (!m) return fail(IFL_ERR_NOMEM, "out of host memory");
-        const size_t total = mgLayout(c, m->launch, nullptr);
+        const size_t total = mgLayout(c, *m, nullptr);
         hipError_t e = hipMalloc((void **)&m->arena, total);
         if (e != hipSuccess) {
             delete m;
             return fail(IFL_ERR_DEVICE, std::string("multigrid levels: ") + hipGetErrorString(e));
         }
-        mgLayout(c, m->launch, m->arena);
+        mgLayout(c, *m, m->arena);
         h->mg = m;
         e = hipMemsetAsync(m->arena, 0, total, h->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(m->arena, m->launch.lv, sizeof(ifl::MgLevel) * ifl::kMgMaxLevels,
-                                                hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess && c.dim == 2)
+            e = hipMemcpyAsync(m->arena, m->launch.lv, sizeof(ifl::MgLevel) * ifl::kMgMaxLevels,
+                               hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail(IFL_ERR_DEVICE, std::string("multigrid levels: ") + hipGetErrorString(e));
     }
-    mgSettings(c, h->mg->launch);
+    mgSettings(c, *h->mg);
     return IFL_OK;
 }
 

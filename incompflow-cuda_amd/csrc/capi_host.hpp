@@ -45,8 +45,9 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Device levels of the multigrid p' solve of one handle (IFL_PSOLVE_MG), made when first needed.
 struct MgHost {
-    char *arena = nullptr;        // levels 1.. (x, b, C, E, N, outlet term) and the level table
-    ifl::MgLaunch launch{};       // launch.dev points into the arena
+    char *arena = nullptr;        // levels 1.. (x, b, C, E, N(, T), outlet term) and the 2D level table
+    ifl::MgLaunch launch{};       // 2D: launch.dev points into the arena
+    ifl::MgLaunch3 launch3{};     // 3D
 };
 
 struct ifl_solver {
@@ -85,7 +86,8 @@ struct Plan {
     ifl::Lc L;
     const ifl_config *cfg;   // sweep counts, smoother, refinement, sweep block (shared by a batch)
     int dim;
-    const ifl::MgLaunch *mg = nullptr;   // multigrid levels of the p' solve (one case)
+    const ifl::MgLaunch *mg = nullptr;   // multigrid levels of the p' solve (one case, 2D)
+    const ifl::MgLaunch3 *mg3 = nullptr; // the same, 3D
 };
 
 int sweepBlockOf(const ifl_config &c, int dim, int slot);
@@ -172,9 +174,9 @@ float *fieldPtr(ifl_solver *h, int id);
 // is left to the caller.  Sets P.partStride; fillParams must have run.
 size_t arenaLayout(ifl_solver *h, char *base, const bool *external);
 // Lay out the multigrid levels 1.. of a 2D case and their device table from base (null: size only).
-size_t mgLayout(const ifl_config &c, ifl::MgLaunch &G, char *base);
+size_t mgLayout(const ifl_config &c, MgHost &m, char *base);
 // Copy the multigrid settings of a configuration into its launch description.
-void mgSettings(const ifl_config &c, ifl::MgLaunch &G);
+void mgSettings(const ifl_config &c, MgHost &m);
 size_t fieldSize(const ifl_config &c, int id);
 
 // A copy of the whole device arena (fields, matrices, monitor, history) and the iteration count.

@@ -65,7 +65,7 @@ extern "C" int ifl_solve_noalloc(const ifl_config *cfg, void *stream, float *u, 
     const size_t arenaBytes = arenaLayout(&h, nullptr, external);
     MgHost mg{};
     const bool useMg = cfg->p_solver == IFL_PSOLVE_MG;
-    const size_t mgBytes = useMg ? mgLayout(*cfg, mg.launch, nullptr) : 0;
+    const size_t mgBytes = useMg ? mgLayout(*cfg, mg, nullptr) : 0;
     const size_t need = arenaBytes + mgBytes;
     if (need > (size_t)IFL_NOALLOC_SCRATCH_BYTES)
         return fail(IFL_ERR_ARG, "ifl_solve_noalloc: the case needs " + std::to_string(need)
@@ -101,8 +101,8 @@ extern "C" int ifl_solve_noalloc(const ifl_config *cfg, void *stream, float *u, 
     if (w) P.w = w;
     if (temp) P.temp = temp;
     if (useMg) {
-        mgLayout(*cfg, mg.launch, base + arenaBytes);
-        mgSettings(*cfg, mg.launch);
+        mgLayout(*cfg, mg, base + arenaBytes);
+        mgSettings(*cfg, mg);
         h.mg = &mg;
     }
     const size_t N = (size_t)P.N;
@@ -114,7 +114,7 @@ extern "C" int ifl_solve_noalloc(const ifl_config *cfg, void *stream, float *u, 
     IFL_HIP(hipMemcpyAsync(P.alt[0], P.u, N * 4, hipMemcpyDeviceToDevice, s));
     IFL_HIP(hipMemcpyAsync(P.alt[1], P.v, N * 4, hipMemcpyDeviceToDevice, s));
     IFL_HIP(hipMemcpyAsync(P.alt[2], P.w, N * 4, hipMemcpyDeviceToDevice, s));
-    if (useMg)
+    if (useMg && cfg->dim == 2)
         IFL_HIP(hipMemcpyAsync(base + arenaBytes, mg.launch.lv, sizeof(ifl::MgLevel) * ifl::kMgMaxLevels,
                                hipMemcpyHostToDevice, s));
     IFL_HIP(hipStreamSynchronize(s));

@@ -157,7 +157,7 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         best = chosen;
         if (bestRate == -1e300) bestRate = 0.0;
     }
-    if ((what & IFL_TUNE_PSOLVE) && h->P.dim == 2) {
+    if (what & IFL_TUNE_PSOLVE) {
         // the p' solver and its strength: the sweeps as configured (with their refinement cycles),
         // or one multigrid V-cycle with a coarse-correction factor of 1.0, 1.4 or 1.8
         Settings x = best;

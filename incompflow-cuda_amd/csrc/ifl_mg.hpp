@@ -52,6 +52,27 @@ struct MgLaunch {
     const MgLevel *dev = nullptr; // device copy of lv (the bottom kernel reads its levels there)
 };
 
+// A level l >= 1 of a 3D grid: the correction x (swept in place), the restricted residual b, the
+// centre (with the outlet term), the +x/+y/+z couplings and the outlet term.  Level 0 is the case's
+// p' system (x = P.pc, b = P.bp, the planes of P.pcoef); only its sizes are used.
+struct MgLevel3 {
+    int nx, ny, nz;
+    float *x, *b, *C, *E, *N, *T, *dg;
+};
+
+// A 3D p' solve as the host launches it (level-wide kernels, mg3d.hip).
+struct MgLaunch3 {
+    int nl = 0;
+    int cycles = 1, nu = 2, ncoarse = 4;
+    float over = 1.0f;
+    MgLevel3 lv[kMgMaxLevels] = {};
+};
+
+// Level sizes of an nx x ny x nz grid (dims[l][0..2]); returns the number of levels.
+int mg3_levels(int nx, int ny, int nz, int (*dims)[3]);
+// Build the coarse matrices, then run G.cycles V-cycles (one case, not batched).
+void launch_mg3_solve(const Lc &L, const MgLaunch3 &G);
+
 // Level sizes of an nx x ny grid (dims[l][0..1]); returns the number of levels.
 int mg_levels(int nx, int ny, int (*dims)[2]);
 // First level that the bottom kernel takes (-1: the coarsest level does not fit in its LDS).

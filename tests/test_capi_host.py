@@ -51,7 +51,6 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "tile", 9), "tile"),
     (lambda c: setattr(c, "p_solver", 5), "p_solver"),
     (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "p_refine", 1)), "p_refine"),
-    (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "dim", 3), setattr(c, "nz", 8)), "2D"),
     (lambda c: (setattr(c, "p_solver", _abi.PSOLVE_MG), setattr(c, "nx", 4), setattr(c, "ny", 60000)), "elongated"),
     (lambda c: setattr(c, "mg_over", 2.5), "mg_over"),
     (lambda c: setattr(c, "mg_sweeps", 4), "mg_sweeps"),
