@@ -165,12 +165,28 @@ def cavity3d_leg(args, incompflow, _abi):
                       ("pcorr_assemble", _abi.KERNEL_PCORR_ASSEMBLE), ("update", _abi.KERNEL_UPDATE)):
         us, _ = s.time_kernel(kid, reps=reps)
         kern[name] = {"us": us, "model_gbs": 4.0 * KERNEL_WORDS_3D[name] * cells / (us * 1e3)}
+    us, _ = s.time_kernel(_abi.KERNEL_PSOLVE, reps=5)
+    kern["p_solve_sweeps"] = {"us": us}
     s.close()
     log(f"[bench] cavity3d {ms:.3f} ms/outer iteration ({sweeps:.0f} sweeps), p' sweep "
         f"{kern['pcorr_sweep']['us']:.1f} us, u+v+w sweep {kern['momentum_sweep_uvw']['us']:.1f} us")
+    # the same case with the multigrid p' solve (one V-cycle, coarse-correction factor 1.4)
+    cm = c.copy()
+    cm.p_solver, cm.mg_cycles, cm.mg_over = _abi.PSOLVE_MG, 1, 1.4
+    s = incompflow.Solver(cm)
+    s.solve(3, history=False)
+    device_sync()
+    t0 = time.perf_counter()
+    s.solve(args.c3_steps, history=False)
+    device_sync()
+    ms_mg = 1000.0 * (time.perf_counter() - t0) / args.c3_steps
+    us_mg, _ = s.time_kernel(_abi.KERNEL_PSOLVE, reps=5)
+    s.close()
+    log(f"[bench] cavity3d multigrid p' solve: {ms_mg:.3f} ms/outer iteration, V-cycle {us_mg:.1f} us")
     out = {"workload": f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10/20 sweeps, alpha 0.7/0.3, "
                        "FP64 reductions", "cells": cells, "ms_per_step": ms, "sweeps_per_step": sweeps,
-           "cell_updates_per_s": cells * sweeps / (ms * 1e-3), "kernels": kern, "cpu_ms_per_step": None}
+           "cell_updates_per_s": cells * sweeps / (ms * 1e-3), "kernels": kern, "cpu_ms_per_step": None,
+           "multigrid": {"ms_per_step": ms_mg, "p_solve_us": us_mg, "mg_cycles": 1, "mg_over": 1.4}}
     # time to residual: measured by the opt-in --c3-converge run and kept under profiles/
     cfile = os.path.join(ROOT, "profiles", f"cavity3d_{n}_time_to_residual.json")
     if args.c3_converge:
@@ -178,10 +194,12 @@ def cavity3d_leg(args, incompflow, _abi):
         if args.c3_converge_out:
             with open(args.c3_converge_out, "w") as fh:
                 json.dump(out["time_to_residual"], fh, indent=1)
-    elif os.path.exists(cfile):
-        with open(cfile) as fh:
-            out["time_to_residual"] = json.load(fh)
-        out["time_to_residual"]["source"] = os.path.relpath(cfile, ROOT)
+    else:
+        for key, fn in (("time_to_residual", cfile), ("time_to_residual_mg", cfile.replace(".json", "_mg.json"))):
+            if os.path.exists(fn):
+                with open(fn) as fh:
+                    out[key] = json.load(fh)
+                out[key]["source"] = os.path.relpath(fn, ROOT)
     if not args.no_cpu and args.c3_cpu_iters > 0:
         from oracle import oracle as orc
         o = orc.Oracle(c)
@@ -492,6 +510,8 @@ def main():
                     help="p' sweeps per launch of the adjoint leg (small tiles: a smaller halo pays)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--p-refine", type=int, default=1)
+    ap.add_argument("--mg-over", type=float, default=0.0,
+                    help="> 0: the timed workload solves p' by one multigrid V-cycle with this mg_over")
     ap.add_argument("--workload", choices=["cavity2d", "cavity3d"], default="cavity2d")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--tune-iters", type=int, default=2000)
@@ -548,6 +568,8 @@ def main():
     cfg.sweep_block = args.sweep_block
     cfg.sweep_block_mom = args.sweep_block_mom
     cfg.tile = args.tile
+    if args.mg_over > 0:            # the multigrid p' solve instead of the sweeps (profiling runs)
+        cfg.p_solver, cfg.mg_cycles, cfg.mg_over, cfg.p_refine = _abi.PSOLVE_MG, 1, args.mg_over, 0
     cells = cfg.nx * cfg.ny * cfg.nz
 
     # ---- timed steps ----
