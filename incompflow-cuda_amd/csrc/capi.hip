@@ -337,7 +337,7 @@ size_t mgLayout(const ifl_config &c, MgHost &m, char *base) {
             L.ny = dims[l][1];
             L.nz = dims[l][2];
             if (l == 0) continue;
-            float **pl[7] = {&L.x, &L.b, &L.C, &L.E, &L.N, &L.T, &L.dg};
+            float **pl[8] = {&L.x, &L.xa, &L.b, &L.C, &L.E, &L.N, &L.T, &L.dg};
             const size_t b = align256((size_t)L.nx * L.ny * L.nz * 4);
             for (float **q : pl) {
                 *q = base ? (float *)(base + off) : nullptr;

@@ -52,12 +52,12 @@ struct MgLaunch {
     const MgLevel *dev = nullptr; // device copy of lv (the bottom kernel reads its levels there)
 };
 
-// A level l >= 1 of a 3D grid: the correction x (swept in place), the restricted residual b, the
+// A level l >= 1 of a 3D grid: the correction x (and its partner for ping-pong sweeps), the restricted residual b, the
 // centre (with the outlet term), the +x/+y/+z couplings and the outlet term.  Level 0 is the case's
 // p' system (x = P.pc, b = P.bp, the planes of P.pcoef); only its sizes are used.
 struct MgLevel3 {
     int nx, ny, nz;
-    float *x, *b, *C, *E, *N, *T, *dg;
+    float *x, *xa, *b, *C, *E, *N, *T, *dg;   // xa: the z-march sweeps' partner of x
 };
 
 // A 3D p' solve as the host launches it (level-wide kernels, mg3d.hip).

@@ -73,6 +73,27 @@ def test_mg3_p_solve_stage_bitwise(name, nu, cycles):
     g.close()
 
 
+@pytest.mark.parametrize("nu,cycles", [(2, 1), (3, 2)])
+def test_mg3_zmarch_levels_bitwise(nu, cycles):
+    """A grid large enough for the z-march smoother on level 0 (at least 512 columns of 32x8x32):
+    odd sides and ragged columns in x, y and z; pairs of sweeps march, an odd last sweep and the
+    prolongation it then needs run level-wide."""
+    cfg = _mg(_outlet_duct(161, 163, 167), cycles=cycles, nu=nu)
+    _log(f"[mg3 z-march] duct 161x163x167 nu={nu} cycles={cycles}")
+    g, o = Solver(cfg), Oracle(cfg)
+    st = random_state(cfg, seed=17)
+    load_state(g, st)
+    load_state(o, st)
+    for sid in (A.STAGE_MOM_ASSEMBLE, A.STAGE_RHIE_CHOW, A.STAGE_PCORR_ASSEMBLE):
+        o.stage(sid)
+        g.stage(sid)
+    og = o.stage(A.STAGE_SMOOTH_P)
+    gg = g.stage(A.STAGE_SMOOTH_P)
+    assert gg[0] == og[0] == cycles * 2 * nu
+    assert_bitwise(g.get(A.FIELD_PCORR, shaped=False), o.get(A.FIELD_PCORR, shaped=False), "p'")
+    g.close()
+
+
 @pytest.mark.parametrize("name,over", [("cavity37x29x23", 1.4), ("duct41x18x13", 1.0)])
 def test_mg3_outer_iterations_bitwise(name, over):
     """20 outer iterations from rest through the captured graph, bit for bit, with the same records."""
