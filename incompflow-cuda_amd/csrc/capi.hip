@@ -330,6 +330,7 @@ size_t mgLayout(const ifl_config &c, MgHost &m, char *base) {
         ifl::MgLaunch3 &G = m.launch3;
         int dims[ifl::kMgMaxLevels][3];
         G.nl = ifl::mg3_levels(c.nx, c.ny, c.nz, dims);
+        G.lb = ifl::mg3_bottom_level(G.nl, dims);
         size_t off = 0;
         for (int l = 0; l < G.nl; ++l) {
             ifl::MgLevel3 &L = G.lv[l];

@@ -5,6 +5,8 @@
 // it); on success the chosen settings are written into it.
 #include "capi_host.hpp"
 
+#include <vector>
+
 using namespace ifl_host;
 
 namespace {
@@ -38,7 +40,10 @@ void applySettings(ifl_solver *h, const Settings &x) {
     h->graphDirty = true;
 }
 
-// n outer iterations from the current state: device time, final scaled residual, divergence.
+// n outer iterations from the current state: device time, scaled residual, divergence.  The
+// scaled residual is the smallest over the last quarter of the run (at most the history ring): the
+// SIMPLE residual does not fall monotonically, and a single end point misranks settings whose
+// residuals oscillate by more than their trends differ over a trial.
 int trialRun(ifl_solver *h, int n, double &ms, double &scaled, bool &diverged) {
     int st = buildGraph(h);
     if (st != IFL_OK) return st;
@@ -53,9 +58,18 @@ int trialRun(ifl_solver *h, int n, double &ms, double &scaled, bool &diverged) {
     ms = f;
     ifl::CaseState s{};
     if ((st = readState(h, s)) != IFL_OK) return st;
+    const long long it0 = h->iter;
     h->iter = s.iter;
     scaled = s.lastScaled;
     diverged = s.done == 2 || !std::isfinite(scaled);
+    const long long ran = s.iter - it0, tail = std::min<long long>(ran / 4, kHistCap);
+    if (!diverged && tail > 1) {
+        std::vector<ifl_iter_record> rec((size_t)tail);
+        if ((st = copyHistory(h, h->stream, s.iter - tail, s.iter, rec.data())) != IFL_OK) return st;
+        IFL_HIP(hipStreamSynchronize(h->stream));
+        for (const ifl_iter_record &r : rec)
+            if (std::isfinite(r.scaled) && r.scaled < scaled) scaled = r.scaled;
+    }
     return IFL_OK;
 }
 

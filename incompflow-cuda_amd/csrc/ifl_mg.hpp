@@ -61,8 +61,11 @@ struct MgLevel3 {
 };
 
 // A 3D p' solve as the host launches it (level-wide kernels, mg3d.hip).
+constexpr int kMg3BottomFloats = 30 * 1024;   // LDS of the 3D bottom kernel (x, b and four matrix planes)
+
 struct MgLaunch3 {
     int nl = 0;
+    int lb = 0;                   // first level of the single-workgroup bottom kernel (nl: none)
     int cycles = 1, nu = 2, ncoarse = 4;
     float over = 1.0f;
     MgLevel3 lv[kMgMaxLevels] = {};
@@ -70,6 +73,8 @@ struct MgLaunch3 {
 
 // Level sizes of an nx x ny x nz grid (dims[l][0..2]); returns the number of levels.
 int mg3_levels(int nx, int ny, int nz, int (*dims)[3]);
+// First level (>= 1) from which all coarser levels fit the bottom kernel's LDS (nl: none).
+int mg3_bottom_level(int nl, const int (*dims)[3]);
 // Build the coarse matrices, then run G.cycles V-cycles (one case, not batched).
 void launch_mg3_solve(const Lc &L, const MgLaunch3 &G);
 

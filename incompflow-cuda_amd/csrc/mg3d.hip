@@ -555,6 +555,115 @@ __global__ __launch_bounds__(kBlock) void k_mg3_zmarch(CaseParams P, MgLevel3 Fl
     }
 }
 
+// ---------------- the coarsest levels in one workgroup ----------------
+// Levels lb .. nl-1 (at most kMg3BottomFloats of x, b and the matrix) of one V-cycle in LDS: down
+// (NU sweeps, restriction), the coarsest level's sweeps, up (prolongation, NU sweeps); the same
+// operations as the level-wide kernels, phase by phase between barriers.  Level lb's source comes
+// restricted from level lb-1 and its correction goes back for the prolongation there.
+constexpr int kMg3BottomThreads = 1024;
+
+struct MgBottom3 {
+    int lb, nl;
+    MgLevel3 lv[kMgMaxLevels];
+};
+
+template <int NU>
+__global__ __launch_bounds__(kMg3BottomThreads) void k_mg3_bottom(CaseParams P, MgBottom3 B, int ncoarse, float over) {
+    __shared__ float lds[kMg3BottomFloats];
+    if (caseDone(P)) return;
+    const int tid = threadIdx.x;
+    struct Lv {
+        int nx, ny, nz, n;
+        float *x, *b, *C, *E, *N, *T;
+    } v[kMgMaxLevels];
+    int off = 0;
+    for (int m = B.lb; m < B.nl; ++m) {
+        const MgLevel3 &F = B.lv[m];
+        Lv &q = v[m];
+        q.nx = F.nx;
+        q.ny = F.ny;
+        q.nz = F.nz;
+        q.n = F.nx * F.ny * F.nz;
+        float **pl[6] = {&q.x, &q.b, &q.C, &q.E, &q.N, &q.T};
+        for (float **a : pl) {
+            *a = lds + off;
+            off += q.n;
+        }
+        for (int g = tid; g < q.n; g += kMg3BottomThreads) {
+            q.x[g] = 0.0f;
+            q.b[g] = m == B.lb ? F.b[g] : 0.0f;
+            q.C[g] = F.C[g];
+            q.E[g] = F.E[g];
+            q.N[g] = F.N[g];
+            q.T[g] = F.T[g];
+        }
+    }
+    __syncthreads();
+    auto sweeps = [&](const Lv &F, int n) {
+        const int hx = (F.nx + 1) >> 1, pl = F.nx * F.ny;
+        for (int s = 0; s < n; ++s)
+            for (int col = 0; col < 2; ++col) {
+                for (int g = tid; g < hx * F.ny * F.nz; g += kMg3BottomThreads) {
+                    const int row = g / hx;
+                    const int j = row % F.ny, k = row / F.ny;
+                    const int i = 2 * (g - row * hx) + ((col + j + k) & 1);
+                    if (i >= F.nx) continue;
+                    const int c = i + F.nx * row;
+                    const float *x = F.x;
+                    const float xE = i < F.nx - 1 ? x[c + 1] : 0.0f, xW = i > 0 ? x[c - 1] : 0.0f;
+                    const float xN = j < F.ny - 1 ? x[c + F.nx] : 0.0f, xS = j > 0 ? x[c - F.nx] : 0.0f;
+                    const float aW = i > 0 ? F.E[c - 1] : 0.0f, aS = j > 0 ? F.N[c - F.nx] : 0.0f;
+                    float val = F.b[c] - F.E[c] * xE - aW * xW - F.N[c] * xN - aS * xS;
+                    const float xT = k < F.nz - 1 ? x[c + pl] : 0.0f, xB = k > 0 ? x[c - pl] : 0.0f;
+                    const float aB = k > 0 ? F.T[c - pl] : 0.0f;
+                    val -= F.T[c] * xT + aB * xB;
+                    F.x[c] = val / F.C[c];
+                }
+                __syncthreads();
+            }
+    };
+    auto res = [&](const Lv &F, int i, int j, int k) {
+        const int pl = F.nx * F.ny, c = i + F.nx * (j + F.ny * k);
+        const float *x = F.x;
+        double r = (double)F.b[c];
+        r -= (double)F.C[c] * (double)x[c];
+        r -= (double)F.E[c] * (double)(i < F.nx - 1 ? x[c + 1] : 0.0f);
+        r -= (double)(i > 0 ? F.E[c - 1] : 0.0f) * (double)(i > 0 ? x[c - 1] : 0.0f);
+        r -= (double)F.N[c] * (double)(j < F.ny - 1 ? x[c + F.nx] : 0.0f);
+        r -= (double)(j > 0 ? F.N[c - F.nx] : 0.0f) * (double)(j > 0 ? x[c - F.nx] : 0.0f);
+        r -= (double)F.T[c] * (double)(k < F.nz - 1 ? x[c + pl] : 0.0f);
+        r -= (double)(k > 0 ? F.T[c - pl] : 0.0f) * (double)(k > 0 ? x[c - pl] : 0.0f);
+        return r;
+    };
+    for (int m = B.lb; m + 1 < B.nl; ++m) {
+        const Lv &F = v[m], &Cl = v[m + 1];
+        sweeps(F, NU);
+        for (int g = tid; g < Cl.n; g += kMg3BottomThreads) {
+            const int I = g % Cl.nx, R = g / Cl.nx, J = R % Cl.ny, K = R / Cl.ny;
+            const int i1 = min(2 * I + 1, F.nx - 1), j1 = min(2 * J + 1, F.ny - 1), k1 = min(2 * K + 1, F.nz - 1);
+            double acc = 0.0;
+            for (int k = 2 * K; k <= k1; ++k)
+                for (int j = 2 * J; j <= j1; ++j)
+                    for (int i = 2 * I; i <= i1; ++i) acc += res(F, i, j, k);
+            Cl.b[g] = (float)acc;
+            Cl.x[g] = 0.0f;
+        }
+        __syncthreads();
+    }
+    sweeps(v[B.nl - 1], ncoarse);
+    for (int m = B.nl - 2; m >= B.lb; --m) {
+        const Lv &F = v[m], &Cl = v[m + 1];
+        for (int g = tid; g < F.n; g += kMg3BottomThreads) {
+            const int i = g % F.nx, r = g / F.nx, j = r % F.ny, k = r / F.ny;
+            F.x[g] = F.x[g] + over * Cl.x[i / 2 + Cl.nx * (j / 2 + Cl.ny * (k / 2))];
+        }
+        __syncthreads();
+        sweeps(F, NU);
+    }
+    const Lv &F = v[B.lb];
+    for (int g = tid; g < F.n; g += kMg3BottomThreads) B.lv[B.lb].x[g] = F.x[g];
+}
+
 inline unsigned blocksFor(long long n) { return (unsigned)((n + kMg3Block - 1) / kMg3Block); }
 
 // z-march columns of a level (the z-march pays once they fill the chip)
@@ -602,7 +711,21 @@ void sweepLevel(const Lc &L, const MgLaunch3 &G, int l, int n, int &rec, bool pr
         }
 }
 
+void launchBottom(const Lc &L, const MgLaunch3 &G) {
+    MgBottom3 B{};
+    B.lb = G.lb;
+    B.nl = G.nl;
+    for (int m = 0; m < G.nl; ++m) B.lv[m] = G.lv[m];
+    if (G.nu == 1) k_mg3_bottom<1><<<1, kMg3BottomThreads, 0, L.s>>>(*L.P, B, G.ncoarse, G.over);
+    else if (G.nu == 3) k_mg3_bottom<3><<<1, kMg3BottomThreads, 0, L.s>>>(*L.P, B, G.ncoarse, G.over);
+    else k_mg3_bottom<2><<<1, kMg3BottomThreads, 0, L.s>>>(*L.P, B, G.ncoarse, G.over);
+}
+
 void cycleLevel(const Lc &L, const MgLaunch3 &G, int l, int &rec) {
+    if (l == G.lb) {
+        launchBottom(L, G);
+        return;
+    }
     if (l + 1 == G.nl) {
         sweepLevel(L, G, l, G.ncoarse, rec, false);
         return;
@@ -633,6 +756,15 @@ int mg3_levels(int nx, int ny, int nz, int (*dims)[3]) {
         ++nl;
     }
     return nl;
+}
+
+int mg3_bottom_level(int nl, const int (*dims)[3]) {
+    for (int l = 1; l < nl; ++l) {
+        long long tot = 0;
+        for (int m = l; m < nl; ++m) tot += 6LL * dims[m][0] * dims[m][1] * dims[m][2];
+        if (tot <= kMg3BottomFloats) return l;
+    }
+    return nl;   // no level qualifies (level 0 alone, or a coarsest level too large): level-wide kernels
 }
 
 void launch_mg3_solve(const Lc &L, const MgLaunch3 &G) {
