@@ -651,7 +651,7 @@ def main():
     tname = {_abi.TILE_128x32: "128x32", _abi.TILE_64x16: "64x16"}.get(tile)
     tfile = os.path.join(ROOT, "profiles", f"k_rb_sweep2d_pc_{tname}_cavity1024_k{kb}_traffic.json")
     if cfg.dim == 3:
-        tfile = os.path.join(ROOT, "profiles", "k_rb_zmarch3d_pc_64x8x32_cavity256_traffic.json")
+        tfile = os.path.join(ROOT, "profiles", "k_rb_zmarch3d_pc_32x8x32_cavity256_traffic.json")
     if ((cfg.dim == 2 and n == 1024 and tname) or (cfg.dim == 3 and n == 256 and args.p_refine == 0)) \
             and os.path.exists(tfile):
         with open(tfile) as fh:
