@@ -97,6 +97,9 @@ def test_two_calls_in_a_row_the_second_smaller():
 
 def test_3d_energy_jacobi_and_a_side_stream():
     _compare("duct 24x12x10", duct3d(24, 12, 10), 12)
+    mg3 = duct3d(36, 20, 18)
+    mg3.p_solver, mg3.p_refine, mg3.mg_over = A.PSOLVE_MG, 0, 1.4
+    _compare("duct 36x20x18, multigrid", mg3, 10)
     _compare("cavity3d 20x17x33 on a side stream", cavity3d(20, 17, 33, re=300.0), 6, stream=torch.cuda.Stream())
     hot = cavity(48, 40)
     hot.energy = 1
