@@ -55,13 +55,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+try:   # imported before any device work: a first import between the warmup and the timed steps
+    import torch  # leaves the GPU idle for seconds, and the timed steps then start at idle clocks
+except ImportError:
+    torch = None
+
+
 def device_sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.synchronize()
 
 
 def sweep_bytes_per_cell(dim):
@@ -584,6 +586,7 @@ def main():
     import incompflow
     from incompflow import _abi
 
+    device_sync()          # torch's own device state, before the timed work (see the import above)
     n = args.n
     if args.workload == "cavity3d":
         if n == 1024:
