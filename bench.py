@@ -454,28 +454,11 @@ def solve_tracking(solver, targets, chunk, budget_s, extra_after, first=0):
     return hit, best, done, ms / 1000.0
 
 
-def proxy_choice(args, incompflow, _abi, n_proxy):
-    """The coarse-correction factor by time to the target residual on a coarser copy of the cavity
-    (n_proxy^2, same physics and settings), solved from rest with each candidate."""
-    times = {}
-    for over in (1.0, 1.4, 1.8):
-        c = _abi.cavity_config(n=n_proxy, p_refine=0, tol_residual=args.target, max_iter=200000)
-        c.sweep_block, c.sweep_block_mom, c.tile = args.sweep_block, args.sweep_block_mom, 0
-        c.p_solver, c.mg_cycles, c.mg_over = _abi.PSOLVE_MG, 1, over
-        s = incompflow.Solver(c)
-        r = s.solve(0, history=False)
-        s.close()
-        times[over] = {"iterations": r["iterations"], "converged": r["converged"], "device_s": r["elapsed_ms"] / 1000}
-    ok = {o: t for o, t in times.items() if t["converged"]}
-    best = min(ok, key=lambda o: ok[o]["device_s"]) if ok else 1.0
-    return best, times
-
-
 def mg_leg(args, incompflow, _abi):
-    """The 1024^2 cavity with the multigrid p' solve: its coarse-correction factor chosen by time to
-    the target on a 256^2 copy of the cavity (the library autotuner's choice, IFL_TUNE_PSOLVE from
-    iteration --tune-start, is reported beside it: its short trials rank the factors unreliably on this
-    slowly converging case), then a solve from rest with it, past the target to the FP32 floor."""
+    """The 1024^2 cavity with the p' solver the autotuner picks by time to the target on a 4x coarser
+    copy of the case (IFL_TUNE_PSOLVE_TARGET; the rate-based IFL_TUNE_PSOLVE choice, from iteration
+    --tune-start, is reported beside it: its short trials rank the factors unreliably on this slowly
+    converging case), then a solve from rest with it, past the target to the FP32 floor."""
     c = _abi.cavity_config(n=args.n, p_refine=0, tol_residual=1e-12, max_iter=10 ** 7)
     c.sweep_block, c.sweep_block_mom, c.tile = args.sweep_block, args.sweep_block_mom, args.tile
     t = incompflow.Solver(c)
@@ -484,11 +467,18 @@ def mg_leg(args, incompflow, _abi):
     t.close()
     log(f"[bench] p' solver chosen by the autotuner: {choice['p_solver']} (mg_over {choice['mg_over']}, "
         f"{choice['decades_per_s']:.2f} decades/s)")
-    n_proxy = max(64, args.n // 4)
-    over, proxy = proxy_choice(args, incompflow, _abi, n_proxy)
-    log(f"[bench] mg_over by time to {args.target:g} on the {n_proxy}^2 cavity: {over} "
-        + ", ".join(f"{o}: {v['iterations']} it {v['device_s']:.2f} s" for o, v in proxy.items()))
-    c.p_solver, c.mg_cycles, c.mg_over = _abi.PSOLVE_MG, 1, over
+    ct = c.copy()
+    ct.stop_rule, ct.tol_residual = _abi.STOP_SCALED, args.target
+    t = incompflow.Solver(ct)
+    t0 = time.perf_counter()
+    target = t.autotune(_abi.TUNE_PSOLVE_TARGET)
+    tune_s = time.perf_counter() - t0
+    t.close()
+    log(f"[bench] p' solver by time to {args.target:g} on the coarsened cavity: {target['p_solver']} "
+        f"(mg_over {target['mg_over']}, {target['trials']} trials, {tune_s:.1f} s)")
+    c.p_solver, c.mg_cycles, c.mg_over = target["p_solver"], max(1, target["mg_cycles"]), target["mg_over"]
+    if c.p_solver == _abi.PSOLVE_SWEEPS:
+        c.p_refine = 1
     s = incompflow.Solver(c)
     s.solve(args.warmup, history=False)
     r = s.solve(args.steps, history=False)
@@ -501,8 +491,9 @@ def mg_leg(args, incompflow, _abi):
                                             extra_after=args.floor_extra)
     s.close()
     h4 = hit[args.target]
-    out = {"p_solver": "multigrid", "mg_cycles": c.mg_cycles, "mg_over": c.mg_over,
-           "mg_over_from": f"time to {args.target:g} on the {n_proxy}^2 cavity", "proxy": {str(k): v for k, v in proxy.items()},
+    out = {"p_solver": "multigrid" if c.p_solver == _abi.PSOLVE_MG else "sweeps", "mg_cycles": c.mg_cycles,
+           "mg_over": c.mg_over, "chosen_by": f"ifl_autotune(IFL_TUNE_PSOLVE_TARGET): time to {args.target:g} on "
+                                             f"a 4x coarser copy of the cavity, {target['trials']} trials, {tune_s:.1f} s",
            "autotuner_choice": {"p_solver": "multigrid" if choice["p_solver"] == _abi.PSOLVE_MG else "sweeps",
                                 "mg_over": choice["mg_over"], "decades_per_s": choice["decades_per_s"],
                                 "trials": choice["trials"], "tune_start": args.tune_start, "trial_iters": args.tune_iters},

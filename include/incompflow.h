@@ -342,9 +342,17 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
  * chosen).  On an error return the configuration is the one the call started with.
  * trial_iters <= 0 selects 300. */
 enum { IFL_TUNE_LAUNCH = 1, IFL_TUNE_SWEEPS = 2,
-       IFL_TUNE_PSOLVE = 4 };   /* the p' solver and its strength (the configured sweeps, or one
+       IFL_TUNE_PSOLVE = 4,     /* the p' solver and its strength (the configured sweeps, or one
                                    multigrid V-cycle with mg_over 1.0, 1.4 or 1.8), by the drop of the
                                    scaled residual per second over trial_iters outer iterations */
+       IFL_TUNE_PSOLVE_TARGET = 8 };  /* the same candidates, by the device time a copy of the case
+                                   coarsened 4x per side (2x, or none, on small grids; same domain,
+                                   physics, boundary conditions and inlet profiles, averaged) takes from
+                                   rest to the handle's scaled-residual target (1e-4 unless the stop rule
+                                   is IFL_STOP_SCALED); the handle's own state is not touched.  The
+                                   SIMPLE residual oscillates too much for short trials to rank the
+                                   coarse-correction factors; the time to the target does.  Wins over
+                                   IFL_TUNE_PSOLVE when both are set */
 
 typedef struct ifl_tune_result {
     int32_t tile;              /* chosen tile (0 in 3D) */

@@ -5,6 +5,7 @@
 // it); on success the chosen settings are written into it.
 #include "capi_host.hpp"
 
+#include <algorithm>
 #include <vector>
 
 using namespace ifl_host;
@@ -91,8 +92,64 @@ struct ConfigGuard {
 
 }  // namespace
 
+// Device time from rest to `tol` of a copy of the case coarsened by f per side with the p' solver
+// of x (ms; < 0 when it did not converge within its cap).
+static int proxyTime(ifl_solver *h, const Settings &x, int f, double tol, double &ms) {
+    ifl_config c = h->cfg;
+    c.nx = std::max(4, h->cfg.nx / f);
+    c.ny = std::max(4, h->cfg.ny / f);
+    if (c.dim == 3) c.nz = std::max(4, h->cfg.nz / f);
+    c.stop_rule = IFL_STOP_SCALED;
+    c.tol_residual = tol;
+    c.num_outer_iter = 200000;
+    c.fail_if_not_converged = 0;
+    c.log_every = 0;
+    c.tile = 0;
+    c.sweep_block = x.K;
+    c.sweep_block_mom = x.Km;
+    for (int q = 0; q < 4; ++q) c.n_iter[q] = x.nIter[q];
+    c.p_solver = x.pSolver;
+    c.mg_cycles = x.mgCycles;
+    c.mg_over = x.mgOver;
+    c.p_refine = x.pRefine;
+    ifl_solver *p = nullptr;
+    int st = ifl_create(&c, &p);
+    if (st != IFL_OK) return st;
+    // inlet profiles: each proxy face value is the mean of the f (x f) fine values it covers
+    for (int face = 0; face < 2 * c.dim && st == IFL_OK; ++face) {
+        if (!h->prof[face]) continue;
+        const int a = face < 2 ? h->cfg.ny : h->cfg.nx, b = face < 4 ? h->cfg.nz : h->cfg.ny;
+        const int ca = face < 2 ? c.ny : c.nx, cb = face < 4 ? c.nz : c.ny;
+        std::vector<float> fine((size_t)a * b);
+        if (hipMemcpy(fine.data(), h->prof[face], fine.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            st = fail(IFL_ERR_DEVICE, "reading an inlet profile");
+            break;
+        }
+        std::vector<double> coarse((size_t)ca * cb, 0.0);
+        for (int jb = 0; jb < cb; ++jb)
+            for (int ja = 0; ja < ca; ++ja) {
+                double sum = 0.0;
+                int cnt = 0;
+                for (int qb = jb * b / cb; qb < (jb + 1) * b / cb; ++qb)
+                    for (int qa = ja * a / ca; qa < (ja + 1) * a / ca; ++qa) {
+                        sum += fine[(size_t)qa + (size_t)a * qb];
+                        ++cnt;
+                    }
+                coarse[(size_t)ja + (size_t)ca * jb] = cnt ? sum / cnt : 0.0;
+            }
+        st = ifl_set_face_profile(p, face, coarse.data(), coarse.size());
+    }
+    ifl_report rep{};
+    if (st == IFL_OK) st = ifl_solve(p, 0, &rep, nullptr, 0);
+    ifl_destroy(p);
+    if (st == IFL_ERR_DIVERGED) st = IFL_OK;
+    ms = (st == IFL_OK && rep.converged) ? rep.elapsed_ms : -1.0;
+    return st;
+}
+
 extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_result *out) {
-    if (!h || !out || what == 0 || (what & ~(IFL_TUNE_LAUNCH | IFL_TUNE_SWEEPS | IFL_TUNE_PSOLVE)))
+    if (!h || !out || what == 0 ||
+        (what & ~(IFL_TUNE_LAUNCH | IFL_TUNE_SWEEPS | IFL_TUNE_PSOLVE | IFL_TUNE_PSOLVE_TARGET)))
         return fail(IFL_ERR_ARG, "bad argument");
     std::memset(out, 0, sizeof(*out));
     const int trial = trial_iters > 0 ? trial_iters : 300;
@@ -104,9 +161,11 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
     double ms = 0, sc = 0;
     bool div = false;
     applySettings(h, best);
-    if (h->iter < 20 && (st = trialRun(h, (int)(20 - h->iter), ms, sc, div)) != IFL_OK) return st;
+    // trials on the handle itself (all modes but IFL_TUNE_PSOLVE_TARGET) start from one snapshot
+    const bool onHandle = (what & (IFL_TUNE_LAUNCH | IFL_TUNE_SWEEPS | IFL_TUNE_PSOLVE)) != 0;
+    if (onHandle && h->iter < 20 && (st = trialRun(h, (int)(20 - h->iter), ms, sc, div)) != IFL_OK) return st;
     Snapshot snap;
-    if ((st = snapTake(h, snap)) != IFL_OK) return st;
+    if (onHandle && (st = snapTake(h, snap)) != IFL_OK) return st;
     ifl::CaseState s0{};
     if ((st = readState(h, s0)) != IFL_OK) return st;
     double bestUs = 0.0, bestRate = 0.0;
@@ -171,7 +230,35 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         best = chosen;
         if (bestRate == -1e300) bestRate = 0.0;
     }
-    if (what & IFL_TUNE_PSOLVE) {
+    if (what & IFL_TUNE_PSOLVE_TARGET) {
+        // the candidates of IFL_TUNE_PSOLVE, by time to the target on the coarsened copy
+        const int minSide = std::min(h->cfg.nx, std::min(h->cfg.ny, h->P.dim == 3 ? h->cfg.nz : h->cfg.ny));
+        const int f = minSide >= (h->P.dim == 3 ? 96 : 128) ? 4 : minSide >= 64 ? 2 : 1;
+        const double tol = c0.stop_rule == IFL_STOP_SCALED && c0.tol_residual > 0 ? c0.tol_residual : 1e-4;
+        Settings x = best;
+        x.pSolver = IFL_PSOLVE_SWEEPS;
+        if (x.pRefine == 0) x.pRefine = c0.p_refine;
+        double ms0;
+        if ((st = proxyTime(h, x, f, tol, ms0)) != IFL_OK) return st;
+        out->trials++;
+        Settings chosen = x;
+        for (double over : {1.0, 1.4, 1.8}) {
+            Settings y = best;
+            y.pSolver = IFL_PSOLVE_MG;
+            y.pRefine = 0;
+            y.mgCycles = 1;
+            y.mgOver = over;
+            double ms;
+            if ((st = proxyTime(h, y, f, tol, ms)) != IFL_OK) return st;
+            out->trials++;
+            if (ms > 0.0 && (ms0 < 0.0 || ms < ms0)) {
+                ms0 = ms;
+                chosen = y;
+            }
+        }
+        if (ms0 > 0.0) best = chosen;   // nothing converged: keep the handle's p' solver
+        bestRate = 0.0;
+    } else if (what & IFL_TUNE_PSOLVE) {
         // the p' solver and its strength: the sweeps as configured (with their refinement cycles),
         // or one multigrid V-cycle with a coarse-correction factor of 1.0, 1.4 or 1.8
         Settings x = best;
@@ -196,7 +283,8 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
         best = chosen;
         bestRate = r0 == -1e300 ? 0.0 : r0;
     }
-    if (!(what & IFL_TUNE_LAUNCH) || bestUs == 0.0 || (what & (IFL_TUNE_SWEEPS | IFL_TUNE_PSOLVE)))
+    if (onHandle && (!(what & IFL_TUNE_LAUNCH) || bestUs == 0.0 ||
+                     (what & (IFL_TUNE_SWEEPS | IFL_TUNE_PSOLVE | IFL_TUNE_PSOLVE_TARGET))))
         if ((st = timeIter(best, bestUs)) != IFL_OK) return st;
     best.stopRule = c0.stop_rule;
     applySettings(h, best);

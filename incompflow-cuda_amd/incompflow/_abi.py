@@ -150,7 +150,7 @@ class Report(C.Structure):
                 ("sweeps_total", C.c_double)]
 
 
-TUNE_LAUNCH, TUNE_SWEEPS, TUNE_PSOLVE = 1, 2, 4
+TUNE_LAUNCH, TUNE_SWEEPS, TUNE_PSOLVE, TUNE_PSOLVE_TARGET = 1, 2, 4, 8
 OBJ_PRESSURE_DROP, OBJ_LINEAR = 0, 1
 
 

@@ -190,3 +190,38 @@ def test_fail_if_not_converged_status():
     r = g.solve(30, history=False)
     assert r["status"] == 0 and r["converged"]
     g.close()
+
+
+@pytest.mark.parametrize("case", ["cavity128", "channel256x64_profile", "cavity3d_40"])
+def test_autotune_by_time_to_target_leaves_the_state(case):
+    """IFL_TUNE_PSOLVE_TARGET: four proxy solves (the sweeps, three coarse-correction factors) on a
+    coarsened copy of the case, inlet profile averaged onto it; the handle's fields and iteration
+    count are untouched, the choice is applied, and the solve then continues with it."""
+    if case == "cavity128":
+        cfg = A.cavity_config(n=128, p_refine=1, tol_residual=1e-4, max_iter=6000)
+        prof = None
+    elif case == "channel256x64_profile":
+        cfg = A.adjoint_channel_config(256, 64)
+        cfg.tol_residual = 1e-4
+        prof = np.array(A.adjoint_inlet_profile(cfg))
+    else:
+        cfg = A.cavity3d_config(n=40, tol_residual=1e-4, max_iter=3000)
+        cfg.p_refine = 0
+        prof = None
+    _log(f"[autotune target] {case}")
+    g = Solver(cfg)
+    if prof is not None:
+        g.set_face_profile(A.WEST, prof)
+    g.solve(10, history=False)
+    fields = (A.FIELD_U, A.FIELD_V, A.FIELD_P, A.FIELD_UF, A.FIELD_VF)
+    before = {f: g.get(f, shaped=False) for f in fields}
+    r = g.autotune(A.TUNE_PSOLVE_TARGET)
+    assert r["trials"] == 4
+    assert r["p_solver"] in (A.PSOLVE_SWEEPS, A.PSOLVE_MG) and g.cfg.p_solver == r["p_solver"]
+    for f in fields:
+        assert_bitwise(g.get(f, shaped=False), before[f], f"untouched {A.FIELD_NAMES[f]}")
+    rr = g.solve(5, history=True)
+    assert rr["iterations"] == 5 and rr["status"] == 0
+    if r["p_solver"] == A.PSOLVE_MG:
+        assert all(x.sweeps[3] == 4 for x in rr["history"])
+    g.close()
