@@ -29,6 +29,8 @@ batch      : the seeded 64-case uneven batch (2D cavities and channels, 64..512 
              solve bit for bit, so both runs execute the same iterations.
 energy     : the same cavity with heated walls and the temperature equation on: ms per outer
              iteration with and without it, and the temperature sweep launch (unless --no-energy).
+channel_3000: the reference's default 512x128 channel, 3000 outer iterations on the GPU and on the FP32
+             oracle (host cores), timed directly.
 adjoint    : the 1024x256 channel with a noisy parabolic inlet solved to the scaled residual
              --adjoint-forward-target, then the discrete adjoint of the pressure drop until dJ/dmu and
              the profile gradient settle; wall times of both, and central differences of FP32 device
@@ -213,6 +215,45 @@ def cavity3d_leg(args, incompflow, _abi):
         out["gpu_vs_cpu_per_step"] = out["cpu_ms_per_step"] / ms
         o.close()
         log(f"[bench] cavity3d cpu oracle {out['cpu_ms_per_step']:.1f} ms/iteration")
+    return out
+
+
+def channel_leg(args, incompflow, _abi):
+    """The reference's default case (512x128 channel, west inlet u = 0.001, east outlet, alpha 0.7/0.3,
+    10/10/20 sweeps): exactly 3000 outer iterations from rest, the GPU against the FP32 oracle on the
+    host cores doing the same 3000 (the two produce the same fields bit for bit: tests/test_gpu_parity.py)."""
+    c = _abi.channel_config(512, 128, 3000)
+    s = incompflow.Solver(c)
+    tune = None
+    if not args.no_tune:
+        # launch settings by the autotuner (a small grid: sweeps per launch set the launch count);
+        # the oracle runs the same settings, so both keep producing the same fields
+        tune = s.autotune(_abi.TUNE_LAUNCH)
+        c.tile, c.sweep_block, c.sweep_block_mom = s.cfg.tile, s.cfg.sweep_block, s.cfg.sweep_block_mom
+        s.close()
+        s = incompflow.Solver(c)
+    device_sync()
+    t0 = time.perf_counter()
+    r = s.solve(3000, history=False)
+    device_sync()
+    tg = time.perf_counter() - t0
+    s.close()
+    out = {"workload": "2D channel 512x128 (the reference's config.h), 3000 outer iterations, RB-GS 10/10/20, "
+                       "alpha 0.7/0.3", "iterations": r["iterations"], "gpu_s": tg, "gpu_ms_per_step": 1000 * tg / 3000,
+           "gpu_device_ms_per_step": r["elapsed_ms"] / 3000, "cpu_s": None,
+           "launch_settings": {"tile": c.tile, "sweep_block": c.sweep_block, "sweep_block_mom": c.sweep_block_mom,
+                               "from": "the autotuner (IFL_TUNE_LAUNCH)" if tune else "defaults"}}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        o = orc.Oracle(c)
+        t0 = time.perf_counter()
+        ro = o.solve(3000, history=False)
+        tc = time.perf_counter() - t0
+        o.close()
+        out.update({"cpu_s": tc, "cpu_iterations": ro["iterations"], "cpu_threads": orc.get_threads(),
+                    "gpu_vs_cpu": tc / tg})
+    log(f"[bench] channel 512x128, 3000 iterations: GPU {tg:.2f} s"
+        + (f", CPU oracle {out['cpu_s']:.1f} s ({out['gpu_vs_cpu']:.0f}x)" if out["cpu_s"] else ""))
     return out
 
 
@@ -548,6 +589,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device 5 times (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--no-cavity3d", action="store_true")
+    ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--c3-cpu-iters", type=int, default=2)
@@ -737,6 +779,9 @@ def main():
     adjoint = None
     if not args.no_adjoint and cfg.dim == 2:
         adjoint = adjoint_leg(args, incompflow, _abi)
+    channel = None
+    if not args.no_channel and cfg.dim == 2:
+        channel = channel_leg(args, incompflow, _abi)
     cavity3d = None
     if not args.no_cavity3d and cfg.dim == 2:
         cavity3d = cavity3d_leg(args, incompflow, _abi)
@@ -801,6 +846,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "adjoint": adjoint,
+        "channel_3000": channel,
         "cavity3d": cavity3d,
         "energy": energy,
     }
