@@ -533,16 +533,58 @@ __device__ __forceinline__ bool launchRuns(const CaseParams &P, const Blk &b, in
     return !stopped;
 }
 
-// Close the smoother calls listed in fin (workgroup 0 of each case only): publish the state after
-// each call's last launch and add its sweeps to the outer iteration's count.
+// Close the smoother calls listed in fin: publish the state after each call's last launch (the
+// decision of launchRuns for a launch past the last) and add its sweeps to the outer iteration's
+// count.  Each call is closed by one workgroup of the case, which sums the
+// partials of all the launch's sweeps in one workgroup reduction (the same per-sweep totals as
+// one reduction per sweep), so the closing adds one short reduction to one workgroup's path.  The
+// workgroup is chosen by the call's equation (q modulo the workgroup count), so calls that add to
+// the same sweep count (p' and its refinement correction) are closed by one workgroup, in order.
 template <bool BATCH>
 __device__ __forceinline__ void runFinish(const CaseParams &P, const BatchMap &M, const Blk &b, const Finish &fin) {
-    if (fin.n == 0 || b.flat != 0) return;
+    if (fin.n == 0 || b.flat >= 5) return;
+    int np = fin.nprev;
+    if constexpr (BATCH) np = M.start[fin.kind][(int)(&P - M.P) + 1] - M.start[fin.kind][(int)(&P - M.P)];
     for (int i = 0; i < fin.n; ++i) {
-        int np = fin.nprev;
-        if constexpr (BATCH) np = M.start[fin.kind][(int)(&P - M.P) + 1] - M.start[fin.kind][(int)(&P - M.P)];
-        launchRuns(P, b, fin.slot[i], fin.nL[i], fin.kLast[i], np, P.tol[fin.q[i]]);
-        if (threadIdx.x == 0) P.st->sweepsIter[fin.q[i]] += P.st->sm[fin.slot[i]].sweeps;
+        if (fin.q[i] % b.n != b.flat) continue;
+        const int slot = fin.slot[i], L = fin.nL[i], kPrev = fin.kLast[i];
+        const double tol = P.tol[fin.q[i]];
+        SmoothState &c = P.st->sm[slot];
+        const int pp = L & 1;
+        const bool stoppedBefore = (L >= 2) && c.stopped[pp];
+        double mx = (L >= 2) ? c.maxNorm[pp] : -1e20;
+        const int before = (L >= 2) ? c.swAt[pp] : 0;
+        bool stopped = stoppedBefore;
+        double T = 0.0;
+        int ran = before;
+        if (L >= 1 && !stoppedBefore) {
+            double acc[kMaxBlock], tot[kMaxBlock];
+#pragma unroll
+            for (int kk = 0; kk < kMaxBlock; ++kk) acc[kk] = 0.0;
+            for (int r = threadIdx.x; r < np; r += kBlock)
+#pragma unroll
+                for (int kk = 0; kk < kMaxBlock; ++kk)
+                    if (kk < kPrev) acc[kk] += sweepPart(P, slot, L - 1, kk)[r];
+            blockSum<kMaxBlock>(acc, tot);
+#pragma unroll
+            for (int kk = 0; kk < kMaxBlock; ++kk)
+                if (kk < kPrev) {
+                    T = tot[kk];
+                    const double norm = sqrt(T / (double)P.N);
+                    mx = fmax(norm, mx);
+                    if (norm / (mx + 1e-20) < tol) stopped = true;
+                }
+            ran = before + kPrev;
+        }
+        if (threadIdx.x == 0 && L >= 1) {
+            const int q = (L - 1) & 1;
+            c.stopped[q] = stopped ? 1 : 0;
+            c.maxNorm[q] = mx;
+            c.swAt[q] = ran;
+            if (!stoppedBefore) c.lastSum = T;
+            c.sweeps = ran;
+        }
+        if (threadIdx.x == 0) P.st->sweepsIter[fin.q[i]] += L >= 1 ? ran : c.sweeps;
     }
 }
 
