@@ -499,6 +499,11 @@ static dim3 adjGrid(const CaseParams &P) {
     return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
 }
 
+int adj_blocks(const CaseParams &P) {
+    const dim3 g = adjGrid(P);
+    return (int)(g.x * g.y);
+}
+
 #define IFL_ADJ_LAUNCH(L, KERNEL, ...)                                                                   \
     do {                                                                                                \
         const CaseParams &P_ = *(L).P;                                                                  \

@@ -13,7 +13,7 @@ int adjAlloc(ifl_solver *h) {
     h->adj = a;
     const ifl::CaseParams &P = h->P;
     const size_t N = (size_t)P.N, nx = P.nx, ny = P.ny, nz = P.nz;
-    const size_t nblk = (size_t)ifl::cell_blocks(P);
+    const size_t nblk = (size_t)ifl::adj_blocks(P);
     ifl::AdjBufs &B = a->B;
     struct Part { void **dst; size_t bytes; };
     std::vector<Part> parts;

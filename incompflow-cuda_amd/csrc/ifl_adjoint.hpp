@@ -56,5 +56,7 @@ void launch_adj_rc(const Lc &L, const AdjBufs &A);
 void launch_adj_coef(const Lc &L, const AdjBufs &A);
 void launch_adj_face(const Lc &L, const AdjBufs &A);
 void launch_adj_reduce(const Lc &L, const AdjBufs &A);
+// Workgroups of the adjoint's cell kernels (their partial-sum records).
+int adj_blocks(const CaseParams &P);
 
 }  // namespace ifl

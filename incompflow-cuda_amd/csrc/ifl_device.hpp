@@ -24,7 +24,7 @@ constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 6;              // smoother call slots: u, v, w, p', p' refinement, temperature
 constexpr int kSlotT = 5;              // the temperature slot (its equation index is 4)
-constexpr int kRows = 4;               // rows per thread in the cell kernels (64 x 16 cells per workgroup)
+constexpr int kRows = 4;               // most rows per thread in the cell kernels (64 x 16 cells per workgroup)
 
 enum { E = 0, W = 1, N = 2, S = 3, T = 4, B = 5 };                // constants.h:7-12
 enum { sC = 0, sE = 1, sW = 2, sN = 3, sS = 4, sT = 5, sB = 6 };  // constants.h:14-20
@@ -61,6 +61,7 @@ struct CaseState {
 struct CaseParams {
     int dim, nx, ny, nz;
     int N;                   // cells (< 2^28, checked on the host)
+    int cellRows;            // rows per thread of the cell kernels (1 or kRows): 64 x 4 cellRows cells per workgroup
     float dx, dy, dz, Ax, Ay, Az, rho, mu;
     int bct[6];
     float bcv[6][3];

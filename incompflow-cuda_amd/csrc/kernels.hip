@@ -20,11 +20,12 @@ template <class F>
 __device__ __forceinline__ void forCells(const CaseParams &P, const Blk &b, F &&f) {
     const int rows = P.ny * P.nz;
     const int i = b.x * 64 + (threadIdx.x & 63);
-    const int r0 = b.y * (4 * kRows) + (threadIdx.x >> 6);
+    const int R = P.cellRows;
+    const int r0 = b.y * (4 * R) + (threadIdx.x >> 6);
 #pragma unroll
     for (int q = 0; q < kRows; ++q) {
         const int r = r0 + 4 * q;
-        if (i < P.nx && r < rows) {
+        if (q < R && i < P.nx && r < rows) {
             const int k = r / P.ny;
             const int j = r - k * P.ny;
             f(i, j, k, i + P.nx * r);
@@ -721,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_cells(CaseParams P0, BatchMap
 
 // ---------------- host-side launchers ----------------
 dim3 cellGrid(const CaseParams &P) {
-    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
+    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * P.cellRows - 1) / (4 * P.cellRows)));
 }
 int cell_blocks(const CaseParams &P) {
     dim3 g = cellGrid(P);
