@@ -265,7 +265,17 @@ def cavity3d_converge(args, incompflow, _abi, n):
     c.p_refine = 0
     c.stop_rule = _abi.STOP_NONE
     c.num_outer_iter = 10 ** 7
-    if args.c3_mg_over > 0:
+    chosen = None
+    if args.c3_mg_over < 0:
+        # the p' solver by the autotuner's time to 1e-4 on a 4x coarser copy of the cavity
+        ct = c.copy()
+        ct.stop_rule, ct.tol_residual = _abi.STOP_SCALED, 1e-4
+        t = incompflow.Solver(ct)
+        chosen = t.autotune(_abi.TUNE_PSOLVE_TARGET)
+        t.close()
+        c.p_solver, c.mg_cycles, c.mg_over = chosen["p_solver"], max(1, chosen["mg_cycles"]), chosen["mg_over"]
+        log(f"[bench] cavity3d p' solver by the autotuner: {chosen['p_solver']} (mg_over {chosen['mg_over']})")
+    elif args.c3_mg_over > 0:
         c.p_solver, c.mg_cycles, c.mg_over = _abi.PSOLVE_MG, 1, args.c3_mg_over
     targets = [1e-3, 1e-4, 1e-5]
     log(f"[bench] cavity3d {n}^3 from rest towards {targets} (budget {args.c3_converge_budget_s:.0f} s)")
@@ -273,13 +283,14 @@ def cavity3d_converge(args, incompflow, _abi, n):
     hit, best, done, dev_s = solve_tracking(s, targets, 2000, args.c3_converge_budget_s, extra_after=10 ** 7,
                                             first=None)
     s.close()
-    psolve = (f"one multigrid V-cycle for p' (mg_over {args.c3_mg_over:g})" if args.c3_mg_over > 0
-              else "20 p' sweeps")
+    psolve = (f"one multigrid V-cycle for p' (mg_over {c.mg_over:g}"
+              + (", chosen by ifl_autotune(IFL_TUNE_PSOLVE_TARGET))" if chosen else ")")
+              if c.p_solver == _abi.PSOLVE_MG else "20 p' sweeps")
     return {"workload": f"3D lid-driven cavity {n}^3, Re=400, FP32 RB-GS 10/10/10 momentum sweeps, {psolve}, "
                         "alpha 0.7/0.3",
             "command": f"python3 bench.py --c3-converge --c3-converge-budget-s {args.c3_converge_budget_s:g} "
-                       + (f"--c3-mg-over {args.c3_mg_over:g} " if args.c3_mg_over > 0 else "")
-                       + "--no-converge --no-mg --no-cpu --no-batch --no-adjoint --no-energy --steps 5 --warmup 2",
+                       + (f"--c3-mg-over {args.c3_mg_over:g} " if args.c3_mg_over != 0 else "")
+                       + "--no-converge --no-mg --no-cpu --no-batch --no-adjoint --no-energy --no-channel --steps 5 --warmup 2",
             "targets": {f"{t:g}": ({"iterations": hit[t][0], "device_s": hit[t][1]} if hit[t] else None)
                         for t in targets},
             "min_scaled": best, "iterations_run": done, "device_s": dev_s}
@@ -598,7 +609,8 @@ def main():
     ap.add_argument("--c3-converge-budget-s", type=float, default=600.0)
     ap.add_argument("--c3-converge-out", default=None, help="write the --c3-converge result to this JSON file")
     ap.add_argument("--c3-mg-over", type=float, default=0.0,
-                    help="> 0: the --c3-converge run solves p' by one multigrid V-cycle with this mg_over")
+                    help="> 0: the --c3-converge run solves p' by one multigrid V-cycle with this mg_over; "
+                         "< 0: with the p' solver the autotuner picks by time to 1e-4 on a coarser copy")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--adjoint-forward-target", type=float, default=1e-5)
