@@ -15,6 +15,8 @@ roofline   : the dominant kernel, the fused red-black p' sweep (k_rb_sweep2d, 5 
 time_to_residual_s: a separate solve from rest to the scaled residual --target (1e-4).  A 1e-5 scaled
              residual is below the FP32 floor of this grid (time_to_1e-5: reached false, with the
              floor measured by continuing the multigrid solve past 1e-4)
+refine_ab  : the same solve from rest without the FP64-defect refinement cycle (p_refine 0): its
+             iterations and time to the target and its time per outer iteration (unless --no-refine-ab)
 p_solver_mg: the same cavity with the multigrid p' solve, its strength chosen by the autotuner
              (IFL_TUNE_PSOLVE), solved from rest to the same residual
 fp64_error : a 256x256 cavity (and the 256x64 adjoint channel) converged on the GPU in FP32 against
@@ -95,7 +97,9 @@ def batch_leg(args, incompflow, _abi):
     conv = sum(1 for r in reps if r["converged"])
     log(f"[bench] batch {tb:.2f} s, {conv}/{len(cfgs)} converged, max {max(its)} iterations")
     seq = 0.0
-    if args.batch_sequential == "full":
+    if args.batch_sequential == "none":
+        kind = "none: not run"
+    elif args.batch_sequential == "full":
         for i, c in enumerate(cfgs):
             s = incompflow.Solver(c)
             t0 = time.perf_counter()
@@ -122,14 +126,15 @@ def batch_leg(args, incompflow, _abi):
             s.close()
         kind = (f"estimated: each case's single-solve wall time per iteration "
                 f"({args.batch_probe_iters} iterations) x its iteration count")
-    log(f"[bench] sequential {seq:.2f} s ({kind.split(':')[0]}), speedup {seq / tb:.2f}x")
+    if args.batch_sequential != "none":
+        log(f"[bench] sequential {seq:.2f} s ({kind.split(':')[0]}), speedup {seq / tb:.2f}x")
     stalled = [{"case": i, "nx": cfgs[i].nx, "ny": cfgs[i].ny, "re": round(1.0 / cfgs[i].viscosity, 1),
                 "kind": "cavity" if cfgs[i].bc_type[_abi.WEST] == _abi.WALL else "channel",
                 "final_scaled": reps[i]["final_scaled"]} for i in range(len(cfgs)) if not reps[i]["converged"]]
     return {"cases": len(cfgs), "cells": cells, "residual_target": args.batch_target,
             "max_iter": args.batch_max_iter, "converged": conv, "not_converged": stalled,
             "iterations_max": max(its), "iterations_sum": sum(its), "batch_s": tb, "sequential_s": seq,
-            "sequential_kind": kind, "speedup": seq / tb, "launch_settings": tune,
+            "sequential_kind": kind, "speedup": seq / tb if seq else None, "launch_settings": tune,
             "cell_updates_per_s": sum(r["sweeps_total"] for r in reps) / tb}
 
 
@@ -479,6 +484,29 @@ def host_cpu():
             "cpus_allowed": allowed}
 
 
+def solve_from_rest(incompflow, cfg, budget_s):
+    """Solve cfg from rest with its own stop rule in chunks of 5000 outer iterations until it
+    converges, fails or the wall budget ends; return (wall seconds, iterations, converged)."""
+    c = incompflow.Solver(cfg)
+    device_sync()
+    t0 = time.perf_counter()
+    total = 0
+    converged = False
+    while True:
+        rr = c.solve(5000, history=False)
+        total += rr["iterations"]
+        log(f"[bench]   {total} iterations, scaled residual {rr['final_scaled']:.3e}, {time.perf_counter() - t0:.1f} s")
+        if rr["converged"] or rr["status"] != 0:
+            converged = rr["converged"]
+            break
+        if time.perf_counter() - t0 > budget_s or total >= cfg.num_outer_iter:
+            break
+    device_sync()
+    wall = time.perf_counter() - t0
+    c.close()
+    return wall, total, converged
+
+
 def solve_tracking(solver, targets, chunk, budget_s, extra_after, first=0):
     """Run a solver (whose own stop rule never triggers) in chunks with its history; return the
     iteration and device time at which each target scaled residual was first reached, the minimum
@@ -566,6 +594,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-refine-ab", action="store_true",
+                    help="skip the solve from rest without the p' refinement cycle")
     ap.add_argument("--target", type=float, default=1e-4)
     ap.add_argument("--converge-budget-s", type=float, default=120.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -594,7 +624,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-target", type=float, default=1e-4)
     ap.add_argument("--batch-max-iter", type=int, default=20000)
-    ap.add_argument("--batch-sequential", choices=["estimate", "full"], default="full")
+    ap.add_argument("--batch-sequential", choices=["estimate", "full", "none"], default="full")
     ap.add_argument("--batch-probe-iters", type=int, default=300)
     ap.add_argument("--batch-trial-iters", type=int, default=150)
     ap.add_argument("--copy-probe", action="store_true",
@@ -683,7 +713,8 @@ def main():
 
     # ---- roofline of the dominant kernel (one launch of the fused red-black p' sweeps) ----
     us, per = s.time_kernel(_abi.KERNEL_SWEEP, reps=args.kernel_reps)
-    kb = max(1, args.sweep_block) if cfg.dim == 2 else 1   # sweeps per launch of the p' kernel
+    # sweeps per launch of the p' kernel (3D: one or two)
+    kb = max(1, args.sweep_block) if cfg.dim == 2 else min(max(1, args.sweep_block), 2)
     # compulsory bytes of one launch: p' in/out, b' and the coefficient planes towards +x, +y (+z)
     # (those towards -x, -y, -z are the neighbours'), once for all kb sweeps of the launch
     alg_bytes = (20.0 * cells if cfg.dim == 2 else 24.0 * cells) / per
@@ -700,7 +731,7 @@ def main():
     tfile = os.path.join(ROOT, "profiles", f"k_rb_sweep2d_pc_{tname}_cavity1024_k{kb}_traffic.json")
     if cfg.dim == 3:
         tfile = os.path.join(ROOT, "profiles", "k_rb_zmarch3d_pc_32x8x32_cavity256_traffic.json")
-    if ((cfg.dim == 2 and n == 1024 and tname) or (cfg.dim == 3 and n == 256 and args.p_refine == 0)) \
+    if ((cfg.dim == 2 and n == 1024 and tname) or (cfg.dim == 3 and n == 256 and args.p_refine == 0 and kb == 1)) \
             and os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
@@ -720,25 +751,33 @@ def main():
     iters_to_res = None
     converged = None
     if not args.no_converge:
-        c = incompflow.Solver(cfg)
         log(f"[bench] solving from rest to scaled residual {cfg.tol_residual:g} (budget {args.converge_budget_s:.0f} s)")
-        device_sync()
-        t0 = time.perf_counter()
-        total = 0
-        converged = False
-        while True:
-            rr = c.solve(5000, history=False)
-            total += rr["iterations"]
-            log(f"[bench]   {total} iterations, scaled residual {rr['final_scaled']:.3e}, {time.perf_counter() - t0:.1f} s")
-            if rr["converged"] or rr["status"] != 0:
-                converged = rr["converged"]
-                break
-            if time.perf_counter() - t0 > args.converge_budget_s or total >= cfg.num_outer_iter:
-                break
-        device_sync()
-        ttr = time.perf_counter() - t0
-        iters_to_res = total
-        c.close()
+        ttr, iters_to_res, converged = solve_from_rest(incompflow, cfg, args.converge_budget_s)
+
+    # ---- the same solve without the refinement cycle: what the FP64-defect refinement buys ----
+    refine_ab = None
+    if not args.no_converge and not args.no_refine_ab and cfg.dim == 2 and cfg.p_refine > 0 \
+            and cfg.p_solver == _abi.PSOLVE_SWEEPS:
+        c0 = cfg.copy()
+        c0.p_refine = 0
+        q = c0.copy()
+        q.stop_rule = _abi.STOP_NONE
+        s0 = incompflow.Solver(q)
+        s0.solve(args.warmup, history=False)
+        r0 = s0.solve(args.steps, history=False)
+        s0.close()
+        ms0 = r0["elapsed_ms"] / args.steps
+        log(f"[bench] without refinement: {ms0:.4f} ms/step (device); solving from rest to {c0.tol_residual:g}")
+        t0r, it0, conv0 = solve_from_rest(incompflow, c0, args.converge_budget_s)
+        refine_ab = {"p_refine_1": {"iterations": iters_to_res, "time_to_residual_s": ttr, "converged": converged,
+                                    "ms_per_step_device": r["elapsed_ms"] / args.steps},
+                     "p_refine_0": {"iterations": it0, "time_to_residual_s": t0r, "converged": conv0,
+                                    "ms_per_step_device": ms0},
+                     "residual_target": cfg.tol_residual,
+                     "note": "the same 1024^2 cavity and launch settings from rest, with and without the "
+                             "FP64-defect refinement cycle of the p' solve (20 more p' sweeps on the defect)"}
+        log(f"[bench] refinement A/B to {cfg.tol_residual:g}: with {iters_to_res} iterations {ttr:.2f} s, "
+            f"without {it0} iterations {t0r:.2f} s")
 
     mg = None
     if not args.no_mg and cfg.dim == 2:
@@ -830,7 +869,8 @@ def main():
                      "traffic_gbs": traffic_gbs,
                      "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None,
                      "kernel": ("k_rb_sweep2d<COEF_PCORR> (%d fused red-black sweeps of p' per launch)" % kb
-                                if cfg.dim == 2 else "k_rb_zmarch3d (one fused red-black sweep of p')"),
+                                if cfg.dim == 2 else ("k_rb_zmarch3d_p2 (two red-black sweeps of p' per launch)" if kb == 2
+                                                      else "k_rb_zmarch3d_p (one red-black sweep of p' per launch)")),
                      "avg_launch_us": us, "alg_bytes_per_launch": alg_bytes,
                      "sweep_model_bytes_per_launch": model_bytes,
                      "sweep_model_frac": model_bytes / (us * 1e-6) / (HBM_PEAK_GBS * 1e9),
@@ -847,6 +887,7 @@ def main():
         "residual_target": cfg.tol_residual,
         "iterations_to_residual": iters_to_res,
         "converged": converged,
+        "refine_ab": refine_ab,
         "time_to_1e-5": {"reached": bool(mg and mg["reached_1e-5"]),
                          "time_s": mg["time_to_1e-5_s"] if mg else None,
                          "fp32_floor_observed": mg["fp32_floor_observed"] if mg else None,

@@ -136,9 +136,10 @@ typedef struct ifl_config {
     int32_t check_every;         /* outer iterations between host reads of the stop flag (0 = auto) */
     int32_t tile;                /* 2D sweep tile: 0 auto, 1 64x16, 2 64x8, 3 32x8, 4 32x4,
                                     5 128x32 (1024-thread workgroups) */
-    int32_t sweep_block;         /* red-black sweeps per launch in 2D (1, 2, 4 or 5) of the p' system: a
-                                    stop test met inside a block takes effect at the block's end
-                                    (1 = the reference's per-sweep exit) */
+    int32_t sweep_block;         /* red-black sweeps per launch of the p' system: 1, 2, 4 or 5 in 2D,
+                                    1 or 2 in 3D (larger values run 2); a stop test met inside a
+                                    block takes effect at the block's end (1 = the reference's
+                                    per-sweep exit) */
     /* Energy (temperature) transport.  The reference declares the thermal properties and the
      * temperature BC table (config.h:19-23, 45-63) but never solves for temperature; here the
      * steady equation  div(rho u T) = div((k / c_p) grad T)  is discretised like the momentum
@@ -331,8 +332,8 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
 /* ---- Autotuner (no reference counterpart) ----
  * Picks, by measurement on the handle's own case, the settings that the reference fixes at
  * compile time (config.h:67-70) or that only exist here:
- *   IFL_TUNE_LAUNCH : 2D sweep tile, sweep_block and sweep_block_mom, by the device time of one
- *                     outer iteration;
+ *   IFL_TUNE_LAUNCH : 2D sweep tile, sweep_block and sweep_block_mom (3D: sweep_block 1 or 2),
+ *                     by the device time of one outer iteration;
  *   IFL_TUNE_SWEEPS : sweeps per outer iteration (n_iter of u, v, w and of p'), by the drop of
  *                     the scaled residual per second over trial_iters outer iterations.
  * Trials start from the current state (a case that has run fewer than 20 outer iterations is

@@ -120,11 +120,12 @@ int partStride(const ifl::CaseParams &P) {
 }
 
 
-// Sweeps per launch of a smoother call in 2D red-black (1 otherwise): sweep_block for p' and its
+// Sweeps per launch of a red-black smoother call (1 for Jacobi).  2D: sweep_block for p' and its
 // refinement, sweep_block_mom for the momentum and temperature equations (0: the smaller of
-// sweep_block and 2).
+// sweep_block and 2).  3D: sweep_block, at most 2, for p' and its refinement; 1 for the others.
 int sweepBlockOf(const ifl_config &c, int dim, int slot) {
-    if (dim != 2 || c.smoother != IFL_SMOOTHER_RBGS) return 1;
+    if (c.smoother != IFL_SMOOTHER_RBGS) return 1;
+    if (dim != 2) return (slot == 3 || slot == 4) ? std::min(std::max(c.sweep_block, 1), 2) : 1;
     if (slot < 3 || slot == ifl::kSlotT) {
         const int k = c.sweep_block_mom > 0 ? c.sweep_block_mom : std::min(std::max(c.sweep_block, 1), 2);
         return std::max(k, 1);

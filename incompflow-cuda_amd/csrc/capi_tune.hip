@@ -207,6 +207,18 @@ extern "C" int ifl_autotune(ifl_solver *h, int what, int trial_iters, ifl_tune_r
                         best.Km = Km;
                     }
                 }
+    } else if ((what & IFL_TUNE_LAUNCH) && h->P.dim == 3 && c0.smoother == IFL_SMOOTHER_RBGS) {
+        bestUs = 1e300;              // 3D: one or two p' sweeps per launch
+        Settings x = best;
+        for (int K : {1, 2}) {
+            x.K = K;
+            double us;
+            if ((st = timeIter(x, us)) != IFL_OK) return st;
+            if (us < bestUs) {
+                bestUs = us;
+                best.K = K;
+            }
+        }
     }
     if (what & IFL_TUNE_SWEEPS) {
         const int uvOpts[3] = {5, 10, 20}, pOpts[3] = {10, 20, 40};

@@ -472,6 +472,215 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p(CaseParams P0, BatchMa
     }
 }
 
+// Two red-black sweeps of the p' system per launch, on the same TX x TY x ZC columns as
+// k_rb_zmarch3d_p (so the workgroup counts and partial records are the same).  The field and the
+// +x, +y, +z links are staged over the tile plus a 4-cell halo; at step t the column runs four
+// phases, each behind a barrier:
+//   red1(t+1) on ring <= 3   reads the old black values of planes t..t+2
+//   black1(t) on ring <= 2   reads red1 of planes t-1..t+1
+//   red2(t-1) on ring <= 1   reads black1 of planes t-2..t
+//   black2(t-2) on the tile  reads red2 of planes t-3..t-1, then plane t-2 is written
+// (ring r: the cells within r of the tile in x and y).  Each phase's ring is the set the next one
+// reads, and below and above the column the phases start 3..0 planes early and end 3..0 planes
+// late (red1 on [z0-3, z1+2], black1 on [z0-2, z1+1], red2 on [z0-1, z1], black2 on [z0, z1)), so
+// every value is recomputed from the same inputs its owning column uses and the launch equals two
+// of the reference's red-pass-then-black-pass sweeps bit for bit.  Live planes at step t: the
+// field t-3..t+2 (six slots), +z links t-3..t+1 (five), +x/+y links t-2..t+1 (four).
+template <int TX, int TY, int ZC, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p2(CaseParams P0, BatchMap M, SweepSel e, int launch) {
+    constexpr int G = 4;                                      // staged halo
+    constexpr int HX = TX + 2 * G, HY = TY + 2 * G, PL = HX * HY;
+    constexpr int NPL = (PL + kBlock - 1) / kBlock;
+    constexpr int NC = ((TX + 6) / 2 * (TY + 6) + kBlock - 1) / kBlock;   // cells per thread, largest ring
+    constexpr int FS = 6, TS = 5, ES = 4;
+    static_assert(TX % 2 == 0, "colour rows need an even tile width");
+    __shared__ float win[FS][HY][HX];                         // p', plane z in slot fs(z)
+    __shared__ float lT[TS][HY][HX];                          // links towards +z, slot ts(z)
+    __shared__ float lE[ES][HY][HX], lN[ES][HY][HX];          // towards +x, +y, slot z & 3
+    Blk blk;
+    const CaseParams &P = caseOf<BATCH>(P0, M, MAP_ZMARCH_P, blk);
+    if (caseDone(P)) return;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, N = P.N;
+    const int gxc = (nx + TX - 1) / TX, gyc = (ny + TY - 1) / TY;
+    const int bx = blk.flat % gxc, byz = blk.flat / gxc;
+    const int by = byz % gyc, bz = byz / gyc;
+    float *fld = slotField(P, e.slot), *alt = slotAlt(P, e.slot);
+    const bool odd = ((launch + e.flip) & 1) != 0;
+    const float *in = odd ? alt : fld;
+    float *out = odd ? fld : alt;
+    const float rlx = P.relax[e.q];
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+    const int z1 = min(z0 + ZC, nz);
+    const int tid = threadIdx.x;
+    const int nbytes = N * 4;
+    const __amdgpu_buffer_rsrc_t bIn = bufferOf(in, nbytes), bSrc = bufferOf(slotSource(P, e.slot), nbytes);
+    const __amdgpu_buffer_rsrc_t bE = bufferOf(P.pcoef + (size_t)N * sE, nbytes);
+    const __amdgpu_buffer_rsrc_t bN = bufferOf(P.pcoef + (size_t)N * sN, nbytes);
+    const __amdgpu_buffer_rsrc_t bT = bufferOf(P.pcoef + (size_t)N * sT, nbytes);
+    auto fs = [](int z) { return (z + 12) % FS; };            // z >= -12
+    auto ts = [](int z) { return (z + 10) % TS; };            // z >= -10
+
+    auto loadPlane = [&](__amdgpu_buffer_rsrc_t b, int z, float (&r)[NPL]) {
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int q = tid + m * kBlock;
+            const int ly = q / HX, lx = q - ly * HX;
+            const int gx = x0 - G + lx, gy = y0 - G + ly;
+            const bool ok = (q < PL) & ((unsigned)z < (unsigned)nz) & ((unsigned)gx < (unsigned)nx)
+                            & ((unsigned)gy < (unsigned)ny);
+            r[m] = bufferLoad(b, ok ? 4 * (gx + nx * (gy + ny * z)) : kNoOffset);
+        }
+    };
+    auto storePlane = [&](float *w, const float (&r)[NPL]) {
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int q = tid + m * kBlock;
+            if (q < PL) w[q] = r[m];
+        }
+    };
+    struct Links {
+        float e[NPL], n[NPL], t[NPL];
+    };
+    auto loadLinks = [&](int z, Links &l) {
+        loadPlane(bE, z, l.e);
+        loadPlane(bN, z, l.n);
+        loadPlane(bT, z, l.t);
+    };
+    auto storeLinks = [&](int z, const Links &l) {
+        storePlane(&lE[z & 3][0][0], l.e);
+        storePlane(&lN[z & 3][0][0], l.n);
+        storePlane(&lT[ts(z)][0][0], l.t);
+    };
+    // cell q of colour c (0 red, 1 black: (x + y + z) even / odd) within ring r of the tile in
+    // plane z, row-major over the ring's rectangle; false past the set or off the grid
+    auto cellAt = [&](int r, int c, int z, int q, int &gx, int &gy) {
+        const int w = TX / 2 + r;
+        const int row = q / w, kk = q - row * w;
+        gy = y0 - r + row;
+        const int xs = x0 - r;
+        gx = xs + 2 * kk + ((xs + gy + z + c) & 1);
+        return q < (TY + 2 * r) * w && z >= 0 && z < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    };
+    auto loadSrc = [&](int r, int c, int z, bool on, float (&b)[NC]) {
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            int gx, gy;
+            const bool ok = on && cellAt(r, c, z, tid + m * kBlock, gx, gy);
+            b[m] = bufferLoad(bSrc, ok ? 4 * (gx + nx * (gy + ny * z)) : kNoOffset);
+        }
+    };
+    auto writePlane = [&](int z, const float *from) {
+        for (int q = tid; q < TX * TY; q += kBlock) {
+            const int r = q / TX, xx = q - r * TX;
+            const int gx = x0 + xx, gy = y0 + r;
+            if (gx < nx && gy < ny) {
+                const int g = gx + nx * (gy + ny * z);
+                out[g] = from ? from[g] : win[fs(z)][r + G][xx + G];
+            }
+        }
+    };
+
+    double acc0 = 0.0, acc1 = 0.0;
+    // the arithmetic of k_rb_zmarch3d_p's relax, term for term
+    auto relax = [&](int gx, int gy, int z, bool owned, float b, bool second) {
+        const int lx = gx - x0 + G, ly = gy - y0 + G;
+        const int c4 = z & 3;
+        const float aE = lE[c4][ly][lx], aW = lE[c4][ly][lx - 1];
+        const float aN = lN[c4][ly][lx], aS = lN[c4][ly - 1][lx];
+        const float aT = lT[ts(z)][ly][lx], aB = lT[ts(z - 1)][ly][lx];
+        float aC = -(aE + aW + aN + aS);
+        aC += -(aT + aB);
+        float(*wc)[HX] = win[fs(z)];
+        const float phi_C = wc[ly][lx];
+        const float phi_E = wc[ly][lx + 1], phi_W = wc[ly][lx - 1];
+        const float phi_N = wc[ly + 1][lx], phi_S = wc[ly - 1][lx];
+        const float phi_T = (z != nz - 1) ? win[fs(z + 1)][ly][lx] : 0.0f;
+        const float phi_B = (z != 0) ? win[fs(z - 1)][ly][lx] : 0.0f;
+        float newPhi = b - aE * phi_E - aW * phi_W - aN * phi_N - aS * phi_S;
+        newPhi -= aT * phi_T + aB * phi_B;
+        newPhi /= aC;
+        const float d = rlx * (newPhi - phi_C);
+        wc[ly][lx] = phi_C + d;
+        if (owned) {
+            if (second) acc1 = sqAdd(acc1, d);
+            else acc0 = sqAdd(acc0, d);
+        }
+    };
+    auto phase = [&](int r, int c, int z, bool on, const float (&b)[NC], bool second) {
+        if (!on) return;
+        const bool ownPlane = z >= z0 && z < z1;
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            int gx, gy;
+            if (cellAt(r, c, z, tid + m * kBlock, gx, gy))
+                relax(gx, gy, z, ownPlane && gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY, b[m], second);
+        }
+    };
+
+    // steps t = tA .. z1+1 (none is needed below plane -1); a phase runs when its plane is in its
+    // range: red1 always, black1 from z0-2, red2 from z0-1, black2 from z0
+    const int tA = max(z0 - 4, -1);
+    float pend[NPL];
+    Links pendL;
+    float sR1[NC], sB1[NC], sR2[NC], sB2[NC];
+    {
+        float f0[NPL], f1[NPL], tl[NPL];
+        loadPlane(bIn, tA, f0);
+        loadPlane(bIn, tA + 1, f1);
+        loadPlane(bT, tA, tl);
+        loadPlane(bIn, tA + 2, pend);
+        loadLinks(tA + 1, pendL);
+        loadSrc(3, 0, tA + 1, true, sR1);
+        loadSrc(2, 1, tA, tA >= z0 - 2, sB1);
+        loadSrc(1, 0, tA - 1, tA >= z0, sR2);
+        loadSrc(0, 1, tA - 2, tA >= z0 + 2, sB2);
+        // the stop decision, with these loads in flight
+        if (!launchRuns(P, blk, e.slot, launch, e.kPrev, blk.n, P.tol[e.q])) {
+            for (int z = z0; z < z1; ++z) writePlane(z, in);      // keeps the ping-pong parity
+            return;
+        }
+        storePlane(&win[fs(tA)][0][0], f0);
+        storePlane(&win[fs(tA + 1)][0][0], f1);
+        storePlane(&lT[ts(tA)][0][0], tl);
+    }
+    for (int t = tA; t <= z1 + 1; ++t) {
+        storePlane(&win[fs(t + 2)][0][0], pend);
+        storeLinks(t + 1, pendL);
+        loadPlane(bIn, t + 3, pend);
+        loadLinks(t + 2, pendL);
+        float nR1[NC], nB1[NC], nR2[NC], nB2[NC];
+        loadSrc(3, 0, t + 2, true, nR1);
+        loadSrc(2, 1, t + 1, t + 1 >= z0 - 2, nB1);
+        loadSrc(1, 0, t, t + 1 >= z0, nR2);
+        loadSrc(0, 1, t - 1, t + 1 >= z0 + 2, nB2);
+        __syncthreads();
+        phase(3, 0, t + 1, true, sR1, false);
+        __syncthreads();
+        phase(2, 1, t, t >= z0 - 2, sB1, false);
+        __syncthreads();
+        phase(1, 0, t - 1, t >= z0, sR2, true);
+        __syncthreads();
+        phase(0, 1, t - 2, t >= z0 + 2, sB2, true);
+        __syncthreads();
+        if (t - 2 >= z0) writePlane(t - 2, nullptr);
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            sR1[m] = nR1[m];
+            sB1[m] = nB1[m];
+            sR2[m] = nR2[m];
+            sB2[m] = nB2[m];
+        }
+    }
+    {
+        double v[2] = {acc0, acc1}, tot[2];
+        blockSum<2>(v, tot);
+        if (tid == 0) {
+            sweepPart(P, e.slot, launch, 0)[blk.flat] = tot[0];
+            sweepPart(P, e.slot, launch, 1)[blk.flat] = tot[1];
+        }
+    }
+}
+
 int zmarch_blocks(const CaseParams &P, bool pc) {
     if (pc) return ((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY) * ((P.nz + ZP_ZC - 1) / ZP_ZC);
     return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
@@ -483,9 +692,10 @@ template <int NEQ, bool PC, bool BATCH>
 static void launchZ(const Lc &L, const SweepSel *e, int launch) {
     const CaseParams &P = *L.P;
     const dim3 grid = BATCH ? dim3((unsigned)L.total[PC ? MAP_ZMARCH_P : MAP_ZMARCH]) : dim3((unsigned)zmarch_blocks(P, PC));
-    if constexpr (PC)
-        k_rb_zmarch3d_p<ZP_TX, ZP_TY, ZP_ZC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], launch);
-    else
+    if constexpr (PC) {
+        if (e[0].k >= 2) k_rb_zmarch3d_p2<ZP_TX, ZP_TY, ZP_ZC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], launch);
+        else k_rb_zmarch3d_p<ZP_TX, ZP_TY, ZP_ZC, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], launch);
+    } else
         k_rb_zmarch3d<Z_TX, Z_TY, Z_ZC, NEQ, BATCH><<<grid, kBlock, 0, L.s>>>(P, L.M, e[0], e[NEQ > 1 ? 1 : 0],
                                                                               e[NEQ > 2 ? 2 : 0], launch);
 }

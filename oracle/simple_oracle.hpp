@@ -149,10 +149,12 @@ public:
         stopRule = c.stop_rule;
         tolRes = c.tol_residual;
         pRefine = c.p_refine;
-        // sweeps per launch on the device (2D red-black): sweep_block for p', sweep_block_mom for the
-        // momentum and temperature equations (0: the smaller of sweep_block and 2)
+        // sweeps per launch on the device (red-black): in 2D sweep_block for p', sweep_block_mom for
+        // the momentum and temperature equations (0: the smaller of sweep_block and 2); in 3D
+        // sweep_block, at most 2, for p' and 1 for the others
         const bool blocked = c.dim == 2 && c.smoother == IFL_SMOOTHER_RBGS;
-        sweepBlock = (blocked && c.sweep_block > 1) ? c.sweep_block : 1;
+        const bool rb3 = c.dim == 3 && c.smoother == IFL_SMOOTHER_RBGS;
+        sweepBlock = (blocked && c.sweep_block > 1) ? c.sweep_block : (rb3 && c.sweep_block > 1) ? 2 : 1;
         sweepBlockMom = !blocked ? 1 : c.sweep_block_mom > 0 ? c.sweep_block_mom : std::min(sweepBlock, 2);
         numOuterIter = c.num_outer_iter;
         pSolver = c.p_solver;

@@ -3,7 +3,7 @@ before the trials (bit for bit), and the chosen settings are the ones the solve 
 import numpy as np
 import pytest
 
-from helpers import STATE_FIELDS, assert_bitwise, cavity
+from helpers import STATE_FIELDS, assert_bitwise, cavity, cavity3d
 from incompflow import Solver, _abi
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +59,32 @@ def test_autotune_launch_only_keeps_state_of_running_case():
             assert_bitwise(v, before[f], f"field {f}")
         rep = s.solve(5)
         assert rep["iterations"] == 5
+
+
+def test_autotune_launch_3d_picks_pcorr_sweeps_per_launch(tmp_path):
+    """3D: the launch trials are one and two p' sweeps per launch; the state is kept and the
+    solve continues with the choice, equal to a solve configured with it from the same state."""
+    c = cavity3d(40, 20, 36)
+    c.stop_rule = _abi.STOP_NONE
+    path = str(tmp_path / "s.ifl")
+    with Solver(c) as s:
+        s.solve(25, history=False)
+        before = _fields(s, c)
+        s.save_state(path)
+        r = s.autotune(_abi.TUNE_LAUNCH)
+        assert r["trials"] == 2 and r["tile"] == 0 and r["sweep_block"] in (1, 2)
+        for f, v in _fields(s, c).items():
+            assert_bitwise(v, before[f], f"field {f}")
+        s.solve(5, history=False)
+        got = _fields(s, c)
+    w = c.copy()
+    w.sweep_block = r["sweep_block"]
+    with Solver(w) as s2:
+        s2.load_state(path)
+        s2.solve(5, history=False)
+        want = _fields(s2, w)
+    for f in want:
+        assert_bitwise(got[f], want[f], f"field {f}")
 
 
 def test_converged_fields_do_not_depend_on_the_tuned_settings():

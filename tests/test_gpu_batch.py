@@ -101,6 +101,17 @@ def test_batch_3d():
     _check_batch(cs, 8, "3d")
 
 
+def test_batch_3d_two_pcorr_sweeps_per_launch():
+    """sweep_block 2 in 3D: the batched two-sweep p' launches (several columns and chunks per
+    case, 7 p' sweeps: three two-sweep launches and a one-sweep launch) equal each case's solve."""
+    cs = [cavity3d(40, 20, 36), duct3d(35, 9, 70), cavity3d(20, 17, 33, re=300.0)]
+    for c in cs:
+        c.num_outer_iter = 6
+        c.sweep_block = 2
+        c.n_iter[3] = 7
+    _check_batch(cs, 6, "3d p' block 2")
+
+
 def test_batch_jacobi():
     cs = _mixed_2d(1)[:4]
     for c in cs:

@@ -351,6 +351,47 @@ def test_3d_zmarch_multiple_tiles_and_chunks(shape):
     assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
 
 
+@pytest.mark.parametrize("p_refine", [0, 1])
+@pytest.mark.parametrize("shape,n_p", [((150, 20, 70), 7), ((9, 17, 33), 8), ((40, 9, 5), 5), ((64, 16, 96), 20)])
+def test_3d_two_pcorr_sweeps_per_launch(shape, n_p, p_refine):
+    """sweep_block 2 in 3D: two red-black p' sweeps per launch (k_rb_zmarch3d_p2: 4-cell halo, the
+    column's phases starting up to 3 planes below and ending up to 3 above it) on ragged x/y tiles
+    and z chunks; an odd sweep count ends with a one-sweep launch; the refinement's sweeps too."""
+    cfg = CASES["duct3d"]()
+    cfg.nx, cfg.ny, cfg.nz = shape
+    cfg.n_iter[3] = n_p
+    cfg.sweep_block = 2
+    cfg.p_refine = p_refine
+    _log(f"[3d p' block 2] grid={shape} n_p={n_p} p_refine={p_refine}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(3)
+    ro = o.solve(3)
+    _compare_all(g, o, cfg, f"3d p' block 2 {shape}")
+    assert [tuple(r.sweeps) for r in rg["history"]] == [tuple(r.sweeps) for r in ro["history"]]
+
+
+def test_3d_two_pcorr_sweeps_stop_inside_a_block():
+    """A loose p' tolerance stops the call after a sweep inside a two-sweep launch: the call ends
+    at the end of that launch on both sides (the oracle's blocked rule), over 6 outer iterations
+    of the 3D cavity."""
+    cfg = CASES["cavity3d"]()
+    cfg.nx, cfg.ny, cfg.nz = 70, 20, 40
+    cfg.sweep_block = 2
+    cfg.n_iter[3] = 20
+    cfg.tol[3] = 0.5
+    _log(f"[3d p' block 2 stop] grid={cfg.nx}x{cfg.ny}x{cfg.nz}")
+    g = Solver(cfg)
+    o = Oracle(cfg)
+    rg = g.solve(6)
+    ro = o.solve(6)
+    _compare_all(g, o, cfg, "3d p' block 2 stop")
+    sweeps = [tuple(r.sweeps) for r in rg["history"]]
+    assert sweeps == [tuple(r.sweeps) for r in ro["history"]]
+    assert any(sw[3] < cfg.n_iter[3] for sw in sweeps), sweeps
+    assert all(sw[3] % 2 == 0 for sw in sweeps), sweeps
+
+
 def test_3d_jacobi_parity():
     cfg = CASES["cavity3d"]()
     cfg.smoother = _abi.SMOOTHER_JACOBI

@@ -476,6 +476,38 @@ def test_sweep_block_only_delays_the_stop():
     assert h4.sweeps[1] == 4
 
 
+def test_sweep_block_3d_applies_to_pcorr_only_and_caps_at_two():
+    """3D: sweep_block groups the p' sweeps in twos (4 and 5 also mean two) and leaves the momentum
+    equations at one; with no stop test met the result is the one-sweep result bit for bit, and a
+    met p' test ends the call at the end of its pair."""
+    base = cavity3d(10, 8, 6)
+    base.stop_rule = A.STOP_NONE
+    for f in range(4):
+        base.tol[f] = 0.0
+    runs = []
+    for blk in (1, 2, 5):
+        c = base.copy()
+        c.sweep_block = blk
+        o = Oracle(c)
+        o.solve(3)
+        runs.append([o.get(f) for f in (A.FIELD_U, A.FIELD_V, A.FIELD_W, A.FIELD_P)])
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    loose = base.copy()
+    loose.tol[3] = 0.5
+    seen = {}
+    for blk in (1, 2, 4):
+        c = loose.copy()
+        c.sweep_block = blk
+        seen[blk] = [tuple(h.sweeps) for h in Oracle(c).solve(4)["history"]]
+    assert seen[2] == seen[4]
+    assert all(s[3] % 2 == 0 for s in seen[2]), seen[2]
+    assert any(s1[3] % 2 == 1 for s1 in seen[1]), seen[1]         # a stop after an odd sweep ...
+    assert all(s2[3] == s1[3] + (s1[3] % 2) for s1, s2 in zip(seen[1][:1], seen[2][:1]))  # ... waits for the pair
+    assert [s[1] for s in seen[1][:1]] == [s[1] for s in seen[2][:1]]                  # momentum: not blocked
+
+
 # ---------------------------------------------------------------- FP64 defect, momentum residual,
 # refinement, initFaceVel and invariant I1 (independent checks; each shown to catch a planted slip)
 from oracle.oracle import (FAULT_DEFECT_NO_WEST, FAULT_DEFECT_SIGN, FAULT_INITFACE_RIGHT,  # noqa: E402
