@@ -682,9 +682,9 @@ def main():
     run_cfg.stop_rule = _abi.STOP_NONE
     s = incompflow.Solver(run_cfg)
     launch_tune = None
-    if cfg.dim == 2 and not args.no_tune and not args.tile:
-        # the launch settings (tile, sweeps per launch) by the library's autotuner, before the warmup;
-        # they change the time per iteration only, not the iterates
+    if not args.no_tune and not args.tile and cfg.smoother == _abi.SMOOTHER_RBGS:
+        # the launch settings (2D: tile and sweeps per launch; 3D: p' sweeps per launch) by the
+        # library's autotuner, before the warmup; they change the time per iteration only
         launch_tune = s.autotune(_abi.TUNE_LAUNCH)
         cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = s.cfg.tile, s.cfg.sweep_block, s.cfg.sweep_block_mom
         args.tile, args.sweep_block, args.sweep_block_mom = cfg.tile, cfg.sweep_block, cfg.sweep_block_mom

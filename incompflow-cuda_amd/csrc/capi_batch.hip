@@ -210,6 +210,40 @@ int ifl_batch_case(ifl_batch *b, int i, ifl_solver **out) {
     return IFL_OK;
 }
 
+namespace {
+// Pinned host buffers of the batch readback: two chunk slots, each with every case's state and the
+// iteration records of one chunk, and the event that marks their arrival.  Released after the
+// stream has drained (an error return may leave copies into them in flight).
+struct PinnedChunks {
+    hipStream_t stream = nullptr;
+    struct Slot {
+        ifl::CaseState *state = nullptr;
+        ifl_iter_record *rec = nullptr;
+        hipEvent_t ev = nullptr;
+        std::vector<int> ids;
+        std::vector<long long> first;
+    } slot[2];
+    int init(hipStream_t s, int n, int chunk) {
+        stream = s;
+        for (Slot &S : slot) {
+            IFL_HIP(hipHostMalloc((void **)&S.state, sizeof(ifl::CaseState) * (size_t)n, hipHostMallocDefault));
+            IFL_HIP(hipHostMalloc((void **)&S.rec, sizeof(ifl_iter_record) * (size_t)n * (size_t)chunk,
+                                  hipHostMallocDefault));
+            IFL_HIP(hipEventCreateWithFlags(&S.ev, hipEventDisableTiming));
+        }
+        return IFL_OK;
+    }
+    ~PinnedChunks() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (Slot &S : slot) {
+            if (S.ev) (void)hipEventDestroy(S.ev);
+            if (S.rec) (void)hipHostFree(S.rec);
+            if (S.state) (void)hipHostFree(S.state);
+        }
+    }
+};
+}  // namespace
+
 int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports) {
     if (!b) return fail(IFL_ERR_ARG, "null batch");
     const int n = (int)b->cases.size();
@@ -241,36 +275,89 @@ int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports) {
             pl.L.s = b->stream;
             ifl::launch_init_faces(pl.L);
         }
-    std::vector<ifl::CaseState> s(n);
     std::vector<int> finalDone(n, 0);
     std::vector<double> finalScaled(n, 0.0);
-    std::vector<ifl_iter_record> ring;
+    // Pipelined chunks: chunk j+1 is enqueued before chunk j's states and iteration records are
+    // read, and both go to pinned host memory in stream order right behind chunk j, so the device
+    // never waits for this bookkeeping.  A case that stops during chunk j runs no further iteration
+    // on the device (its workgroups return at once), so the chunk launched ahead of that knowledge
+    // changes nothing of its result.  A running case executes every iteration of a chunk, so the
+    // first record of its next chunk is known before the readback.
+    PinnedChunks pc;
+    if ((st = pc.init(b->stream, n, chunk)) != IFL_OK) return st;
+    std::vector<long long> next(n);          // first iteration of each case's next chunk
+    for (int i = 0; i < n; ++i) next[i] = b->cases[i]->iter;
+    std::vector<char> stopped(n, 0);
     long long launched = 0;
-    while (launched < limit && !live.empty()) {
+    auto enqueueChunk = [&](int slot) -> int {
+        PinnedChunks::Slot &S = pc.slot[slot];
         const int k = (int)std::min<long long>(chunk, limit - launched);
         for (int q = 0; q < k; ++q) IFL_HIP(hipGraphLaunch(b->graph, b->stream));
         launched += k;
-        for (int i : live)
-            IFL_HIP(hipMemcpyAsync(&s[i], b->cases[i]->P.st, sizeof(ifl::CaseState), hipMemcpyDeviceToHost, b->stream));
-        IFL_HIP(hipStreamSynchronize(b->stream));
-        std::vector<int> still;
-        for (int i : live) {
+        S.ids = live;
+        S.first.resize(live.size());
+        for (size_t c = 0; c < live.size(); ++c) {
+            const int i = live[c];
             ifl_solver *h = b->cases[i];
-            const long long a = h->iter, e = s[i].iter;
-            if (e > a) {
-                ring.resize((size_t)(e - a));
-                if ((st = copyHistory(h, b->stream, a, e, ring.data())) != IFL_OK) return st;
-                IFL_HIP(hipStreamSynchronize(b->stream));
-                tally(h, t[i], ring.data(), a, e, s[i].done != 0, 0);
-            }
-            h->iter = e;
-            finalDone[i] = s[i].done;
-            finalScaled[i] = s[i].lastScaled;
-            if (!s[i].done) still.push_back(i);
+            IFL_HIP(hipMemcpyAsync(&S.state[c], h->P.st, sizeof(ifl::CaseState), hipMemcpyDeviceToHost, b->stream));
+            int r = copyHistory(h, b->stream, next[i], next[i] + k, S.rec + c * (size_t)chunk);
+            if (r != IFL_OK) return r;
+            S.first[c] = next[i];
+            next[i] += k;
         }
+        IFL_HIP(hipEventRecord(S.ev, b->stream));
+        return IFL_OK;
+    };
+    auto readChunk = [&](int slot) -> int {
+        PinnedChunks::Slot &S = pc.slot[slot];
+        IFL_HIP(hipEventSynchronize(S.ev));
+        for (size_t c = 0; c < S.ids.size(); ++c) {
+            const int i = S.ids[c];
+            if (stopped[i]) continue;                 // stopped in an earlier chunk
+            ifl_solver *h = b->cases[i];
+            const ifl::CaseState &cs = S.state[c];
+            const long long a = h->iter, e = cs.iter;
+            if (a != S.first[c] || e < a || e - a > chunk)
+                return fail(IFL_ERR_DEVICE, "batch: case " + std::to_string(i) + " ran an unexpected number of iterations");
+            if (e > a) tally(h, t[i], S.rec + c * (size_t)chunk, a, e, cs.done != 0, 0);
+            h->iter = e;
+            finalDone[i] = cs.done;
+            finalScaled[i] = cs.lastScaled;
+            if (cs.done) stopped[i] = 1;
+        }
+        std::vector<int> still;
+        for (int i : live)
+            if (!stopped[i]) still.push_back(i);
         live.swap(still);
-        if (!live.empty() && launched < limit && 2 * live.size() <= b->inGraph.size())
-            if ((st = batchBuild(b, live)) != IFL_OK) return st;
+        return IFL_OK;
+    };
+    int cur = 0;
+    bool pending[2] = {false, false};
+    if (launched < limit && !live.empty()) {
+        if ((st = enqueueChunk(cur)) != IFL_OK) return st;
+        pending[cur] = true;
+    }
+    while (pending[cur]) {
+        const int nx = cur ^ 1;
+        if (launched < limit && !live.empty()) {
+            if ((st = enqueueChunk(nx)) != IFL_OK) return st;
+            pending[nx] = true;
+        }
+        if ((st = readChunk(cur)) != IFL_OK) return st;
+        pending[cur] = false;
+        if (!live.empty() && launched < limit && 2 * live.size() <= b->inGraph.size()) {
+            // shrink the graph to the cases still running, after the chunk in flight has ended
+            if (pending[nx]) {
+                if ((st = readChunk(nx)) != IFL_OK) return st;
+                pending[nx] = false;
+            }
+            if (!live.empty() && launched < limit) {
+                if ((st = batchBuild(b, live)) != IFL_OK) return st;
+                if ((st = enqueueChunk(nx)) != IFL_OK) return st;
+                pending[nx] = true;
+            }
+        }
+        cur = nx;
     }
     IFL_HIP(hipEventRecord(b->ev1, b->stream));
     IFL_HIP(hipEventSynchronize(b->ev1));
