@@ -921,7 +921,9 @@ int ifl_time_kernel(ifl_solver *h, int kernel, int reps, double *avg_us, int *la
             case IFL_KERNEL_MOM_ASSEMBLE: ifl::launch_mom_assemble(L); break;
             case IFL_KERNEL_RHIE_CHOW: ifl::launch_rhie_chow(L); break;
             case IFL_KERNEL_PCORR_ASSEMBLE: ifl::launch_pcorr_assemble(L); break;
-            case IFL_KERNEL_UPDATE: ifl::launch_update(L); break;
+            case IFL_KERNEL_UPDATE:      // with the partner-buffer writes of the outer iteration's launch
+                ifl::launch_update(L, ifl::Finish{}, momentumStartsInPartner(h->cfg, pl.dim));
+                break;
             case IFL_KERNEL_PSOLVE: enqueuePressureCorrection(pl); break;
             default: break;
         }
