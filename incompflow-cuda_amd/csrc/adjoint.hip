@@ -14,15 +14,18 @@ namespace ifl {
 
 namespace {
 
+// rows per thread as the forward cell kernels (P.cellRows: 1 on grids up to 2^19 cells, where a
+// launch is latency bound, kRows above)
 template <class F>
 __device__ __forceinline__ void forCellsA(const CaseParams &P, F &&f) {
     const int rows = P.ny * P.nz;
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * (4 * kRows) + (threadIdx.x >> 6);
+    const int R = P.cellRows;
+    const int r0 = blockIdx.y * (4 * R) + (threadIdx.x >> 6);
 #pragma unroll
     for (int q = 0; q < kRows; ++q) {
         const int r = r0 + 4 * q;
-        if (i < P.nx && r < rows) {
+        if (q < R && i < P.nx && r < rows) {
             const int k = r / P.ny;
             const int j = r - k * P.ny;
             f(i, j, k, i + P.nx * r);
@@ -496,7 +499,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_reduce(AdjBufs A) {
 
 // ---------------- launchers ----------------
 static dim3 adjGrid(const CaseParams &P) {
-    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * kRows - 1) / (4 * kRows)));
+    const int R = P.cellRows;
+    return dim3((unsigned)((P.nx + 63) / 64), (unsigned)((P.ny * P.nz + 4 * R - 1) / (4 * R)));
 }
 
 int adj_blocks(const CaseParams &P) {
