@@ -52,13 +52,14 @@ int adjAlloc(ifl_solver *h) {
     return IFL_OK;
 }
 
-// Tile of the 2D transposed sweeps: the configured one, else 64x8 with two sweeps per launch
-// (the fastest on a 1024x256 channel: 0.199 ms per adjoint iteration against 0.212-0.237 ms for
-// the other shapes), or 32x8 with four when the grid has fewer than 256 such tiles.
+// Tile of the 2D transposed sweeps: the configured one, else 32x16 with four sweeps per launch
+// (the fastest on the 1024x256 channel once the adjoint cell kernels run one row per thread:
+// 0.155 ms per adjoint iteration against 0.170-0.196 ms for the other shapes), or 32x8 with four
+// when the grid has fewer than 256 such tiles.
 int adjShape(const ifl::CaseParams &P, int tile) {
     if (tile > 0) return tile - 1;
-    const long tiles = (long)((P.nx + 63) / 64) * ((P.ny + 7) / 8);
-    return tiles >= 256 ? 2 : 3;
+    const long tiles = (long)((P.nx + 31) / 32) * ((P.ny + 15) / 16);
+    return tiles >= 256 ? 1 : 3;
 }
 
 // n transposed sweeps of one equation from *x (the result ends in *x; *alt is its partner).
