@@ -315,6 +315,18 @@ def adjoint_leg(args, incompflow, _abi):
     c.num_outer_iter = 200000
     c.sweep_block = args.sweep_block_batch
     prof = np.array(_abi.adjoint_inlet_profile(c))
+    launch = None
+    if not args.no_tune:
+        # the forward's launch settings by the autotuner on a scratch handle of the same case (they
+        # change the time per iteration and where a met smoother stop test takes effect, not the
+        # converged state; the solve below starts from rest)
+        t = incompflow.Solver(c)
+        t.set_face_profile(_abi.WEST, prof)
+        launch = t.autotune(_abi.TUNE_LAUNCH)
+        t.close()
+        c.tile, c.sweep_block, c.sweep_block_mom = launch["tile"], launch["sweep_block"], launch["sweep_block_mom"]
+        log(f"[bench] adjoint channel launch settings by the autotuner: tile {c.tile}, sweep_block "
+            f"{c.sweep_block}, sweep_block_mom {c.sweep_block_mom}")
     log(f"[bench] adjoint channel 1024x256: forward to {c.tol_residual:g}")
     s = incompflow.Solver(c)
     s.set_face_profile(_abi.WEST, prof)
@@ -342,6 +354,9 @@ def adjoint_leg(args, incompflow, _abi):
                                   "256x64 is in fp64_error)",
            "forward_iterations": rf["iterations"], "forward_converged": rf["converged"],
            "forward_final_scaled": rf["final_scaled"], "forward_s": tf,
+           "forward_launch_settings": ({"tile": c.tile, "sweep_block": c.sweep_block,
+                                        "sweep_block_mom": c.sweep_block_mom, "from": "IFL_TUNE_LAUNCH"}
+                                       if launch else None),
            "adjoint_iterations": ra["iterations"], "adjoint_converged": ra["converged"], "adjoint_s": ta,
            "adjoint_grad_change": ra["grad_change"],
            "adjoint_over_forward": ta / tf, "adjoint_ms_per_iteration": ra["elapsed_ms"] / max(1, ra["iterations"]),
@@ -684,7 +699,8 @@ def main():
     launch_tune = None
     if not args.no_tune and not args.tile and cfg.smoother == _abi.SMOOTHER_RBGS:
         # the launch settings (2D: tile and sweeps per launch; 3D: p' sweeps per launch) by the
-        # library's autotuner, before the warmup; they change the time per iteration only
+        # library's autotuner, before the warmup; they change the time per iteration (and where a
+        # met smoother stop test takes effect: the end of a block), not the converged state
         launch_tune = s.autotune(_abi.TUNE_LAUNCH)
         cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = s.cfg.tile, s.cfg.sweep_block, s.cfg.sweep_block_mom
         args.tile, args.sweep_block, args.sweep_block_mom = cfg.tile, cfg.sweep_block, cfg.sweep_block_mom
