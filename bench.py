@@ -59,6 +59,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# the CPU oracle's OpenMP threads, one per physical core, kept there (read when the OpenMP runtime
+# starts, so before torch loads it)
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
 try:   # imported before any device work: a first import between the warmup and the timed steps
     import torch  # leaves the GPU idle for seconds, and the timed steps then start at idle clocks
 except ImportError:
@@ -496,7 +501,8 @@ def host_cpu():
     except AttributeError:
         allowed = None
     return {"cpu_model": model, "sockets": sockets, "physical_cores": len(phys) or None, "logical_cpus": logical or None,
-            "cpus_allowed": allowed}
+            "cpus_allowed": allowed, "omp_places": os.environ.get("OMP_PLACES"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
 
 def solve_from_rest(incompflow, cfg, budget_s):
@@ -648,7 +654,7 @@ def main():
     ap.add_argument("--no-channel", action="store_true")
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-steps", type=int, default=20)
-    ap.add_argument("--c3-cpu-iters", type=int, default=2)
+    ap.add_argument("--c3-cpu-iters", type=int, default=50)
     ap.add_argument("--c3-converge", action="store_true",
                     help="solve the 3D cavity from rest towards 1e-5 within --c3-converge-budget-s")
     ap.add_argument("--c3-converge-budget-s", type=float, default=600.0)
