@@ -26,8 +26,9 @@
 namespace ifl {
 
 // Column shapes, measured per launch on the 256^3 cavity (us).  u+v+w sweep: 128x8x32 310,
-// 128x8x16 334, 64x8x32 461, 64x8x16 393, 128x16x16 448, 256x4x32 522, 256x8x16 479: it moves three
-// fields and seven coefficient planes per cell and prefers wide columns.  p' sweep (k_rb_zmarch3d_p,
+// 128x8x16 334, 64x8x32 461, 64x8x16 393, 128x16x16 448, 256x4x32 522, 256x8x16 479, 128x4x32 362
+// (three workgroups per CU instead of two): it moves three fields and seven coefficient planes per
+// cell and prefers wide columns.  p' sweep (k_rb_zmarch3d_p,
 // links staged in LDS): 32x8x32 138, 64x8x43 139, 64x16x32 139, 64x4x32 146, 32x8x64 145,
 // 64x8x32 165, 64x8x64 166, 128x8x32 216 (64x8x32 with per-cell link loads: 149).
 constexpr int Z_TX = 128, Z_TY = 8, Z_ZC = 32;     // momentum, temperature
