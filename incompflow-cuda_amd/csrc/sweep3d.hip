@@ -1,4 +1,5 @@
-// sweep3d.hip - one full red-black Gauss-Seidel sweep per launch for 3D grids.
+// sweep3d.hip - red-black Gauss-Seidel sweeps for 3D grids: one full sweep per launch
+// (k_rb_zmarch3d, k_rb_zmarch3d_p) or, for the p' system, two (k_rb_zmarch3d_p2, further down).
 //
 // A workgroup owns a TX x TY column of the grid over ZC planes [z0, z1).  It marches up the
 // column with a rolling window of four planes of the field in LDS (2-cell halo in x and y) and
