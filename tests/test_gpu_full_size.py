@@ -126,3 +126,29 @@ def test_c4_full_uneven_batch():
         for f in FIELDS_2D:
             assert_bitwise(got[i][f], o.get(f, shaped=False), f"batch case {i} vs oracle {_abi.FIELD_NAMES[f]}")
         o.close()
+
+
+@pytest.mark.parametrize("shape", [(16384, 8192, 1), (512, 512, 512)])
+def test_largest_grids(shape):
+    """2^27 cells (half the largest grid the library accepts, 2^28): every index and byte offset of
+    a plane stays far inside 31 bits; the device equals the oracle after the first iterations."""
+    nx, ny, nz = shape
+    if nz == 1:
+        cfg = _abi.cavity_config(n=nx, p_refine=1, tol_residual=1e-4)
+        cfg.nx, cfg.ny = nx, ny
+        cfg.tile, cfg.sweep_block, cfg.sweep_block_mom = _abi.TILE_128x32, 5, 5
+        iters, fields = 2, FIELDS_2D
+    else:
+        cfg = _abi.cavity3d_config(n=nx)
+        cfg.p_refine = 0
+        iters, fields = 1, FIELDS_3D
+    _log(f"[full size] largest grid {nx}x{ny}x{nz}, {iters} outer iterations")
+    g = Solver(cfg)
+    rg = g.solve(iters)
+    o = Oracle(cfg)
+    ro = o.solve(iters)
+    assert rg["iterations"] == ro["iterations"] == iters
+    _same(g, o, cfg, fields, f"grid {nx}x{ny}x{nz}")
+    np.testing.assert_allclose(_records(rg["history"]), _records(ro["history"]), rtol=1e-11, atol=1e-300)
+    g.close()
+    o.close()
