@@ -440,8 +440,9 @@ int ifl_batch_case(ifl_batch *b, int i, ifl_solver **out);
 int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports);
 /* Autotuner of a batch (no reference counterpart): the tile, sweep_block and sweep_block_mom that
  * every case of the batch shares, by the device time of trial_iters (<= 0: 30) outer iterations of
- * the whole batch per candidate (2D red-black; tiles 64x16, 64x8, 32x8, 32x4 x sweep_block 2, 4, 5
- * x sweep_block_mom 0 or 4).  Every case is restored after each trial, so the batch solves on from
+ * the whole batch per candidate (2D red-black; tiles 128x32, 64x16, 64x8, 32x8, 32x4 x sweep_block
+ * 2, 4, 5 x sweep_block_mom 0, 4 or 5, at most sweep_block).  Every case is restored after each
+ * trial, so the batch solves on from
  * where it was, with the chosen settings; each case still equals its own ifl_solve with the same
  * settings, bit for bit.  On an error return the batch keeps its previous settings. */
 int ifl_batch_autotune(ifl_batch *b, int trial_iters, ifl_tune_result *out);

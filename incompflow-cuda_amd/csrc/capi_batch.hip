@@ -469,9 +469,9 @@ int ifl_batch_autotune(ifl_batch *b, int trial_iters, ifl_tune_result *out) {
     };
     double bestUs = 1e300;
     int bt = b->tile, bK = out->sweep_block, bKm = out->sweep_block_mom;
-    for (int tile : {ifl::TILE_64x16, ifl::TILE_64x8, ifl::TILE_32x8, ifl::TILE_32x4})
+    for (int tile : {ifl::TILE_128x32, ifl::TILE_64x16, ifl::TILE_64x8, ifl::TILE_32x8, ifl::TILE_32x4})
         for (int K : {2, 4, 5})
-            for (int Km : {0, 4}) {
+            for (int Km : {0, 4, 5}) {
                 if (Km > K) continue;
                 apply(tile, K, Km);
                 double us;

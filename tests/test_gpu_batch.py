@@ -49,9 +49,20 @@ def _mixed_2d(tile=3):
     return cs
 
 
-@pytest.mark.parametrize("tile", [1, 3, 4])
+@pytest.mark.parametrize("tile", [1, 3, 4, 5])
 def test_batch_fixed_iterations_equal_single_solves(tile):
     _check_batch(_mixed_2d(tile), 30, f"fixed tile={tile}")
+
+
+def test_batch_128x32_tiles_five_sweep_blocks():
+    """1024-thread 128x32 tiles in a batch of small uneven cases (mostly partial tiles), five p' and
+    five momentum sweeps per launch (odd launch counts: 10/7 sweeps of u/v)."""
+    cs = _mixed_2d(5)
+    for c in cs:
+        c.sweep_block, c.sweep_block_mom = 5, 5
+        c.n_iter[1] = 7
+        c.num_outer_iter = 20
+    _check_batch(cs, 20, "128x32 K=5 Km=5")
 
 
 def test_batch_stop_rules_caps_and_repacking():
@@ -184,8 +195,8 @@ def test_batch_autotune_restores_state_and_applies_its_choice():
         b.solve(20)
         before = [_fields(b.cases[i], cs[i]) for i in range(len(cs))]
         r = b.autotune(12)
-        assert r["trials"] == 20, r         # 4 tiles x (K 2 | K 4, 5 with Km 0, 4)
-        assert r["tile"] in (1, 2, 3, 4) and r["sweep_block"] in (2, 4, 5) and r["sweep_block_mom"] in (0, 4)
+        assert r["trials"] == 30, r         # 5 tiles x (K 2 | K 4 with Km 0, 4 | K 5 with Km 0, 4, 5)
+        assert r["tile"] in (1, 2, 3, 4, 5) and r["sweep_block"] in (2, 4, 5) and r["sweep_block_mom"] in (0, 4, 5)
         for i, c in enumerate(cs):
             assert b.cases[i].cfg.tile == r["tile"] and b.cases[i].cfg.sweep_block == r["sweep_block"]
             for f, v in _fields(b.cases[i], c).items():
