@@ -267,7 +267,11 @@ int ifl_batch_solve(ifl_batch *b, int max_iter, ifl_report *reports) {
     std::vector<int> live(n);
     for (int i = 0; i < n; ++i) live[i] = i;
     if ((st = batchBuild(b, live)) != IFL_OK) return st;
-    const int chunk = b->cfg.check_every > 0 ? std::min(b->cfg.check_every, kHistCap) : 64;
+    // iterations between host reads; the pinned readback holds two chunks of every case's records,
+    // so the chunk shrinks to keep that under 256 MiB (it changes only how soon the host notices)
+    const size_t perIter = 2 * sizeof(ifl_iter_record) * (size_t)n;
+    const int chunkCap = (int)std::max<size_t>(1, ((size_t)256 << 20) / perIter);
+    const int chunk = std::min(b->cfg.check_every > 0 ? std::min(b->cfg.check_every, kHistCap) : 64, chunkCap);
     IFL_HIP(hipEventRecord(b->ev0, b->stream));
     for (ifl_solver *h : b->cases)
         if (h->iter == 0) {                                           // Solver.cu:162-164
