@@ -212,3 +212,31 @@ def test_batch_autotune_restores_state_and_applies_its_choice():
             want = _fields(s, c2)
         for f in want:
             assert_bitwise(got[i][f], want[f], f"tuned case {i} field {f}")
+
+
+def test_batch_case_that_diverges_stops_alone():
+    """One case of a batch blows up (alpha_p = 1.9): it stops on the device at the iteration its own
+    solve reports divergence, the batch returns IFL_ERR_DIVERGED with that case's status, and the
+    other cases run on to their own ends, each equal to its single solve (the readback of a chunk
+    runs one chunk behind the device, check every 8 iterations)."""
+    cs = [cavity(32, 32), cavity(40, 24), channel(48, 16, 30)]
+    cs[1].relax[3] = 1.9
+    for c in cs:
+        c.stop_rule = _abi.STOP_NONE
+        c.num_outer_iter = 300
+        c.check_every = 8
+        c.tile = 3
+    with Batch(cs) as b:
+        st, reps = b.solve(300)
+        got = [_fields(b.cases[i], cs[i]) for i in range(len(cs))]
+    assert st == _abi.IFL_ERR_DIVERGED
+    assert reps[1]["status"] == _abi.IFL_ERR_DIVERGED and reps[1]["iterations"] < 300
+    assert reps[0]["status"] == 0 and reps[0]["iterations"] == 300 and reps[2]["iterations"] == 300
+    for i, c in enumerate(cs):
+        with Solver(c) as s:
+            r = s.solve(300, history=False)
+            assert r["iterations"] == reps[i]["iterations"], i
+            assert r["status"] == reps[i]["status"], i
+            if i != 1:
+                for f, v in _fields(s, c).items():
+                    assert_bitwise(got[i][f], v, f"case {i} field {f}")
