@@ -103,6 +103,19 @@ def test_energy_2d_tiles_and_sweep_blocks(tile, block, mblock):
     _compare(g, o, cfg, f"tile {tile} block {block}")
 
 
+def test_energy_3d_two_pcorr_sweeps_per_launch():
+    """3D with temperature and two p' sweeps per launch (k_rb_zmarch3d_p2 between the momentum and
+    temperature z-marches), several columns and chunks, 5 outer iterations against the oracle."""
+    cfg = energy(duct3d(40, 12, 36), n=5)
+    cfg.sweep_block = 2
+    cfg.n_iter[3] = 9
+    cfg.stop_rule = A.STOP_NONE
+    g, o = Solver(cfg), Oracle(cfg)
+    g.solve(5)
+    o.solve(5)
+    _compare(g, o, cfg, "3d energy, p' sweep_block 2")
+
+
 def test_energy_jacobi():
     cfg = CASES["channel2d"]()
     cfg.smoother = A.SMOOTHER_JACOBI

@@ -100,6 +100,9 @@ def test_3d_energy_jacobi_and_a_side_stream():
     mg3 = duct3d(36, 20, 18)
     mg3.p_solver, mg3.p_refine, mg3.mg_over = A.PSOLVE_MG, 0, 1.4
     _compare("duct 36x20x18, multigrid", mg3, 10)
+    two = duct3d(36, 20, 18)
+    two.sweep_block = 2
+    _compare("duct 36x20x18, two p' sweeps per launch", two, 8)
     _compare("cavity3d 20x17x33 on a side stream", cavity3d(20, 17, 33, re=300.0), 6, stream=torch.cuda.Stream())
     hot = cavity(48, 40)
     hot.energy = 1
