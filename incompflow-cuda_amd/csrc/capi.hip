@@ -73,6 +73,8 @@ void fillParams(ifl_solver *h) {
     // bound (a 512x128 iteration 0.109 -> 0.100 ms, 1024x256 0.151 -> 0.142), four above (1024^2 and
     // 256^3 measured 1-2 % faster with four)
     P.cellRows = P.N <= (1 << 19) ? 1 : ifl::kRows;
+    P.zcM = ifl::zmarch_chunk(P, false);
+    P.zcP = ifl::zmarch_chunk(P, true);
     // constants.h:22-28 in float, exactly as the reference's constexprs
     const float xmin = (float)c.xmin, xmax = (float)c.xmax, ymin = (float)c.ymin, ymax = (float)c.ymax;
     const float zmin = (float)c.zmin, zmax = (float)c.zmax;

@@ -62,6 +62,7 @@ struct CaseParams {
     int dim, nx, ny, nz;
     int N;                   // cells (< 2^28, checked on the host)
     int cellRows;            // rows per thread of the cell kernels (1 or kRows): 64 x 4 cellRows cells per workgroup
+    int zcM, zcP;            // z planes per column chunk of the 3D z-march sweeps (momentum/temperature, p')
     float dx, dy, dz, Ax, Ay, Az, rho, mu;
     int bct[6];
     float bcv[6][3];

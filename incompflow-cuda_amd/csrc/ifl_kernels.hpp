@@ -39,6 +39,7 @@ int tile_count(int nx, int ny, int tile);
 int tile_dimx(int nx, int tile);
 int default_tile(int nx, int ny);
 int zmarch_blocks(const CaseParams &P, bool pc);   // pc: the p' sweep's column shape
+int zmarch_chunk(const CaseParams &P, bool pc);    // z planes per column chunk (sets P.zcM / P.zcP)
 int zmarch_dimx(const CaseParams &P, bool pc);
 int cell_dimx(const CaseParams &P);
 

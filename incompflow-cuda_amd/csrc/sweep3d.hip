@@ -66,8 +66,9 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d(CaseParams P0, BatchMap 
         erlx[q] = P.relax[eqs[q].q];
     }
     const float *coef = slotCoef(P, e0.slot);    // momentum or temperature planes
-    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
-    const int z1 = min(z0 + ZC, nz);
+    const int zc = P.zcM;                    // planes per chunk (at most ZC)
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * zc;
+    const int z1 = min(z0 + zc, nz);
     const int tid = threadIdx.x;
 
     auto wslot = [](int z) { return z & 3; };
@@ -285,8 +286,9 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p(CaseParams P0, BatchMa
     const float *in = odd ? alt : fld;
     float *out = odd ? fld : alt;
     const float rlx = P.relax[e.q];
-    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
-    const int z1 = min(z0 + ZC, nz);
+    const int zc = P.zcP;                    // planes per chunk (at most ZC)
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * zc;
+    const int z1 = min(z0 + zc, nz);
     const int tid = threadIdx.x;
     // raw buffer loads: an offset past the array reads 0, so no branch surrounds a load
     const int nbytes = N * 4;
@@ -511,8 +513,9 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p2(CaseParams P0, BatchM
     const float *in = odd ? alt : fld;
     float *out = odd ? fld : alt;
     const float rlx = P.relax[e.q];
-    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
-    const int z1 = min(z0 + ZC, nz);
+    const int zc = P.zcP;                    // planes per chunk (at most ZC)
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * zc;
+    const int z1 = min(z0 + zc, nz);
     const int tid = threadIdx.x;
     const int nbytes = N * 4;
     const __amdgpu_buffer_rsrc_t bIn = bufferOf(in, nbytes), bSrc = bufferOf(slotSource(P, e.slot), nbytes);
@@ -683,9 +686,23 @@ __global__ __launch_bounds__(kBlock) void k_rb_zmarch3d_p2(CaseParams P0, BatchM
     }
 }
 
+// z planes per column chunk: 32 (Z_ZC, ZP_ZC), halved while a launch would have fewer than 512
+// workgroups (two per CU), down to 4.  A column's march is a chain of dependent plane steps, so on
+// grids with few columns shorter chunks spread it over more CUs; the planes each chunk recomputes
+// at its edges cost less than the idle CUs (64^3: 32 columns of 32 planes took 40.6 us per p'
+// sweep).  256^3 keeps 32.
+int zmarch_chunk(const CaseParams &P, bool pc) {
+    const long long cols = pc ? (long long)((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY)
+                              : (long long)((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY);
+    int zc = pc ? ZP_ZC : Z_ZC;
+    while (zc > 4 && cols * ((P.nz + zc - 1) / zc) < 512) zc /= 2;
+    return zc;
+}
+
 int zmarch_blocks(const CaseParams &P, bool pc) {
-    if (pc) return ((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY) * ((P.nz + ZP_ZC - 1) / ZP_ZC);
-    return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + Z_ZC - 1) / Z_ZC);
+    const int zc = pc ? P.zcP : P.zcM;
+    if (pc) return ((P.nx + ZP_TX - 1) / ZP_TX) * ((P.ny + ZP_TY - 1) / ZP_TY) * ((P.nz + zc - 1) / zc);
+    return ((P.nx + Z_TX - 1) / Z_TX) * ((P.ny + Z_TY - 1) / Z_TY) * ((P.nz + zc - 1) / zc);
 }
 
 int zmarch_dimx(const CaseParams &P, bool pc) { return pc ? (P.nx + ZP_TX - 1) / ZP_TX : (P.nx + Z_TX - 1) / Z_TX; }
